@@ -1,0 +1,134 @@
+"""ctypes binding of libbmi.so (include/bmi.h).
+
+The library is built in-tree (beastgpu_amd/libbmi.so) by ``__graft_entry__.build()``
+or ``make -C beastgpu_amd/csrc``.  There is no fallback: if the library or a
+GPU is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbmi.so")
+
+OK, EINVAL, ENOMEM, EDEVICE, EBUSY, ECANCELED, EUNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
+SPACE_LAGRANGE_C0D1, SPACE_RWG = 1, 2
+OP_HH3D_SINGLELAYER, OP_MW_SINGLELAYER = 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+C64, C32 = 0, 1
+CLASS_NAMES = ("far", "common_vertex", "common_edge", "common_face", "near", "far_fp64")
+
+# every entry point declared in include/bmi.h
+EXPORTS = (
+    "bmi_version", "bmi_last_error", "bmi_options_default",
+    "bmi_context_create", "bmi_context_destroy",
+    "bmi_mesh_create", "bmi_mesh_free",
+    "bmi_space_create", "bmi_space_free",
+    "bmi_plan_create", "bmi_plan_free",
+    "bmi_matrix_create", "bmi_matrix_free", "bmi_matrix_device_ptr",
+    "bmi_assemble", "bmi_plan_stats", "bmi_matrix_to_host",
+)
+
+
+class BMIError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"bmi status {status}: {message}")
+        self.status = status
+
+
+class Operator(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("alpha", ctypes.c_double * 2),
+        ("beta", ctypes.c_double * 2),
+        ("gamma", ctypes.c_double * 2),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("far_test_order", ctypes.c_int32),
+        ("far_trial_order", ctypes.c_int32),
+        ("ss_order", ctypes.c_int32 * 3),
+        ("near_order", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("near_factor", ctypes.c_double),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("classify_ms", ctypes.c_double),
+        ("special_ms", ctypes.c_double),
+        ("far_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("pairs", ctypes.c_int64 * 6),
+        ("far_launches", ctypes.c_int32),
+        ("n_colours_test", ctypes.c_int32),
+        ("n_colours_trial", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {
+            "classify_ms": self.classify_ms,
+            "special_ms": self.special_ms,
+            "far_ms": self.far_ms,
+            "total_ms": self.total_ms,
+            "pairs": {CLASS_NAMES[i]: int(self.pairs[i]) for i in range(6)},
+            "far_launches": int(self.far_launches),
+            "n_colours_test": int(self.n_colours_test),
+            "n_colours_trial": int(self.n_colours_trial),
+        }
+
+
+_lib = None
+vp = ctypes.c_void_p
+pp = ctypes.POINTER(ctypes.c_void_p)
+
+
+def lib():
+    """Load libbmi.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BMIError(EDEVICE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    i32, i64, st = ctypes.c_int32, ctypes.c_int64, ctypes.c_int32
+    L.bmi_version.restype = ctypes.c_char_p
+    L.bmi_last_error.restype = ctypes.c_char_p
+    L.bmi_options_default.argtypes = [ctypes.POINTER(Options)]
+    L.bmi_options_default.restype = None
+    L.bmi_context_create.argtypes = [i32, pp]
+    L.bmi_context_destroy.argtypes = [vp]
+    L.bmi_mesh_create.argtypes = [vp, vp, i64, vp, i64, pp]
+    L.bmi_mesh_free.argtypes = [vp]
+    L.bmi_space_create.argtypes = [vp, i32, vp, vp, i64, pp]
+    L.bmi_space_free.argtypes = [vp]
+    L.bmi_plan_create.argtypes = [vp, ctypes.POINTER(Operator), vp, vp, ctypes.POINTER(Options), pp]
+    L.bmi_plan_free.argtypes = [vp]
+    L.bmi_matrix_create.argtypes = [vp, i64, i64, i32, pp]
+    L.bmi_matrix_free.argtypes = [vp]
+    L.bmi_matrix_device_ptr.argtypes = [vp, pp, ctypes.POINTER(i64)]
+    L.bmi_assemble.argtypes = [vp, vp]
+    L.bmi_plan_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.bmi_matrix_to_host.argtypes = [vp, vp, i32]
+    for name in EXPORTS:
+        f = getattr(L, name)
+        if name not in ("bmi_version", "bmi_last_error", "bmi_options_default"):
+            f.restype = st
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = lib().bmi_last_error()
+        raise BMIError(status, msg.decode() if msg else "")
+
+
+def default_options() -> Options:
+    o = Options()
+    lib().bmi_options_default(ctypes.byref(o))
+    return o

@@ -1,0 +1,210 @@
+"""Host-side mirror of the reference's assembly entry point.
+
+``assemble_gpu(operator, test_functions, trial_functions, config)`` follows
+assemble_gpu (assamble_gpu.jl:25-137): it assembles the dense Galerkin matrix
+of ``operator`` on the device and returns it as a host column-major complex
+matrix, the same value BEAST's ``assemble(op, X, Y)`` returns.  Everything
+runs through libbmi.so; if the library or the GPU is missing it raises.
+
+``Assembler`` keeps mesh, spaces, plan and matrix resident on the device for
+repeated assemblies (benchmarks, serving).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import _bmi
+from .operators import HH3D_SINGLELAYER, MW_SINGLELAYER, Operator
+from .spaces import P1, RWG, Space
+
+
+@dataclasses.dataclass
+class GPUConfiguration:
+    """Options of one assembly; mirrors the reference's GPUConfiguration
+    (utils/configuration.jl:11-26) for the fields that change what is computed
+    or where the result lives.
+
+    writeBackStrategy: "gpu" keeps the matrix on the device and copies it to the
+        host at the end (GpuWriteBackTrueInstance); "cpu" is accepted for
+        compatibility and behaves the same (the device write-back needs no
+        host scatter).
+    floatType / precision: "fp64" (plain) or "mixed" (FP32 far pairs).
+    output: "c64" (ComplexF64) or "c32" (ComplexF32) matrix.
+    """
+
+    writeBackStrategy: str = "gpu"
+    precision: str = "fp64"
+    output: str = "c64"
+    device: int = 0
+    ss_order: tuple = (5, 5, 5)
+    far_test_order: int = 2
+    far_trial_order: int = 3
+    near_order: int = 4
+    near_factor: float = 2.0
+
+    def options(self) -> _bmi.Options:
+        o = _bmi.default_options()
+        o.far_test_order = int(self.far_test_order)
+        o.far_trial_order = int(self.far_trial_order)
+        for i in range(3):
+            o.ss_order[i] = int(self.ss_order[i])
+        o.near_order = int(self.near_order)
+        if self.precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        o.precision = _bmi.PREC_MIXED if self.precision == "mixed" else _bmi.PREC_FP64
+        o.near_factor = float(self.near_factor)
+        return o
+
+    def dtype(self) -> int:
+        if self.output not in ("c64", "c32"):
+            raise ValueError("output must be 'c64' or 'c32'")
+        return _bmi.C32 if self.output == "c32" else _bmi.C64
+
+
+class _Handle:
+    _free = None
+
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+
+    def close(self):
+        if self.h and self.h.value:
+            getattr(_bmi.lib(), self._free)(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context(_Handle):
+    _free = "bmi_context_destroy"
+
+    def __init__(self, device: int = 0):
+        super().__init__()
+        _bmi.check(_bmi.lib().bmi_context_create(int(device), ctypes.byref(self.h)))
+        self.device = device
+
+
+class DeviceMesh(_Handle):
+    _free = "bmi_mesh_free"
+
+    def __init__(self, ctx: Context, mesh):
+        super().__init__()
+        self.ctx = ctx
+        self._xyz = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+        self._tri = np.ascontiguousarray(mesh.faces, dtype=np.int32)
+        _bmi.check(_bmi.lib().bmi_mesh_create(ctx.h, self._xyz.ctypes.data, self._xyz.shape[0],
+                                              self._tri.ctypes.data, self._tri.shape[0], ctypes.byref(self.h)))
+
+
+class DeviceSpace(_Handle):
+    _free = "bmi_space_free"
+
+    def __init__(self, dmesh: DeviceMesh, space: Space):
+        super().__init__()
+        self.dmesh = dmesh
+        self.space = space
+        kind = {P1: _bmi.SPACE_LAGRANGE_C0D1, RWG: _bmi.SPACE_RWG}[space.kind]
+        idx = np.ascontiguousarray(space.idx, dtype=np.int64)
+        cf = np.ascontiguousarray(space.coeff, dtype=np.float64)
+        _bmi.check(_bmi.lib().bmi_space_create(dmesh.h, kind, idx.ctypes.data, cf.ctypes.data, int(space.nfun),
+                                               ctypes.byref(self.h)))
+
+
+def _c_operator(op: Operator) -> _bmi.Operator:
+    o = _bmi.Operator()
+    o.kind = {HH3D_SINGLELAYER: _bmi.OP_HH3D_SINGLELAYER, MW_SINGLELAYER: _bmi.OP_MW_SINGLELAYER}[op.kind]
+    for name in ("alpha", "beta", "gamma"):
+        z = complex(getattr(op, name))
+        getattr(o, name)[0] = z.real
+        getattr(o, name)[1] = z.imag
+    return o
+
+
+class Plan(_Handle):
+    _free = "bmi_plan_free"
+
+    def __init__(self, ctx: Context, op: Operator, dX: DeviceSpace, dY: DeviceSpace, config: GPUConfiguration):
+        super().__init__()
+        self.ctx, self.dX, self.dY = ctx, dX, dY
+        o = config.options()
+        cop = _c_operator(op)
+        _bmi.check(_bmi.lib().bmi_plan_create(ctx.h, ctypes.byref(cop), dX.h, dY.h, ctypes.byref(o),
+                                              ctypes.byref(self.h)))
+
+    def stats(self) -> dict:
+        s = _bmi.Stats()
+        _bmi.check(_bmi.lib().bmi_plan_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+
+class DeviceMatrix(_Handle):
+    _free = "bmi_matrix_free"
+
+    def __init__(self, ctx: Context, rows: int, cols: int, dtype: int = _bmi.C64):
+        super().__init__()
+        self.ctx, self.rows, self.cols, self.dtype = ctx, int(rows), int(cols), dtype
+        _bmi.check(_bmi.lib().bmi_matrix_create(ctx.h, self.rows, self.cols, int(dtype), ctypes.byref(self.h)))
+
+    def device_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        ld = ctypes.c_int64()
+        _bmi.check(_bmi.lib().bmi_matrix_device_ptr(self.h, ctypes.byref(p), ctypes.byref(ld)))
+        return p.value
+
+    def to_host(self, dtype: int | None = None) -> np.ndarray:
+        dtype = self.dtype if dtype is None else dtype
+        out = np.empty((self.rows, self.cols), dtype=np.complex128 if dtype == _bmi.C64 else np.complex64, order="F")
+        _bmi.check(_bmi.lib().bmi_matrix_to_host(self.h, out.ctypes.data, int(dtype)))
+        return out
+
+
+class Assembler:
+    """Resident assembly of one (operator, test space, trial space)."""
+
+    def __init__(self, op: Operator, X: Space, Y: Space, config: GPUConfiguration | None = None,
+                 ctx: Context | None = None):
+        self.config = config or GPUConfiguration()
+        self.ctx = ctx or Context(self.config.device)
+        self.dmx = DeviceMesh(self.ctx, X.mesh)
+        self.dX = DeviceSpace(self.dmx, X)
+        if Y is X:
+            self.dmy, self.dY = self.dmx, self.dX
+        else:
+            self.dmy = self.dmx if Y.mesh is X.mesh else DeviceMesh(self.ctx, Y.mesh)
+            self.dY = DeviceSpace(self.dmy, Y)
+        self.plan = Plan(self.ctx, op, self.dX, self.dY, self.config)
+        self.matrix = DeviceMatrix(self.ctx, X.nfun, Y.nfun, self.config.dtype())
+
+    def assemble(self) -> DeviceMatrix:
+        _bmi.check(_bmi.lib().bmi_assemble(self.plan.h, self.matrix.h))
+        return self.matrix
+
+    def stats(self) -> dict:
+        return self.plan.stats()
+
+    def close(self):
+        for o in (self.matrix, self.plan, self.dY, self.dX, self.dmy, self.dmx):
+            o.close()
+
+
+def assemble_gpu(operator: Operator, test_functions: Space, trial_functions: Space,
+                 config: GPUConfiguration | None = None, writeBackStrategy: str | None = None) -> np.ndarray:
+    """Dense Galerkin matrix of ``operator`` on (test, trial) as a host
+    column-major complex array (ComplexF64 unless config.output == "c32")."""
+    config = dataclasses.replace(config or GPUConfiguration())
+    if writeBackStrategy is not None:
+        if writeBackStrategy not in ("gpu", "cpu"):
+            raise ValueError("writeBackStrategy must be 'gpu' or 'cpu'")
+        config.writeBackStrategy = writeBackStrategy
+    a = Assembler(operator, test_functions, trial_functions, config)
+    try:
+        return a.assemble().to_host()
+    finally:
+        a.close()

@@ -1,0 +1,971 @@
+// gfx950 kernels of the BEM assembly: pair classification, special-pair
+// integrals (Sauter-Schwab, near, FP64-kept far pairs) and the far-pair block
+// kernel with its owner-computes write-back.
+//
+// Reference path replaced (file:line in the reference):
+//   quadrule_determine_type          quadrule_determine.jl:54-77      -> k_classify
+//   sort_data (compaction)           nonMainCaseQuadratures.jl:13-32  -> k_classify pass 1 + scan
+//   test_toll! + SS kernel           pulledback_integrand_gpu.jl:22-228,
+//                                    SauterSchwab.jl:212-298,
+//                                    Integrand_gpu.jl:58-132          -> k_special
+//   doubleQuadRuleMainCaseKernel!    doubleQuadRuleMainCase.jl:60-105 -> k_blocks (phase 1)
+//   store_with_kernel_register!      store_with_kernel.jl:88-97       -> k_blocks (phase 3)
+//   make_complex                     assamble_gpu.jl:141-144          -> not needed: complex is written directly
+#include "bmi_internal.h"
+
+#define DTOL (2.220446049250313e-16 * 1.0e3)
+
+// ------------------------------------------------------------------ math
+
+// 1/sqrt(x) in FP64: hardware estimate plus two Newton steps.
+__device__ __forceinline__ double rsqrt64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * fma(-hx * y, y, 1.5);
+    const double r = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, r, y);
+}
+
+// sin and cos of 2*pi*t (t in turns), FP64.  Reduction to a quarter turn
+// (exact: t - q/4 by Sterbenz), then Taylor polynomials on |x| <= pi/4:
+// sine to x^15, cosine to x^16 (truncation below 5e-17).
+__device__ __forceinline__ void sincos_turns64(double t, double &s, double &c) {
+    const double q = rint(4.0 * t);
+    const double f = fma(q, -0.25, t);
+    const double x = f * 6.283185307179586476925;
+    const double x2 = x * x;
+    double ps = 1.0 / 1307674368000.0;
+    ps = fma(ps, -x2, 1.0 / 6227020800.0);
+    ps = fma(ps, -x2, 1.0 / 39916800.0);
+    ps = fma(ps, -x2, 1.0 / 362880.0);
+    ps = fma(ps, -x2, 1.0 / 5040.0);
+    ps = fma(ps, -x2, 1.0 / 120.0);
+    ps = fma(ps, -x2, 1.0 / 6.0);
+    const double sp = fma(-x * x2, ps, x);
+    double pc = 1.0 / 20922789888000.0;
+    pc = fma(pc, -x2, 1.0 / 87178291200.0);
+    pc = fma(pc, -x2, 1.0 / 479001600.0);
+    pc = fma(pc, -x2, 1.0 / 3628800.0);
+    pc = fma(pc, -x2, 1.0 / 40320.0);
+    pc = fma(pc, -x2, 1.0 / 720.0);
+    pc = fma(pc, -x2, 1.0 / 24.0);
+    pc = fma(pc, -x2, 0.5);
+    const double cp = fma(-x2, pc, 1.0);
+    const int qi = ((int)q) & 3;
+    const double a = (qi & 1) ? cp : sp;
+    const double b = (qi & 1) ? sp : cp;
+    s = (qi & 2) ? -a : a;
+    c = ((qi + 1) & 2) ? -b : b;
+}
+
+// sin and cos of 2*pi*f for |f| <= 1/2, FP32; Taylor to x^9 / x^10 on |x| <= pi/4.
+__device__ __forceinline__ void sincos_turns32(float f, float &s, float &c) {
+    const float q = rintf(4.0f * f);
+    const float g = fmaf(q, -0.25f, f);
+    const float x = g * 6.28318530717958647692f;
+    const float x2 = x * x;
+    float ps = 1.0f / 362880.0f;
+    ps = fmaf(ps, -x2, 1.0f / 5040.0f);
+    ps = fmaf(ps, -x2, 1.0f / 120.0f);
+    ps = fmaf(ps, -x2, 1.0f / 6.0f);
+    const float sp = fmaf(-x * x2, ps, x);
+    float pc = 1.0f / 3628800.0f;
+    pc = fmaf(pc, -x2, 1.0f / 40320.0f);
+    pc = fmaf(pc, -x2, 1.0f / 720.0f);
+    pc = fmaf(pc, -x2, 1.0f / 24.0f);
+    pc = fmaf(pc, -x2, 0.5f);
+    const float cp = fmaf(-x2, pc, 1.0f);
+    const int qi = ((int)q) & 3;
+    const float a = (qi & 1) ? cp : sp;
+    const float b = (qi & 1) ? sp : cp;
+    s = (qi & 2) ? -a : a;
+    c = ((qi + 1) & 2) ? -b : b;
+}
+
+// G * w for one point pair in FP64: h = w4pi * exp(-gamma R) / R, returned as (re, im).
+__device__ __forceinline__ void green64(const OpConst &op, double w4pi, double dx, double dy, double dz,
+                                        double &hr, double &hi) {
+    const double R2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double rinv = rsqrt64(R2);
+    const double R = R2 * rinv;
+    double s, c;
+    sincos_turns64(R * op.k2pi, s, c);
+    double h = w4pi * rinv;
+    if (op.lossy) h *= exp(-op.gamma_re * R);
+    hr = h * c;
+    hi = -h * s;
+}
+
+// ------------------------------------------------- moment-form epilogues
+
+// EFIE on RWG.  With x' = x - c_K, a_i = v_i - c_K (and y', b_j on the trial
+// cell), the reference-weighted sums give
+//   sum w w G (x - v_i).(y - v_j) = Mxy - a_i.My - b_j.Mx + (a_i.b_j) M0
+// and the divergence term 4 M0; the area factors of f_i = (x - v_i)/(2A)
+// cancel against the Jacobians 2A of the point weights.
+struct MomMW {
+    double m0r, m0i, mxr[3], mxi[3], myr[3], myi[3], mxyr, mxyi;
+};
+
+__device__ __forceinline__ void mw_epilogue(const OpConst &op, const MomMW &m, const double *xa, const double *ya,
+                                            double2 loc[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double a0 = xa[3 * i], a1 = xa[3 * i + 1], a2 = xa[3 * i + 2];
+        const double ur0 = m.mxr[0] - a0 * m.m0r, ur1 = m.mxr[1] - a1 * m.m0r, ur2 = m.mxr[2] - a2 * m.m0r;
+        const double ui0 = m.mxi[0] - a0 * m.m0i, ui1 = m.mxi[1] - a1 * m.m0i, ui2 = m.mxi[2] - a2 * m.m0i;
+        const double ar = m.mxyr - (a0 * m.myr[0] + a1 * m.myr[1] + a2 * m.myr[2]);
+        const double ai = m.mxyi - (a0 * m.myi[0] + a1 * m.myi[1] + a2 * m.myi[2]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double b0 = ya[3 * j], b1 = ya[3 * j + 1], b2 = ya[3 * j + 2];
+            const double vr = ar - (b0 * ur0 + b1 * ur1 + b2 * ur2);
+            const double vi = ai - (b0 * ui0 + b1 * ui1 + b2 * ui2);
+            double2 z;
+            z.x = op.alpha_re * vr - op.alpha_im * vi + 4.0 * (op.beta_re * m.m0r - op.beta_im * m.m0i);
+            z.y = op.alpha_re * vi + op.alpha_im * vr + 4.0 * (op.beta_re * m.m0i + op.beta_im * m.m0r);
+            loc[3 * i + j] = z;
+        }
+    }
+}
+
+__device__ __forceinline__ void hh_epilogue(const OpConst &op, const double *pr, const double *pi, double scale,
+                                            double2 loc[9]) {
+    const double ar = op.alpha_re * scale, ai = op.alpha_im * scale;
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        loc[v].x = ar * pr[v] - ai * pi[v];
+        loc[v].y = ar * pi[v] + ai * pr[v];
+    }
+}
+
+// -------------------------------------------------- far pair, FP64 rule
+
+// Local 3x3 matrix of one far pair with the product rule
+// (doubleQuadRuleMainCase.jl:83-101), FP64 throughout.
+//   xp, yp: rule points relative to the cell centroids (BMI_MAX_RULE x 3)
+//   xa, ya: vertices relative to the centroids (3 x 3)
+template <int OP>
+__device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &rule, const double *xp,
+                                          const double *xa, const double *xc, double xs, const double *yp,
+                                          const double *ya, const double *yc, double ys, double2 loc[9]) {
+    const double dcx = xc[0] - yc[0], dcy = xc[1] - yc[1], dcz = xc[2] - yc[2];
+    if (OP == OPK_MW) {
+        MomMW m = {};
+#pragma unroll
+        for (int I = 0; I < BMI_MAX_RULE; ++I) {
+            if (I >= rule.nt) break;
+            const double x0 = xp[3 * I], x1 = xp[3 * I + 1], x2 = xp[3 * I + 2];
+            const double ux = dcx + x0, uy = dcy + x1, uz = dcz + x2;
+            double sr = 0, si = 0, tr0 = 0, tr1 = 0, tr2 = 0, ti0 = 0, ti1 = 0, ti2 = 0;
+#pragma unroll
+            for (int J = 0; J < BMI_MAX_RULE; ++J) {
+                if (J >= rule.nb) break;
+                const double y0 = yp[3 * J], y1 = yp[3 * J + 1], y2 = yp[3 * J + 2];
+                double hr, hi;
+                green64(op, rule.w4pi[I][J], ux - y0, uy - y1, uz - y2, hr, hi);
+                sr += hr;
+                si += hi;
+                tr0 = fma(hr, y0, tr0); tr1 = fma(hr, y1, tr1); tr2 = fma(hr, y2, tr2);
+                ti0 = fma(hi, y0, ti0); ti1 = fma(hi, y1, ti1); ti2 = fma(hi, y2, ti2);
+            }
+            m.m0r += sr; m.m0i += si;
+            m.mxr[0] = fma(x0, sr, m.mxr[0]); m.mxr[1] = fma(x1, sr, m.mxr[1]); m.mxr[2] = fma(x2, sr, m.mxr[2]);
+            m.mxi[0] = fma(x0, si, m.mxi[0]); m.mxi[1] = fma(x1, si, m.mxi[1]); m.mxi[2] = fma(x2, si, m.mxi[2]);
+            m.myr[0] += tr0; m.myr[1] += tr1; m.myr[2] += tr2;
+            m.myi[0] += ti0; m.myi[1] += ti1; m.myi[2] += ti2;
+            m.mxyr += x0 * tr0 + x1 * tr1 + x2 * tr2;
+            m.mxyi += x0 * ti0 + x1 * ti1 + x2 * ti2;
+        }
+        mw_epilogue(op, m, xa, ya, loc);
+    } else {
+        double pr[9] = {}, pi[9] = {};
+#pragma unroll
+        for (int I = 0; I < BMI_MAX_RULE; ++I) {
+            if (I >= rule.nt) break;
+            const double ux = dcx + xp[3 * I], uy = dcy + xp[3 * I + 1], uz = dcz + xp[3 * I + 2];
+            double vr[3] = {}, vi[3] = {};
+#pragma unroll
+            for (int J = 0; J < BMI_MAX_RULE; ++J) {
+                if (J >= rule.nb) break;
+                double hr, hi;
+                green64(op, rule.w4pi[I][J], ux - yp[3 * J], uy - yp[3 * J + 1], uz - yp[3 * J + 2], hr, hi);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    vr[j] = fma(hr, rule.lb[J][j], vr[j]);
+                    vi[j] = fma(hi, rule.lb[J][j], vi[j]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    pr[3 * i + j] = fma(rule.lt[I][i], vr[j], pr[3 * i + j]);
+                    pi[3 * i + j] = fma(rule.lt[I][i], vi[j], pi[3 * i + j]);
+                }
+        }
+        hh_epilogue(op, pr, pi, 4.0 * xs * ys, loc);
+    }
+}
+
+// -------------------------------------------- far pair, mixed precision
+
+// FP32 evaluation with the phase reduced modulo one turn (2 pi) in FP64
+// before the FP32 sin/cos; per-pair sums in FP32, result widened to FP64.
+__device__ __forceinline__ void green32(const OpConst &op, float w4pi, float dx, float dy, float dz, float &hr,
+                                        float &hi) {
+    const float R2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = __builtin_amdgcn_rsqf(R2);
+    const float R = R2 * rinv;
+    const double tt = (double)R * op.k2pi;
+    const float f = (float)(tt - rint(tt));
+    float s, c;
+    sincos_turns32(f, s, c);
+    float h = w4pi * rinv;
+    if (op.lossy) h *= __expf(-(float)op.gamma_re * R);
+    hr = h * c;
+    hi = -h * s;
+}
+
+template <int OP>
+__device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &rule, const double *xp,
+                                           const double *xa, const double *xc, double xs, const double *yp,
+                                           const double *ya, const double *yc, double ys, double2 loc[9]) {
+    const float dcx = (float)(xc[0] - yc[0]), dcy = (float)(xc[1] - yc[1]), dcz = (float)(xc[2] - yc[2]);
+    float ypf[3 * BMI_MAX_RULE];
+#pragma unroll
+    for (int v = 0; v < 3 * BMI_MAX_RULE; ++v) ypf[v] = (float)yp[v];
+    if (OP == OPK_MW) {
+        float m0r = 0, m0i = 0, mxr[3] = {}, mxi[3] = {}, myr[3] = {}, myi[3] = {}, mxyr = 0, mxyi = 0;
+#pragma unroll
+        for (int I = 0; I < BMI_MAX_RULE; ++I) {
+            if (I >= rule.nt) break;
+            const float x0 = (float)xp[3 * I], x1 = (float)xp[3 * I + 1], x2 = (float)xp[3 * I + 2];
+            const float ux = dcx + x0, uy = dcy + x1, uz = dcz + x2;
+            float sr = 0, si = 0, tr0 = 0, tr1 = 0, tr2 = 0, ti0 = 0, ti1 = 0, ti2 = 0;
+#pragma unroll
+            for (int J = 0; J < BMI_MAX_RULE; ++J) {
+                if (J >= rule.nb) break;
+                const float y0 = ypf[3 * J], y1 = ypf[3 * J + 1], y2 = ypf[3 * J + 2];
+                float hr, hi;
+                green32(op, (float)rule.w4pi[I][J], ux - y0, uy - y1, uz - y2, hr, hi);
+                sr += hr;
+                si += hi;
+                tr0 = fmaf(hr, y0, tr0); tr1 = fmaf(hr, y1, tr1); tr2 = fmaf(hr, y2, tr2);
+                ti0 = fmaf(hi, y0, ti0); ti1 = fmaf(hi, y1, ti1); ti2 = fmaf(hi, y2, ti2);
+            }
+            m0r += sr; m0i += si;
+            mxr[0] = fmaf(x0, sr, mxr[0]); mxr[1] = fmaf(x1, sr, mxr[1]); mxr[2] = fmaf(x2, sr, mxr[2]);
+            mxi[0] = fmaf(x0, si, mxi[0]); mxi[1] = fmaf(x1, si, mxi[1]); mxi[2] = fmaf(x2, si, mxi[2]);
+            myr[0] += tr0; myr[1] += tr1; myr[2] += tr2;
+            myi[0] += ti0; myi[1] += ti1; myi[2] += ti2;
+            mxyr += x0 * tr0 + x1 * tr1 + x2 * tr2;
+            mxyi += x0 * ti0 + x1 * ti1 + x2 * ti2;
+        }
+        const float alr = (float)op.alpha_re, ali = (float)op.alpha_im;
+        const float b4r = 4.0f * (float)op.beta_re, b4i = 4.0f * (float)op.beta_im;
+        const float dr = b4r * m0r - b4i * m0i, di = b4r * m0i + b4i * m0r;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float a0 = (float)xa[3 * i], a1 = (float)xa[3 * i + 1], a2 = (float)xa[3 * i + 2];
+            const float ur0 = mxr[0] - a0 * m0r, ur1 = mxr[1] - a1 * m0r, ur2 = mxr[2] - a2 * m0r;
+            const float ui0 = mxi[0] - a0 * m0i, ui1 = mxi[1] - a1 * m0i, ui2 = mxi[2] - a2 * m0i;
+            const float ar = mxyr - (a0 * myr[0] + a1 * myr[1] + a2 * myr[2]);
+            const float ai = mxyi - (a0 * myi[0] + a1 * myi[1] + a2 * myi[2]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float b0 = (float)ya[3 * j], b1 = (float)ya[3 * j + 1], b2 = (float)ya[3 * j + 2];
+                const float vr = ar - (b0 * ur0 + b1 * ur1 + b2 * ur2);
+                const float vi = ai - (b0 * ui0 + b1 * ui1 + b2 * ui2);
+                loc[3 * i + j].x = (double)(alr * vr - ali * vi + dr);
+                loc[3 * i + j].y = (double)(alr * vi + ali * vr + di);
+            }
+        }
+    } else {
+        float pr[9] = {}, pi[9] = {};
+#pragma unroll
+        for (int I = 0; I < BMI_MAX_RULE; ++I) {
+            if (I >= rule.nt) break;
+            const float ux = dcx + (float)xp[3 * I], uy = dcy + (float)xp[3 * I + 1], uz = dcz + (float)xp[3 * I + 2];
+            float vr[3] = {}, vi[3] = {};
+#pragma unroll
+            for (int J = 0; J < BMI_MAX_RULE; ++J) {
+                if (J >= rule.nb) break;
+                float hr, hi;
+                green32(op, (float)rule.w4pi[I][J], ux - ypf[3 * J], uy - ypf[3 * J + 1], uz - ypf[3 * J + 2], hr, hi);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    vr[j] = fmaf(hr, (float)rule.lb[J][j], vr[j]);
+                    vi[j] = fmaf(hi, (float)rule.lb[J][j], vi[j]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    pr[3 * i + j] = fmaf((float)rule.lt[I][i], vr[j], pr[3 * i + j]);
+                    pi[3 * i + j] = fmaf((float)rule.lt[I][i], vi[j], pi[3 * i + j]);
+                }
+        }
+        const float sc = 4.0f * (float)(xs * ys);
+        const float ar = (float)op.alpha_re * sc, ai = (float)op.alpha_im * sc;
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            loc[v].x = (double)(ar * pr[v] - ai * pi[v]);
+            loc[v].y = (double)(ar * pi[v] + ai * pr[v]);
+        }
+    }
+}
+
+// ------------------------------------------------------- classification
+
+// One work-group per (test patch, trial patch) block, one thread per pair.
+// Class as quadrule_determine.jl:54-77: hits = #vertex pairs closer than
+// 1e3 eps; otherwise near iff dmin^2 max(16|gamma|^2, 1/A_trial) < 0.64.
+// The mixed path also keeps far pairs whose centroids are closer than
+// near_factor * max edge in FP64 (class 5).
+// pass 0 counts special pairs per block; pass 1 writes them, in thread order,
+// at the block's offset.
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(ClassifyArgs a, int pass) {
+    __shared__ double tv[BMI_PATCH][9], bv[BMI_PATCH][9];
+    __shared__ double tc[BMI_PATCH][3], bc[BMI_PATCH][3];
+    __shared__ double barea[BMI_PATCH];
+    __shared__ int tt[BMI_PATCH], bt[BMI_PATCH];
+    __shared__ int wsum[BMI_BLOCK_THREADS / 64];
+
+    const int b = blockIdx.x;
+    const int p = b / a.npb, q = b - p * a.npb;
+    const int t = threadIdx.x;
+    if (t < BMI_PATCH) tt[t] = a.X.patch_tri[p * BMI_PATCH + t];
+    else if (t < 2 * BMI_PATCH) bt[t - BMI_PATCH] = a.Y.patch_tri[q * BMI_PATCH + t - BMI_PATCH];
+    __syncthreads();
+    for (int i = t; i < BMI_PATCH * 9; i += BMI_BLOCK_THREADS) {
+        const int c = i / 9, e = i - 9 * (i / 9);
+        const int K = tt[c], L = bt[c];
+        tv[c][e] = K >= 0 ? a.X.vtx[9 * K + e] : 0.0;
+        bv[c][e] = L >= 0 ? a.Y.vtx[9 * L + e] : 0.0;
+    }
+    if (t < BMI_PATCH * 3) {
+        const int c = t / 3, e = t - 3 * (t / 3);
+        tc[c][e] = tt[c] >= 0 ? a.X.cen[3 * tt[c] + e] : 0.0;
+        bc[c][e] = bt[c] >= 0 ? a.Y.cen[3 * bt[c] + e] : 0.0;
+    }
+    if (t < BMI_PATCH) barea[t] = bt[t] >= 0 ? a.Y.area[bt[t]] : 1.0;
+    __syncthreads();
+
+    const int kl = t >> 4, ll = t & 15;
+    const int K = tt[kl], L = bt[ll];
+    int cls = -1;
+    if (K >= 0 && L >= 0) {
+        int hits = 0;
+        double dmin2 = 1.79769313486231570815e308;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double dx = tv[kl][3 * i] - bv[ll][3 * j];
+                const double dy = tv[kl][3 * i + 1] - bv[ll][3 * j + 1];
+                const double dz = tv[kl][3 * i + 2] - bv[ll][3 * j + 2];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                dmin2 = fmin(dmin2, d2);
+                hits += sqrt(d2) < DTOL;
+            }
+        if (hits > 0) {
+            cls = hits > 3 ? 3 : hits;
+        } else {
+            const double m = fmax(a.k2x16, 1.0 / barea[ll]);
+            if (dmin2 * m < 0.2 * 0.2 * 16) {
+                cls = CLS_NEAR;
+            } else {
+                cls = CLS_FAR;
+                if (a.near_d2 >= 0.0) {
+                    const double cx = tc[kl][0] - bc[ll][0], cy = tc[kl][1] - bc[ll][1], cz = tc[kl][2] - bc[ll][2];
+                    if (cx * cx + cy * cy + cz * cz < a.near_d2) cls = CLS_FAR64;
+                }
+            }
+        }
+    }
+    const int special = cls > 0;
+    const unsigned long long mask = __ballot(special);
+    const int lane = t & 63, wv = t >> 6;
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wv] = __popcll(mask);
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
+        base += i < wv ? wsum[i] : 0;
+        total += wsum[i];
+    }
+    if (pass == 0) {
+        if (t == 0) a.blk_count[b] = total;
+    } else if (special) {
+        const int e = a.blk_off[b] + base + before;
+        if (e < a.capacity) {
+            a.spec_K[e] = K;
+            a.spec_L[e] = L;
+            a.spec_cls[e] = (unsigned char)cls;
+            a.spec_loc[e] = (unsigned char)t;
+        }
+    }
+}
+
+// -------------------------------------------------------------- scan
+
+#define SCAN_PER_THREAD 8
+#define SCAN_CHUNK (BMI_BLOCK_THREADS * SCAN_PER_THREAD)
+
+int scan_partials_needed(int n) { return (n + SCAN_CHUNK - 1) / SCAN_CHUNK; }
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int &total) {
+    __shared__ int ws[BMI_BLOCK_THREADS / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wv] = x;
+    __syncthreads();
+    int base = 0;
+    total = 0;
+#pragma unroll
+    for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
+        base += i < wv ? ws[i] : 0;
+        total += ws[i];
+    }
+    __syncthreads();
+    return base + x - v;
+}
+
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_scan_reduce(const int *in, int n, int *partials) {
+    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; ++i) s += (base + i < n) ? in[base + i] : 0;
+    int total;
+    block_exclusive_scan(s, total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_scan_top(int *partials, int np, int *off, int n, int *total_out) {
+    const int per = (np + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
+    const int b0 = threadIdx.x * per;
+    int s = 0;
+    for (int i = 0; i < per; ++i) s += (b0 + i < np) ? partials[b0 + i] : 0;
+    int total;
+    int run = block_exclusive_scan(s, total);
+    for (int i = 0; i < per; ++i) {
+        if (b0 + i < np) {
+            const int v = partials[b0 + i];
+            partials[b0 + i] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 0) {
+        off[n] = total;
+        *total_out = total;
+    }
+}
+
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_scan_apply(const int *in, int n, const int *partials, int *out) {
+    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+    int v[SCAN_PER_THREAD];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    int total;
+    int run = block_exclusive_scan(s, total) + partials[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+}
+
+hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *total, hipStream_t s) {
+    const int np = scan_partials_needed(n);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(BMI_BLOCK_THREADS), 0, s, count, n, partials);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BMI_BLOCK_THREADS), 0, s, partials, np, off, n, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BMI_BLOCK_THREADS), 0, s, count, n, partials, off);
+    return hipGetLastError();
+}
+
+hipError_t launch_classify(const ClassifyArgs &a, int pass, hipStream_t s) {
+    const long long nb = (long long)a.npa * a.npb;
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)nb), dim3(BMI_BLOCK_THREADS), 0, s, a, pass);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- special pairs
+
+// Accumulator of one lane for a special pair (moment form for RWG, 3x3
+// shape products for P1).
+struct SpecAcc {
+    double v[18];
+};
+
+// reference simplex vertices of the permuted charts (pulledback_integrand_gpu.jl:201-202)
+__constant__ double c_vref[3][2] = {{1.0, 0.0}, {0.0, 1.0}, {0.0, 0.0}};
+
+struct CellG {
+    double o[3];   // v3 - centroid
+    double t1[3];  // v1 - v3
+    double t2[3];  // v2 - v3
+};
+
+template <int OP>
+__device__ __forceinline__ void spec_point(const OpConst &op, SpecAcc &acc, const CellG &X, const CellG &Y,
+                                           const double *dc, double bx1, double bx2, double by1, double by2,
+                                           double w) {
+    double x[3], y[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[d] = X.o[d] + X.t1[d] * bx1 + X.t2[d] * bx2;
+        y[d] = Y.o[d] + Y.t1[d] * by1 + Y.t2[d] * by2;
+    }
+    double hr, hi;
+    green64(op, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1], dc[2] + x[2] - y[2], hr, hi);
+    if (OP == OPK_MW) {
+        acc.v[0] += hr;
+        acc.v[1] += hi;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            acc.v[2 + d] = fma(hr, x[d], acc.v[2 + d]);
+            acc.v[5 + d] = fma(hi, x[d], acc.v[5 + d]);
+            acc.v[8 + d] = fma(hr, y[d], acc.v[8 + d]);
+            acc.v[11 + d] = fma(hi, y[d], acc.v[11 + d]);
+        }
+        const double xy = x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+        acc.v[14] = fma(hr, xy, acc.v[14]);
+        acc.v[15] = fma(hi, xy, acc.v[15]);
+    } else {
+        const double f[3] = {bx1, bx2, 1.0 - bx1 - bx2};
+        const double g[3] = {by1, by2, 1.0 - by1 - by2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double fr = hr * f[i], fi = hi * f[i];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                acc.v[2 * (3 * i + j)] = fma(fr, g[j], acc.v[2 * (3 * i + j)]);
+                acc.v[2 * (3 * i + j) + 1] = fma(fi, g[j], acc.v[2 * (3 * i + j) + 1]);
+            }
+        }
+    }
+}
+
+struct RefChart {
+    double v3[2], t1[2], t2[2];
+};
+
+template <int OP>
+__device__ __forceinline__ void ss_region(const OpConst &op, SpecAcc &acc, const CellG &X, const CellG &Y,
+                                          const double *dc, const RefChart &c1, const RefChart &c2, double u1,
+                                          double u2, double v1, double v2, double mul) {
+    // calc_bary (Integrand_gpu.jl:51-55)
+    const double bx1 = c1.v3[0] + c1.t1[0] * u1 + c1.t2[0] * u2;
+    const double bx2 = c1.v3[1] + c1.t1[1] * u1 + c1.t2[1] * u2;
+    const double by1 = c2.v3[0] + c2.t1[0] * v1 + c2.t2[0] * v2;
+    const double by2 = c2.v3[1] + c2.t1[1] * v1 + c2.t2[1] * v2;
+    spec_point<OP>(op, acc, X, Y, dc, bx1, bx2, by1, by2, mul);
+}
+
+// One wave per special pair: Sauter-Schwab (classes 1-3) or the collapsed
+// near rule (class 4).  Lanes stride over quadrature nodes; a butterfly over
+// the wave reduces the accumulators.
+template <int OP>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_special(SpecialArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x) >> 6;
+    const int nwaves = gridDim.x * (BMI_BLOCK_THREADS / 64);
+    const int ns = min(*a.n_special, a.capacity);
+    for (int e = wave; e < ns; e += nwaves) {
+        const int cls = a.spec_cls[e];
+        if (cls < CLS_CV || cls > CLS_NEAR) continue;
+        const int K = a.spec_K[e], L = a.spec_L[e];
+        double tvx[9], svx[9], tcn[3], scn[3];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            tvx[i] = a.X.vtx[9 * K + i];
+            svx[i] = a.Y.vtx[9 * L + i];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            tcn[d] = a.X.cen[3 * K + d];
+            scn[d] = a.Y.cen[3 * L + d];
+        }
+        CellG X, Y;
+        double dc[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            X.o[d] = tvx[6 + d] - tcn[d];
+            X.t1[d] = tvx[d] - tvx[6 + d];
+            X.t2[d] = tvx[3 + d] - tvx[6 + d];
+            Y.o[d] = svx[6 + d] - scn[d];
+            Y.t1[d] = svx[d] - svx[6 + d];
+            Y.t2[d] = svx[3 + d] - svx[6 + d];
+            dc[d] = tcn[d] - scn[d];
+        }
+        SpecAcc acc;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) acc.v[i] = 0.0;
+
+        if (cls == CLS_NEAR) {
+            const int np = a.near_np, tot = np * np;
+            for (int idx = lane; idx < tot; idx += 64) {
+                const int I = idx / np, J = idx - np * (idx / np);
+                spec_point<OP>(a.op, acc, X, Y, dc, a.nb1[I], a.nb2[I], a.nb1[J], a.nb2[J], a.nw[I] * a.nw[J]);
+            }
+        } else {
+            // vertex correspondence (pulledback_integrand_gpu.jl:22-155)
+            int mi[3] = {0, 0, 0}, mj[3] = {0, 0, 0}, ne = 0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double dx = tvx[3 * i] - svx[3 * j], dy = tvx[3 * i + 1] - svx[3 * j + 1],
+                                 dz = tvx[3 * i + 2] - svx[3 * j + 2];
+                    if (sqrt(dx * dx + dy * dy + dz * dz) < DTOL && ne < 3) {
+                        mi[ne] = i;
+                        mj[ne] = j;
+                        ++ne;
+                    }
+                }
+            int PT[3], PS[3];
+            if (cls == CLS_CF) {
+                PT[0] = 0; PT[1] = 1; PT[2] = 2;
+                PS[0] = PS[1] = PS[2] = 0;
+                for (int k = 0; k < ne; ++k) PS[mi[k]] = mj[k];
+            } else if (cls == CLS_CE) {
+                // common edge between reference vertices 1 and 3
+                PT[0] = mi[1]; PT[1] = 3 - mi[0] - mi[1]; PT[2] = mi[0];
+                PS[0] = mj[1]; PS[1] = 3 - mj[0] - mj[1]; PS[2] = mj[0];
+            } else {
+                // common vertex at reference vertex 1
+                PT[0] = mi[0]; PT[1] = mi[0] == 0 ? 1 : 0; PT[2] = 3 - PT[0] - PT[1];
+                PS[0] = mj[0]; PS[1] = mj[0] == 0 ? 1 : 0; PS[2] = 3 - PS[0] - PS[1];
+            }
+            RefChart c1, c2;
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                c1.v3[d] = c_vref[PT[2]][d];
+                c1.t1[d] = c_vref[PT[0]][d] - c_vref[PT[2]][d];
+                c1.t2[d] = c_vref[PT[1]][d] - c_vref[PT[2]][d];
+                c2.v3[d] = c_vref[PS[2]][d];
+                c2.t1[d] = c_vref[PS[0]][d] - c_vref[PS[2]][d];
+                c2.t2[d] = c_vref[PS[1]][d] - c_vref[PS[2]][d];
+            }
+            const int g = cls - 1;
+            const int n = a.ss_n[g];
+            const int n2 = n * n, n4 = n2 * n2;
+            for (int idx = lane; idx < n4; idx += 64) {
+                const int i1 = idx / (n2 * n), r1 = idx - i1 * n2 * n;
+                const int i2 = r1 / n2, r2 = r1 - i2 * n2;
+                const int i3 = r2 / n, i4 = r2 - i3 * n;
+                const double e1 = a.glx[g][i1], e2 = a.glx[g][i2], e3 = a.glx[g][i3], xi = a.glx[g][i4];
+                const double w = a.glw[g][i1] * a.glw[g][i2] * a.glw[g][i3] * a.glw[g][i4];
+                const double xi3 = xi * xi * xi;
+                if (cls == CLS_CV) {
+                    // SauterSchwab.jl:265-272
+                    const double xe1 = xi * e1, xe2 = xi * e2;
+                    const double m = w * xi3 * e2;
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, xe1, 1 - xe2, xe2 * e3, m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xe2, xe2 * e3, 1 - xi, xe1, m);
+                } else if (cls == CLS_CE) {
+                    // SauterSchwab.jl:274-288
+                    const double xe1 = xi * e1, e12 = e1 * e2, e23 = e2 * e3, e123 = e12 * e3;
+                    const double m = w * xi3 * e1 * e1 * e2;
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, xe1, 1 - xi * (1 - e123), xe1 * e2 * (1 - e3), m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi * (1 - e12), xe1 * (1 - e2), 1 - xi, xe1 * e23, m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi * (1 - e123), xe1 * e2 * (1 - e3), 1 - xi, xe1, m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi * (1 - e123), xe1 * (1 - e23), 1 - xi, xe1 * e2, m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, xe1 * e3, 1 - xi * (1 - e12), xe1 * (1 - e2),
+                                  w * xi3 * e1 * e1);
+                } else {
+                    // SauterSchwab.jl:290-298
+                    const double m = w * xi3 * e1 * e1 * e2;
+                    const double p = xi * e1;
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, xi - p + p * e2, 1 - (xi - p * e2 * e3), xi - p, m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - (xi - p * e2 * e3), xi - p, 1 - xi, xi - p + p * e2, m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, p * (1 - e2 + e2 * e3), 1 - (xi - p * e2), p * (1 - e2), m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - (xi - p * e2), p * (1 - e2), 1 - xi, p * (1 - e2 + e2 * e3), m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - (xi - p * e2 * e3), p * (1 - e2 * e3), 1 - xi, p * (1 - e2), m);
+                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, p * (1 - e2), 1 - (xi - p * e2 * e3), p * (1 - e2 * e3), m);
+                }
+            }
+        }
+        // butterfly over the 64 lanes
+#pragma unroll
+        for (int i = 0; i < 18; ++i) {
+            double v = acc.v[i];
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+            acc.v[i] = v;
+        }
+        if (lane == 0) {
+            double2 loc[9];
+            if (OP == OPK_MW) {
+                MomMW m;
+                m.m0r = acc.v[0]; m.m0i = acc.v[1];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    m.mxr[d] = acc.v[2 + d]; m.mxi[d] = acc.v[5 + d];
+                    m.myr[d] = acc.v[8 + d]; m.myi[d] = acc.v[11 + d];
+                }
+                m.mxyr = acc.v[14]; m.mxyi = acc.v[15];
+                double xa[9], ya[9];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        xa[3 * i + d] = tvx[3 * i + d] - tcn[d];
+                        ya[3 * i + d] = svx[3 * i + d] - scn[d];
+                    }
+                mw_epilogue(a.op, m, xa, ya, loc);
+            } else {
+                double pr[9], pi[9];
+#pragma unroll
+                for (int v = 0; v < 9; ++v) {
+                    pr[v] = acc.v[2 * v];
+                    pi[v] = acc.v[2 * v + 1];
+                }
+                hh_epilogue(a.op, pr, pi, 4.0 * a.X.area[K] * a.Y.area[L], loc);
+            }
+#pragma unroll
+            for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
+        }
+    }
+}
+
+// One thread per far pair that the mixed path keeps in FP64 (class 5).
+template <int OP>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(SpecialArgs a) {
+    const int e = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
+    if (e >= min(*a.n_special, a.capacity)) return;
+    if (a.spec_cls[e] != CLS_FAR64) return;
+    const int K = a.spec_K[e], L = a.spec_L[e];
+    double xp[3 * BMI_MAX_RULE], yp[3 * BMI_MAX_RULE], xa[9], ya[9], xc[3], yc[3];
+#pragma unroll
+    for (int v = 0; v < 3 * BMI_MAX_RULE; ++v) {
+        xp[v] = a.X.pts[3 * BMI_MAX_RULE * (long long)K + v];
+        yp[v] = a.Y.pts[3 * BMI_MAX_RULE * (long long)L + v];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        xc[d] = a.X.cen[3 * K + d];
+        yc[d] = a.Y.cen[3 * L + d];
+    }
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+        xa[v] = a.X.vtx[9 * K + v] - xc[v % 3];
+        ya[v] = a.Y.vtx[9 * L + v] - yc[v % 3];
+    }
+    double2 loc[9];
+    pair_fp64<OP>(a.op, a.rule, xp, xa, xc, a.X.area[K], yp, ya, yc, a.Y.area[L], loc);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
+}
+
+hipError_t launch_special(const SpecialArgs &a, int capacity, int mixed, int op_kind, hipStream_t s) {
+    if (capacity <= 0) return hipSuccess;
+    int waves_wg = BMI_BLOCK_THREADS / 64;
+    int grid = (capacity + waves_wg - 1) / waves_wg;
+    if (grid > 16384) grid = 16384;
+    if (op_kind == OPK_MW) hipLaunchKernelGGL(k_special<OPK_MW>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+    else hipLaunchKernelGGL(k_special<OPK_HH3D>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+    if (mixed) {
+        const int g2 = (capacity + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
+        if (op_kind == OPK_MW) hipLaunchKernelGGL(k_far64<OPK_MW>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
+        else hipLaunchKernelGGL(k_far64<OPK_HH3D>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ far-pair blocks
+
+__device__ __forceinline__ void out_write(double2 *Z, long long off, double re, double im, bool first) {
+    double2 v;
+    if (first) {
+        v.x = re;
+        v.y = im;
+    } else {
+        v = Z[off];
+        v.x += re;
+        v.y += im;
+    }
+    Z[off] = v;
+}
+
+__device__ __forceinline__ void out_write(float2 *Z, long long off, double re, double im, bool first) {
+    float2 v;
+    if (first) {
+        v.x = (float)re;
+        v.y = (float)im;
+    } else {
+        v = Z[off];
+        v.x = (float)((double)v.x + re);
+        v.y = (float)((double)v.y + im);
+    }
+    Z[off] = v;
+}
+
+// One work-group per (test patch p, trial patch q) block of 16 x 16 cell
+// pairs.  Launches go colour pair by colour pair: within one launch no two
+// work-groups share an output entry, so the write-back needs no atomics.
+//   phase 1: thread (kl, ll) integrates cell pair (p[kl], q[ll]) -> LDS
+//   phase 2: special pairs of the block (singular, near, FP64-kept) overwrite
+//            their slots with the values computed by k_special / k_far64
+//   phase 3: each output entry (row of p, column of q) gathers its slot
+//            contributions in a fixed order and is stored, or added to what
+//            an earlier launch stored (first-touch flags decide).
+template <int OP, int MIXED, typename OutT>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_blocks(BlockArgs a) {
+    __shared__ double2 V[9 * BMI_BLOCK_THREADS];  // V[v * 256 + kl * 16 + ll]
+    __shared__ double sxp[BMI_PATCH][3 * BMI_MAX_RULE], syp[BMI_PATCH][3 * BMI_MAX_RULE];
+    __shared__ double sxa[BMI_PATCH][9], sya[BMI_PATCH][9];
+    __shared__ double sxc[BMI_PATCH][3], syc[BMI_PATCH][3];
+    __shared__ double sxs[BMI_PATCH], sys[BMI_PATCH];
+    __shared__ int stt[BMI_PATCH], sbt[BMI_PATCH];
+    __shared__ int s_rfn[BMI_MAX_ROWS], s_cfn[BMI_MAX_ROWS];
+    __shared__ unsigned char s_rfirst[BMI_MAX_ROWS], s_cfirst[BMI_MAX_ROWS];
+    __shared__ int s_rs[BMI_MAX_ROWS + 1], s_cs[BMI_MAX_ROWS + 1];
+    __shared__ unsigned char s_rkl[BMI_MAX_ROWS], s_ckl[BMI_MAX_ROWS];
+    __shared__ double s_rcf[BMI_MAX_ROWS], s_ccf[BMI_MAX_ROWS];
+
+    const int t = threadIdx.x;
+    const int w = blockIdx.x;
+    const int wp = w / a.nbq;
+    const int p = a.tp_list[wp];
+    const int q = a.bq_list[w - wp * a.nbq];
+
+    const int r0 = a.X.row_ptr[p], nr = a.X.row_ptr[p + 1] - r0;
+    const int c0 = a.Y.row_ptr[q], nc = a.Y.row_ptr[q + 1] - c0;
+    const int rs0 = a.X.row_slot_ptr[r0], nrs = a.X.row_slot_ptr[r0 + nr] - rs0;
+    const int cs0 = a.Y.row_slot_ptr[c0], ncs = a.Y.row_slot_ptr[c0 + nc] - cs0;
+
+    if (t < BMI_PATCH) stt[t] = a.X.patch_tri[p * BMI_PATCH + t];
+    else if (t < 2 * BMI_PATCH) sbt[t - BMI_PATCH] = a.Y.patch_tri[q * BMI_PATCH + t - BMI_PATCH];
+    if (t < nr) {
+        s_rfn[t] = a.X.row_fn[r0 + t];
+        s_rfirst[t] = a.X.row_first[r0 + t];
+        s_rs[t] = a.X.row_slot_ptr[r0 + t] - rs0;
+    }
+    if (t == 0) s_rs[nr] = nrs;
+    if (t < nc) {
+        s_cfn[t] = a.Y.row_fn[c0 + t];
+        s_cfirst[t] = a.Y.row_first[c0 + t];
+        s_cs[t] = a.Y.row_slot_ptr[c0 + t] - cs0;
+    }
+    if (t == 0) s_cs[nc] = ncs;
+    if (t < nrs) {
+        s_rkl[t] = a.X.slot_kl[rs0 + t];
+        s_rcf[t] = a.X.slot_cf[rs0 + t];
+    }
+    if (t < ncs) {
+        s_ckl[t] = a.Y.slot_kl[cs0 + t];
+        s_ccf[t] = a.Y.slot_cf[cs0 + t];
+    }
+    __syncthreads();
+    for (int i = t; i < BMI_PATCH * 3 * BMI_MAX_RULE; i += BMI_BLOCK_THREADS) {
+        const int c = i / (3 * BMI_MAX_RULE), e = i - c * (3 * BMI_MAX_RULE);
+        const int K = stt[c], L = sbt[c];
+        sxp[c][e] = K >= 0 ? a.X.pts[3 * BMI_MAX_RULE * (long long)K + e] : 0.0;
+        syp[c][e] = L >= 0 ? a.Y.pts[3 * BMI_MAX_RULE * (long long)L + e] : 0.0;
+    }
+    if (t < BMI_PATCH * 3) {
+        const int c = t / 3, e = t - 3 * (t / 3);
+        const int K = stt[c], L = sbt[c];
+        sxc[c][e] = K >= 0 ? a.X.cen[3 * K + e] : 0.0;
+        syc[c][e] = L >= 0 ? a.Y.cen[3 * L + e] : 0.0;
+    }
+    if (t < BMI_PATCH) {
+        sxs[t] = stt[t] >= 0 ? a.X.area[stt[t]] : 0.0;
+        sys[t] = sbt[t] >= 0 ? a.Y.area[sbt[t]] : 0.0;
+    }
+    __syncthreads();
+    if (OP == OPK_MW && t < BMI_PATCH * 9) {
+        const int c = t / 9, e = t - 9 * (t / 9);
+        const int K = stt[c], L = sbt[c];
+        sxa[c][e] = K >= 0 ? a.X.vtx[9 * K + e] - sxc[c][e % 3] : 0.0;
+        sya[c][e] = L >= 0 ? a.Y.vtx[9 * L + e] - syc[c][e % 3] : 0.0;
+    }
+    __syncthreads();
+
+    // phase 1
+    {
+        const int kl = t >> 4, ll = t & 15;
+        double2 loc[9];
+        if (stt[kl] >= 0 && sbt[ll] >= 0) {
+            if (MIXED)
+                pair_mixed<OP>(a.op, a.rule, sxp[kl], sxa[kl], sxc[kl], sxs[kl], syp[ll], sya[ll], syc[ll], sys[ll], loc);
+            else
+                pair_fp64<OP>(a.op, a.rule, sxp[kl], sxa[kl], sxc[kl], sxs[kl], syp[ll], sya[ll], syc[ll], sys[ll], loc);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) V[v * BMI_BLOCK_THREADS + t] = loc[v];
+    }
+    __syncthreads();
+
+    // phase 2
+    {
+        const int b = p * a.npb + q;
+        const int e0 = a.blk_off[b], e1 = min(a.blk_off[b + 1], a.capacity);
+        for (int e = e0 + t; e < e1; e += BMI_BLOCK_THREADS) {
+            const int loc = a.spec_loc[e];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) V[v * BMI_BLOCK_THREADS + loc] = a.spec_val[9 * (long long)e + v];
+        }
+    }
+    __syncthreads();
+
+    // phase 3
+    OutT *Z = (OutT *)a.Z;
+    const int ne = nr * nc;
+    for (int e = t; e < ne; e += BMI_BLOCK_THREADS) {
+        const int c = e / nr, r = e - c * nr;
+        double sr = 0.0, si = 0.0;
+        for (int u = s_rs[r]; u < s_rs[r + 1]; ++u) {
+            const int ki = s_rkl[u];
+            const int k = ki / 3, i = ki - 3 * k;
+            const double cf = s_rcf[u];
+            for (int v = s_cs[c]; v < s_cs[c + 1]; ++v) {
+                const int lj = s_ckl[v];
+                const int l = lj / 3, j = lj - 3 * l;
+                const double cc = cf * s_ccf[v];
+                const double2 z = V[(3 * i + j) * BMI_BLOCK_THREADS + k * BMI_PATCH + l];
+                sr = fma(cc, z.x, sr);
+                si = fma(cc, z.y, si);
+            }
+        }
+        const long long off = (long long)s_rfn[r] + (long long)s_cfn[c] * a.ld;
+        out_write(Z, off, sr, si, s_rfirst[r] && s_cfirst[c]);
+    }
+}
+
+template <int OP, int MIXED>
+static void launch_blocks_t(const BlockArgs &a, int ntp, int c32, hipStream_t s) {
+    const unsigned grid = (unsigned)((long long)ntp * a.nbq);
+    if (c32)
+        hipLaunchKernelGGL((k_blocks<OP, MIXED, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_blocks<OP, MIXED, double2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+}
+
+hipError_t launch_blocks(const BlockArgs &a, int ntp, int op_kind, int mixed, int c32, hipStream_t s) {
+    if (ntp <= 0 || a.nbq <= 0) return hipSuccess;
+    if (op_kind == OPK_MW) {
+        if (mixed) launch_blocks_t<OPK_MW, 1>(a, ntp, c32, s);
+        else launch_blocks_t<OPK_MW, 0>(a, ntp, c32, s);
+    } else {
+        if (mixed) launch_blocks_t<OPK_HH3D, 1>(a, ntp, c32, s);
+        else launch_blocks_t<OPK_HH3D, 0>(a, ntp, c32, s);
+    }
+    return hipGetLastError();
+}

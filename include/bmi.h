@@ -1,0 +1,162 @@
+/*
+ * bmi.h -- C ABI of the MI355X BEM matrix-assembly library (libbmi.so).
+ *
+ * Drop-in target: BEAST's assemble(op, test_space, trial_space) as replaced by
+ * the reference's assemble_gpu (assamble_gpu.jl:25-137).  A Julia caller
+ * reaches these entry points through ccall; tests and benchmarks use ctypes.
+ *
+ * Conventions
+ *   - Arrays are plain host pointers with explicit sizes; no language types
+ *     cross the boundary.  Indices are 0-based (a Julia shim subtracts 1).
+ *   - Matrices are column-major (row index fastest), as Julia's Matrix and the
+ *     reference's store index mapping (store_with_kernel.jl:94).
+ *   - Complex values are interleaved (re, im) pairs.
+ *   - The library owns every device object; the caller owns host buffers.
+ *   - No exception or longjmp crosses the ABI; every call returns a status and
+ *     bmi_last_error() gives a thread-local message for the last failure.
+ *   - A context is externally synchronised: one thread at a time per context.
+ */
+#ifndef BMI_H
+#define BMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t bmi_status;
+#define BMI_OK 0
+#define BMI_EINVAL (-1)
+#define BMI_ENOMEM (-2)
+#define BMI_EDEVICE (-3)
+#define BMI_EBUSY (-4)
+#define BMI_ECANCELED (-5)
+#define BMI_EUNSUPPORTED (-6)
+
+/* function spaces (BEAST lagrangec0d1 / raviartthomas) */
+#define BMI_SPACE_LAGRANGE_C0D1 1
+#define BMI_SPACE_RWG 2
+
+/* operators: Helmholtz3D.singlelayer, Maxwell3D.singlelayer */
+#define BMI_OP_HH3D_SINGLELAYER 1
+#define BMI_OP_MW_SINGLELAYER 2
+
+/* arithmetic of the far-pair double quadrature */
+#define BMI_PREC_FP64 0
+#define BMI_PREC_MIXED 1
+
+/* matrix element types */
+#define BMI_C64 0 /* ComplexF64 */
+#define BMI_C32 1 /* ComplexF32 */
+
+/* pair classes, in the order of quadrule_determine.jl:76 (0 far, 1 common
+ * vertex, 2 common edge, 3 common face, 4 near); 5 = far pair kept in FP64
+ * by the mixed-precision path */
+#define BMI_CLASS_FAR 0
+#define BMI_CLASS_CV 1
+#define BMI_CLASS_CE 2
+#define BMI_CLASS_CF 3
+#define BMI_CLASS_NEAR 4
+#define BMI_CLASS_FAR64 5
+
+typedef struct bmi_context bmi_context;
+typedef struct bmi_mesh bmi_mesh;
+typedef struct bmi_space bmi_space;
+typedef struct bmi_plan bmi_plan;
+typedef struct bmi_matrix bmi_matrix;
+
+/* Operator coefficients: Z_mn = alpha <f_m, G f_n> + beta <div f_m, G div f_n>,
+ * G = exp(-gamma R)/(4 pi R).  Fields mirror BEAST's operator structs
+ * (alpha, gamma; HH3DSingleLayerFDBIO at assamble_chunk_gpu.jl:44-47; beta
+ * for MWSingleLayer3D).  beta is ignored by the Helmholtz operator. */
+typedef struct {
+    int32_t kind;
+    double alpha[2];
+    double beta[2];
+    double gamma[2];
+} bmi_operator;
+
+/* Quadrature and precision.  Replaces the quadstrat / qd arguments of
+ * assemble_gpu (assamble_chunk_gpu.jl:67-75) and GPUConfiguration fields
+ * (utils/configuration.jl:11-26). */
+typedef struct {
+    int32_t far_test_order;  /* triangle rule for test cells: 1, 2 (3 points) or 3 (4 points) */
+    int32_t far_trial_order; /* triangle rule for trial cells */
+    int32_t ss_order[3];     /* Gauss-Legendre points per hypercube dimension for CV, CE, CF (1..8) */
+    int32_t near_order;      /* collapsed Gauss order for class-4 pairs (1..8) */
+    int32_t precision;       /* BMI_PREC_FP64 | BMI_PREC_MIXED */
+    double near_factor;      /* mixed: far pairs with centroid distance < near_factor * max edge stay FP64 */
+} bmi_options;
+
+/* Per-assembly phase timings (milliseconds, HIP events on the library stream)
+ * and pair counts of the last bmi_assemble on a plan. */
+typedef struct {
+    double classify_ms;
+    double special_ms;  /* Sauter-Schwab, near and FP64-kept far pairs */
+    double far_ms;      /* far-pair block kernels incl. write-back */
+    double total_ms;
+    int64_t pairs[6];   /* per BMI_CLASS_* */
+    int32_t far_launches;
+    int32_t n_colours_test;
+    int32_t n_colours_trial;
+} bmi_stats;
+
+/* Library version string. */
+const char *bmi_version(void);
+
+/* Message of the last failed call on this thread. */
+const char *bmi_last_error(void);
+
+/* Fills opts with the defaults: 3 x 4 point far rule, 5-point Gauss-Legendre
+ * Sauter-Schwab, near_order 4, FP64, near_factor 2. */
+void bmi_options_default(bmi_options *opts);
+
+/* One context per device; owns a HIP stream.
+ * Replaces GPUConfiguration.backend (utils/configuration.jl:20). */
+bmi_status bmi_context_create(int32_t device, bmi_context **out);
+bmi_status bmi_context_destroy(bmi_context *ctx);
+
+/* Triangle mesh: xyz is 3 x nv column-major, tri is 3 x nt (0-based).
+ * Replaces the element packing of load_data_elements (load_parameters.jl:25-46). */
+bmi_status bmi_mesh_create(bmi_context *ctx, const double *xyz, int64_t nv, const int32_t *tri, int64_t nt,
+                           bmi_mesh **out);
+bmi_status bmi_mesh_free(bmi_mesh *mesh);
+
+/* Function space: gidx / coeff are 3 x nt (local shape i of cell K -> global
+ * function and coefficient; gidx < 0 means no function), the first-slot layout
+ * of extract_values_and_indexes (load_parameters.jl:50-55). */
+bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, const double *coeff, int64_t nfun,
+                            bmi_space **out);
+bmi_status bmi_space_free(bmi_space *space);
+
+/* Assembly plan for (operator, test space, trial space, options): device
+ * geometry, patch colouring and special-pair buffers.  Replaces the set-up half
+ * of assemblechunk_gpu! / assemblechunk_body_gpu! (assamble_chunk_gpu.jl:49-144). */
+bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_space *test, const bmi_space *trial,
+                           const bmi_options *opts, bmi_plan **out);
+bmi_status bmi_plan_free(bmi_plan *plan);
+
+/* Device-resident matrix, column-major rows x cols.
+ * Replaces the device result cache (CustomDataStructs/GpuWriteBack.jl:11-58). */
+bmi_status bmi_matrix_create(bmi_context *ctx, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out);
+bmi_status bmi_matrix_free(bmi_matrix *m);
+bmi_status bmi_matrix_device_ptr(const bmi_matrix *m, void **ptr, int64_t *ld);
+
+/* Assemble the whole matrix into out (blocking).  Classification, singular
+ * integrals, far pairs and write-back all run on the device.
+ * Replaces assemble_gpu's compute path (assamble_gpu.jl:25-87). */
+bmi_status bmi_assemble(bmi_plan *plan, bmi_matrix *out);
+
+/* Timings and pair counts of the last bmi_assemble on this plan. */
+bmi_status bmi_plan_stats(const bmi_plan *plan, bmi_stats *stats);
+
+/* Copy a device matrix to caller-owned host memory as C64 or C32.
+ * Replaces copy_to_CPU (utils/copy_to_CPU.jl:5-61). */
+bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BMI_H */

@@ -1,0 +1,550 @@
+/*
+ * oracle.c -- CPU FP64 restatement of the BEASTGPU assembly path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this file; only tests/, the smoke check and the CPU-baseline leg of
+ * bench.py use it, as the checker.
+ *
+ * What it restates (file:line in the reference):
+ *   - pair classification: hits = #vertex pairs closer than 1e3*eps, near test
+ *     dmin^2 * max(16|gamma|^2, 1/A_trial) < 0.64        quadrule_determine.jl:1-3, :54-77
+ *   - far-pair product rule, weights carry the Jacobian    doubleQuadRuleMainCase.jl:83-103,
+ *                                                          GPU_scheduler.jl:61-99
+ *   - Sauter-Schwab common face/edge/vertex region maps    SauterSchwab.jl:265-298,
+ *                                                          strategy_sauterschwabints.jl:181-220
+ *   - vertex correspondence / reference-chart permutation  pulledback_integrand_gpu.jl:22-155, :208-228
+ *   - point map x = v3 + t1*b1 + t2*b2, Jacobian 4*A_x*A_y Integrand_gpu.jl:51-132
+ *   - local -> global scatter with coefficient product     store_with_kernel.jl:88-97
+ * and extends them, from BEAST's operator semantics, to the Maxwell
+ * single-layer (EFIE) operator on RWG (Raviart-Thomas, unit-flux) functions,
+ * which the reference does not implement:
+ *   Z_mn = alpha <f_m, G f_n> + beta <div f_m, G div f_n>,  G = exp(-gamma R)/(4 pi R).
+ *
+ * Pair classes whose rule the reference leaves open: class-4 ("near") pairs are
+ * dropped by the reference (doubleQuadRuleMainCase.jl:154); here they are
+ * integrated with a Duffy-collapsed Gauss-Legendre product rule of
+ * near_order^2 points per triangle.
+ *
+ * Parity status of each part, and the checks that pin it, are listed in the
+ * header of oracle/oracle.py.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+enum { ORC_HH3D_SL = 1, ORC_MW_SL = 2 };
+enum { ORC_P1 = 1, ORC_RWG = 2 };
+enum { ORC_KERNEL_GREEN = 0, ORC_KERNEL_ONE = 1, ORC_KERNEL_LINEAR = 2, ORC_KERNEL_QUAD = 3 };
+enum { ORC_FAR = 0, ORC_CV = 1, ORC_CE = 2, ORC_CF = 3, ORC_NEAR = 4 };
+
+typedef struct {
+    int32_t op;      /* ORC_HH3D_SL | ORC_MW_SL */
+    int32_t kernel;  /* ORC_KERNEL_*; anything but GREEN is a polynomial test kernel */
+    double alpha[2], beta[2], gamma[2];
+    double lin[3];   /* ORC_KERNEL_LINEAR: G = 1 + lin . (x - y) */
+} orc_operator;
+
+typedef struct {
+    const double *xyz;   /* 3 x nv, column-major (x,y,z per vertex) */
+    int64_t nv;
+    const int32_t *tri;  /* 3 x nt, 0-based vertex ids */
+    int64_t nt;
+    int32_t kind;        /* ORC_P1 | ORC_RWG */
+    const int64_t *idx;  /* 3 x nt: global function of local shape i on cell K, or -1 */
+    const double *coeff; /* 3 x nt: coefficient of that local shape */
+    int64_t nfun;
+} orc_space;
+
+typedef struct {
+    int32_t far_test;    /* triangle rule: 1 -> 1 point, 2 -> 3 points, 3 -> 4 points */
+    int32_t far_trial;
+    int32_t ss_order[3]; /* Gauss-Legendre points per hypercube dimension: CV, CE, CF */
+    int32_t near_order;  /* collapsed Gauss rule order for class-4 pairs */
+} orc_quadrature;
+
+/* ------------------------------------------------------------------ rules */
+
+typedef struct { int n; double b1[512], b2[512], w[512]; } tri_rule;
+
+/* Symmetric triangle rules on the reference simplex {b1,b2 >= 0, b1+b2 <= 1}
+ * (area 1/2, weights sum to 1/2).  Degree 1: centroid.  Degree 2: the
+ * Strang-Fix interior 3-point rule (2/3,1/6,1/6).  Degree 3: the 4-point rule
+ * with the centroid at weight -27/96 and (3/5,1/5,1/5) at 25/96. */
+static int make_far_rule(int order, tri_rule *r) {
+    if (order == 1) {
+        r->n = 1; r->b1[0] = 1.0 / 3; r->b2[0] = 1.0 / 3; r->w[0] = 0.5;
+        return 0;
+    }
+    if (order == 2) {
+        const double a = 2.0 / 3, b = 1.0 / 6;
+        double l[3][3] = {{a, b, b}, {b, a, b}, {b, b, a}};
+        r->n = 3;
+        for (int i = 0; i < 3; i++) { r->b1[i] = l[i][0]; r->b2[i] = l[i][1]; r->w[i] = 1.0 / 6; }
+        return 0;
+    }
+    if (order == 3) {
+        const double a = 3.0 / 5, b = 1.0 / 5;
+        double l[4][3] = {{1.0 / 3, 1.0 / 3, 1.0 / 3}, {a, b, b}, {b, a, b}, {b, b, a}};
+        double w[4] = {-27.0 / 96, 25.0 / 96, 25.0 / 96, 25.0 / 96};
+        r->n = 4;
+        for (int i = 0; i < 4; i++) { r->b1[i] = l[i][0]; r->b2[i] = l[i][1]; r->w[i] = w[i]; }
+        return 0;
+    }
+    return -1;
+}
+
+/* Gauss-Legendre nodes/weights on [0,1] (weights sum to 1), by Newton on P_n. */
+void orc_gauss_legendre01(int n, double *x, double *w) {
+    for (int i = 0; i < n; i++) {
+        double z = cos(M_PI * (i + 0.75) / (n + 0.5));
+        double pp = 0.0;
+        for (int it = 0; it < 100; it++) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 1; j <= n; j++) {
+                double p3 = p2; p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            double dz = p1 / pp;
+            z -= dz;
+            if (fabs(dz) < 1e-16) break;
+        }
+        /* recompute derivative at the converged node */
+        double p1 = 1.0, p2 = 0.0;
+        for (int j = 1; j <= n; j++) {
+            double p3 = p2; p2 = p1;
+            p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+        }
+        pp = n * (z * p1 - p2) / (z * z - 1.0);
+        /* z descends with i, so (1 - z)/2 ascends */
+        x[i] = 0.5 * (1.0 - z);
+        w[i] = 1.0 / ((1.0 - z * z) * pp * pp);
+    }
+}
+
+/* Duffy-collapsed product rule on the reference simplex, order^2 points. */
+static int make_collapsed_rule(int order, tri_rule *r) {
+    if (order < 1 || order * order > 512) return -1;
+    double x[64], w[64];
+    orc_gauss_legendre01(order, x, w);
+    r->n = 0;
+    for (int i = 0; i < order; i++)
+        for (int j = 0; j < order; j++) {
+            double s = x[i], t = x[j];
+            r->b1[r->n] = s;
+            r->b2[r->n] = t * (1.0 - s);
+            r->w[r->n] = w[i] * w[j] * (1.0 - s);
+            r->n++;
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------- geometry */
+
+typedef struct {
+    double v[3][3];  /* vertices */
+    double t1[3], t2[3];
+    double area;
+} cell;
+
+static void load_cell(const orc_space *s, int64_t k, cell *c) {
+    for (int i = 0; i < 3; i++) {
+        int64_t vid = s->tri[3 * k + i];
+        for (int d = 0; d < 3; d++) c->v[i][d] = s->xyz[3 * vid + d];
+    }
+    for (int d = 0; d < 3; d++) {
+        c->t1[d] = c->v[0][d] - c->v[2][d];
+        c->t2[d] = c->v[1][d] - c->v[2][d];
+    }
+    double n0 = c->t1[1] * c->t2[2] - c->t1[2] * c->t2[1];
+    double n1 = c->t1[2] * c->t2[0] - c->t1[0] * c->t2[2];
+    double n2 = c->t1[0] * c->t2[1] - c->t1[1] * c->t2[0];
+    c->area = 0.5 * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+}
+
+static inline void chart_point(const cell *c, double b1, double b2, double *x) {
+    for (int d = 0; d < 3; d++) x[d] = c->v[2][d] + c->t1[d] * b1 + c->t2[d] * b2;
+}
+
+/* local shape values at chart parameters (b1,b2) */
+typedef struct { double val[3][3]; double div[3]; double sval[3]; } shapes;
+
+static void eval_shapes(int kind, const cell *c, double b1, double b2, const double *x, shapes *s) {
+    if (kind == ORC_P1) {
+        s->sval[0] = b1; s->sval[1] = b2; s->sval[2] = 1.0 - b1 - b2;
+    } else {
+        /* Raviart-Thomas reference shapes, unit flux through the edge
+         * opposite vertex i: f_i(x) = (x - v_i) / (2A), div f_i = 1/A */
+        double inv2a = 1.0 / (2.0 * c->area);
+        for (int i = 0; i < 3; i++) {
+            for (int d = 0; d < 3; d++) s->val[i][d] = (x[d] - c->v[i][d]) * inv2a;
+            s->div[i] = 1.0 / c->area;
+        }
+    }
+}
+
+static inline cplx kernel_value(const orc_operator *op, const double *x, const double *y) {
+    double r0 = x[0] - y[0], r1 = x[1] - y[1], r2 = x[2] - y[2];
+    double R2 = r0 * r0 + r1 * r1 + r2 * r2;
+    switch (op->kernel) {
+    case ORC_KERNEL_ONE: return 1.0;
+    case ORC_KERNEL_LINEAR: return 1.0 + op->lin[0] * r0 + op->lin[1] * r1 + op->lin[2] * r2;
+    case ORC_KERNEL_QUAD: return R2;
+    default: {
+        double R = sqrt(R2);
+        cplx g = op->gamma[0] + I * op->gamma[1];
+        return cexp(-g * R) / (4.0 * M_PI * R);
+    }
+    }
+}
+
+/* integrand for local shapes (i,j) at (x,y); accumulates w * integrand into Z */
+static void accumulate(const orc_operator *op, int tkind, int bkind, const shapes *f, const shapes *g,
+                       cplx G, double w, cplx Z[3][3]) {
+    cplx a = op->alpha[0] + I * op->alpha[1];
+    if (op->op == ORC_HH3D_SL) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                Z[i][j] += w * a * G * f->sval[i] * g->sval[j];
+    } else {
+        cplx b = op->beta[0] + I * op->beta[1];
+        (void)tkind; (void)bkind;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double dot = f->val[i][0] * g->val[j][0] + f->val[i][1] * g->val[j][1] + f->val[i][2] * g->val[j][2];
+                Z[i][j] += w * G * (a * dot + b * f->div[i] * g->div[j]);
+            }
+    }
+}
+
+/* --------------------------------------------------------- classification */
+
+static const double DTOL = 2.220446049250313e-16 * 1.0e3;
+
+/* quadrule_determine.jl:54-77 */
+static int classify(const orc_operator *op, const cell *t, const cell *s, int match[3]) {
+    int hits = 0;
+    double dmin2 = 1.79769313486231570815e308;
+    for (int i = 0; i < 3; i++) match[i] = -1;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double dx = t->v[i][0] - s->v[j][0], dy = t->v[i][1] - s->v[j][1], dz = t->v[i][2] - s->v[j][2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < dmin2) dmin2 = d2;
+            if (sqrt(d2) < DTOL) { hits++; match[i] = j; }
+        }
+    if (hits > 0) return hits;
+    double k2x16 = 16.0 * (op->gamma[0] * op->gamma[0] + op->gamma[1] * op->gamma[1]);
+    double inva = 1.0 / s->area;
+    double m = k2x16 > inva ? k2x16 : inva;
+    if (dmin2 * m < 0.2 * 0.2 * 16) return ORC_NEAR;
+    return ORC_FAR;
+}
+
+/* ------------------------------------------------------------ far / near */
+
+static void double_rule(const orc_operator *op, const orc_space *X, const orc_space *Y,
+                        const cell *t, const cell *s, const tri_rule *rt, const tri_rule *rs,
+                        cplx Z[3][3]) {
+    for (int ia = 0; ia < rt->n; ia++) {
+        double x[3];
+        chart_point(t, rt->b1[ia], rt->b2[ia], x);
+        shapes f;
+        eval_shapes(X->kind, t, rt->b1[ia], rt->b2[ia], x, &f);
+        double wx = rt->w[ia] * 2.0 * t->area;
+        for (int jb = 0; jb < rs->n; jb++) {
+            double y[3];
+            chart_point(s, rs->b1[jb], rs->b2[jb], y);
+            shapes g;
+            eval_shapes(Y->kind, s, rs->b1[jb], rs->b2[jb], y, &g);
+            double wy = rs->w[jb] * 2.0 * s->area;
+            accumulate(op, X->kind, Y->kind, &f, &g, kernel_value(op, x, y), wx * wy, Z);
+        }
+    }
+}
+
+/* ---------------------------------------------------------- Sauter-Schwab */
+
+/* reference simplex vertices used for the permuted charts
+ * (pulledback_integrand_gpu.jl:201-202, strategy_sauterschwabints.jl:33-35) */
+static const double VREF[3][2] = {{1.0, 0.0}, {0.0, 1.0}, {0.0, 0.0}};
+
+/* Vertex correspondence (pulledback_integrand_gpu.jl:22-155), 0-based.
+ * PT[k], PS[k]: which vertex of the test / trial cell sits at reference-chart vertex k. */
+static void ss_permutation(int cls, const cell *t, const cell *s, int PT[3], int PS[3]) {
+    int mi[3], mj[3], e = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double dx = t->v[i][0] - s->v[j][0], dy = t->v[i][1] - s->v[j][1], dz = t->v[i][2] - s->v[j][2];
+            if (sqrt(dx * dx + dy * dy + dz * dz) < DTOL && e < 3) { mi[e] = i; mj[e] = j; e++; }
+        }
+    if (cls == ORC_CF) {
+        /* I = identity, PS[i] = trial vertex matching test vertex i */
+        for (int k = 0; k < 3; k++) PT[k] = k;
+        for (int k = 0; k < e; k++) PS[mi[k]] = mj[k];
+    } else if (cls == ORC_CE) {
+        /* matched pair first, complement third, then rotate left so the
+         * common edge joins reference vertices 1 and 3 */
+        int a[3] = {mi[0], mi[1], 3 - mi[0] - mi[1]};
+        int b[3] = {mj[0], mj[1], 3 - mj[0] - mj[1]};
+        PT[0] = a[1]; PT[1] = a[2]; PT[2] = a[0];
+        PS[0] = b[1]; PS[1] = b[2]; PS[2] = b[0];
+    } else {
+        /* common vertex first, then the smallest other index */
+        PT[0] = mi[0]; PT[1] = (mi[0] == 0) ? 1 : 0; PT[2] = 3 - PT[0] - PT[1];
+        PS[0] = mj[0]; PS[1] = (mj[0] == 0) ? 1 : 0; PS[2] = 3 - PS[0] - PS[1];
+    }
+}
+
+typedef struct { double v3[2], t1[2], t2[2]; } ref_chart;
+
+static void make_ref_chart(const int P[3], ref_chart *c) {
+    for (int d = 0; d < 2; d++) {
+        c->v3[d] = VREF[P[2]][d];
+        c->t1[d] = VREF[P[0]][d] - VREF[P[2]][d];
+        c->t2[d] = VREF[P[1]][d] - VREF[P[2]][d];
+    }
+}
+
+static void ss_eval(const orc_operator *op, const orc_space *X, const orc_space *Y,
+                    const cell *t, const cell *s, const ref_chart *c1, const ref_chart *c2,
+                    double u1, double u2, double v1, double v2, double mul, cplx Z[3][3]) {
+    /* calc_bary (Integrand_gpu.jl:51-55) */
+    double bx1 = c1->v3[0] + c1->t1[0] * u1 + c1->t2[0] * u2;
+    double bx2 = c1->v3[1] + c1->t1[1] * u1 + c1->t2[1] * u2;
+    double by1 = c2->v3[0] + c2->t1[0] * v1 + c2->t2[0] * v2;
+    double by2 = c2->v3[1] + c2->t1[1] * v1 + c2->t2[1] * v2;
+    double x[3], y[3];
+    chart_point(t, bx1, bx2, x);
+    chart_point(s, by1, by2, y);
+    shapes f, g;
+    eval_shapes(X->kind, t, bx1, bx2, x, &f);
+    eval_shapes(Y->kind, s, by1, by2, y, &g);
+    accumulate(op, X->kind, Y->kind, &f, &g, kernel_value(op, x, y), 4.0 * t->area * s->area * mul, Z);
+}
+
+static void sauter_schwab(const orc_operator *op, const orc_space *X, const orc_space *Y,
+                          const cell *t, const cell *s, int cls, int n, cplx Z[3][3]) {
+    int PT[3], PS[3];
+    ss_permutation(cls, t, s, PT, PS);
+    ref_chart c1, c2;
+    make_ref_chart(PT, &c1);
+    make_ref_chart(PS, &c2);
+    double gx[64], gw[64];
+    orc_gauss_legendre01(n, gx, gw);
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++)
+            for (int c = 0; c < n; c++)
+                for (int d = 0; d < n; d++) {
+                    double e1 = gx[a], e2 = gx[b], e3 = gx[c], xi = gx[d];
+                    double w = gw[a] * gw[b] * gw[c] * gw[d];
+                    if (cls == ORC_CV) {
+                        /* SauterSchwab.jl:265-272 */
+                        double xe1 = xi * e1, xe2 = xi * e2;
+                        double m = w * xi * xi * xi * e2;
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi, xe1, 1 - xe2, xe2 * e3, m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xe2, xe2 * e3, 1 - xi, xe1, m, Z);
+                    } else if (cls == ORC_CE) {
+                        /* SauterSchwab.jl:274-288 */
+                        double xe1 = xi * e1, e12 = e1 * e2, e23 = e2 * e3, e123 = e12 * e3;
+                        double m = w * xi * xi * xi * e1 * e1 * e2;
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi, xe1, 1 - xi * (1 - e123), xe1 * e2 * (1 - e3), m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi * (1 - e12), xe1 * (1 - e2), 1 - xi, xe1 * e23, m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi * (1 - e123), xe1 * e2 * (1 - e3), 1 - xi, xe1, m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi * (1 - e123), xe1 * (1 - e23), 1 - xi, xe1 * e2, m, Z);
+                        double m5 = w * xi * xi * xi * e1 * e1;
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi, xe1 * e3, 1 - xi * (1 - e12), xe1 * (1 - e2), m5, Z);
+                    } else {
+                        /* SauterSchwab.jl:290-298 */
+                        double m = w * xi * xi * xi * e1 * e1 * e2;
+                        double p = xi * e1;
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi, xi - p + p * e2, 1 - (xi - p * e2 * e3), xi - p, m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - (xi - p * e2 * e3), xi - p, 1 - xi, xi - p + p * e2, m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi, p * (1 - e2 + e2 * e3), 1 - (xi - p * e2), p * (1 - e2), m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - (xi - p * e2), p * (1 - e2), 1 - xi, p * (1 - e2 + e2 * e3), m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - (xi - p * e2 * e3), p * (1 - e2 * e3), 1 - xi, p * (1 - e2), m, Z);
+                        ss_eval(op, X, Y, t, s, &c1, &c2, 1 - xi, p * (1 - e2), 1 - (xi - p * e2 * e3), p * (1 - e2 * e3), m, Z);
+                    }
+                }
+}
+
+/* ---------------------------------------------------------------- pairs */
+
+/* Local element matrix of one (test cell p, trial cell q) pair; returns its class. */
+int orc_pair_block(const orc_operator *op, const orc_space *X, const orc_space *Y,
+                   const orc_quadrature *q, int64_t p, int64_t qq, double *out /* 18 doubles, (i,j) row-major, re/im */) {
+    cell t, s;
+    load_cell(X, p, &t);
+    load_cell(Y, qq, &s);
+    int match[3];
+    int cls = classify(op, &t, &s, match);
+    cplx Z[3][3];
+    memset(Z, 0, sizeof(Z));
+    if (cls == ORC_FAR) {
+        tri_rule rt, rs;
+        if (make_far_rule(q->far_test, &rt) || make_far_rule(q->far_trial, &rs)) return -1;
+        double_rule(op, X, Y, &t, &s, &rt, &rs, Z);
+    } else if (cls == ORC_NEAR) {
+        tri_rule rn;
+        if (make_collapsed_rule(q->near_order, &rn)) return -1;
+        double_rule(op, X, Y, &t, &s, &rn, &rn, Z);
+    } else if (cls <= 3) {
+        int n = q->ss_order[cls - 1];
+        if (n < 1 || n > 64) return -1;
+        sauter_schwab(op, X, Y, &t, &s, cls, n, Z);
+    } else {
+        return -1;
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            out[2 * (3 * i + j)] = creal(Z[i][j]);
+            out[2 * (3 * i + j) + 1] = cimag(Z[i][j]);
+        }
+    return cls;
+}
+
+/* Greedy colouring of test cells so that no two cells of one colour share a
+ * selected output row; cells touching no selected row get colour -1. */
+static int colour_cells(const orc_space *X, const int64_t *rowmap, int *colour) {
+    int64_t nt = X->nt;
+    /* function -> list of cells */
+    int64_t *cnt = calloc((size_t)X->nfun + 1, sizeof(int64_t));
+    for (int64_t k = 0; k < nt; k++)
+        for (int i = 0; i < 3; i++) {
+            int64_t f = X->idx[3 * k + i];
+            if (f >= 0 && rowmap[f] >= 0) cnt[f + 1]++;
+        }
+    for (int64_t f = 0; f < X->nfun; f++) cnt[f + 1] += cnt[f];
+    int64_t *lst = malloc(sizeof(int64_t) * (size_t)(cnt[X->nfun] + 1));
+    int64_t *pos = malloc(sizeof(int64_t) * (size_t)(X->nfun + 1));
+    memcpy(pos, cnt, sizeof(int64_t) * (size_t)(X->nfun + 1));
+    for (int64_t k = 0; k < nt; k++)
+        for (int i = 0; i < 3; i++) {
+            int64_t f = X->idx[3 * k + i];
+            if (f >= 0 && rowmap[f] >= 0) lst[pos[f]++] = k;
+        }
+    int ncol = 0;
+    for (int64_t k = 0; k < nt; k++) {
+        colour[k] = -1;
+        int used[256] = {0};
+        int touches = 0;
+        for (int i = 0; i < 3; i++) {
+            int64_t f = X->idx[3 * k + i];
+            if (f < 0 || rowmap[f] < 0) continue;
+            touches = 1;
+            for (int64_t e = cnt[f]; e < cnt[f + 1]; e++) {
+                int64_t o = lst[e];
+                if (o < k && colour[o] >= 0 && colour[o] < 256) used[colour[o]] = 1;
+            }
+        }
+        if (!touches) continue;
+        int c = 0;
+        while (c < 256 && used[c]) c++;
+        colour[k] = c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    free(cnt); free(lst); free(pos);
+    return ncol;
+}
+
+/*
+ * Assemble Z (complex interleaved, column-major, nrows_out x Y->nfun) for the
+ * selected test functions `rows` (all functions if rows == NULL).
+ * class_counts[0..4] receives the number of (cell, cell) pairs per class
+ * (far, CV, CE, CF, near) over the cells that were processed.
+ */
+int orc_assemble(const orc_operator *op, const orc_space *X, const orc_space *Y, const orc_quadrature *q,
+                 const int64_t *rows, int64_t nrows, double *Z, int nthreads, int64_t *class_counts) {
+    int64_t nout = rows ? nrows : X->nfun;
+    int64_t *rowmap = malloc(sizeof(int64_t) * (size_t)(X->nfun > 0 ? X->nfun : 1));
+    for (int64_t f = 0; f < X->nfun; f++) rowmap[f] = rows ? -1 : f;
+    if (rows)
+        for (int64_t r = 0; r < nrows; r++) {
+            if (rows[r] < 0 || rows[r] >= X->nfun) { free(rowmap); return -1; }
+            rowmap[rows[r]] = r;
+        }
+    memset(Z, 0, sizeof(double) * 2 * (size_t)nout * (size_t)Y->nfun);
+    int *colour = malloc(sizeof(int) * (size_t)X->nt);
+    int ncol = colour_cells(X, rowmap, colour);
+    int64_t counts[5] = {0, 0, 0, 0, 0};
+    int err = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    for (int c = 0; c < ncol; c++) {
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : counts[:5]) reduction(| : err)
+        for (int64_t p = 0; p < X->nt; p++) {
+            if (colour[p] != c) continue;
+            for (int64_t qq = 0; qq < Y->nt; qq++) {
+                double blk[18];
+                int cls = orc_pair_block(op, X, Y, q, p, qq, blk);
+                if (cls < 0) { err |= 1; continue; }
+                counts[cls]++;
+                for (int i = 0; i < 3; i++) {
+                    int64_t m = X->idx[3 * p + i];
+                    if (m < 0) continue;
+                    int64_t r = rowmap[m];
+                    if (r < 0) continue;
+                    double a = X->coeff[3 * p + i];
+                    for (int j = 0; j < 3; j++) {
+                        int64_t n = Y->idx[3 * qq + j];
+                        if (n < 0) continue;
+                        double cf = a * Y->coeff[3 * qq + j];
+                        size_t o = 2 * ((size_t)n * (size_t)nout + (size_t)r);
+                        Z[o] += cf * blk[2 * (3 * i + j)];
+                        Z[o + 1] += cf * blk[2 * (3 * i + j) + 1];
+                    }
+                }
+            }
+        }
+    }
+    if (class_counts)
+        for (int i = 0; i < 5; i++) class_counts[i] = counts[i];
+    free(rowmap);
+    free(colour);
+    return err ? -1 : 0;
+}
+
+/* Pair classes of every (test cell, trial cell) pair, written as int8 (nt_test x nt_trial, row-major). */
+int orc_classify_all(const orc_operator *op, const orc_space *X, const orc_space *Y, int8_t *cls_out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < X->nt; p++) {
+        cell t, s;
+        load_cell(X, p, &t);
+        for (int64_t qq = 0; qq < Y->nt; qq++) {
+            load_cell(Y, qq, &s);
+            int match[3];
+            cls_out[p * Y->nt + qq] = (int8_t)classify(op, &t, &s, match);
+        }
+    }
+    return 0;
+}
+
+/* Far-rule and SS rule tables, exposed for the rule-exactness tests. */
+int orc_far_rule(int order, double *b1, double *b2, double *w) {
+    tri_rule r;
+    if (make_far_rule(order, &r)) return -1;
+    for (int i = 0; i < r.n; i++) { b1[i] = r.b1[i]; b2[i] = r.b2[i]; w[i] = r.w[i]; }
+    return r.n;
+}
+
+int orc_collapsed_rule(int order, double *b1, double *b2, double *w) {
+    tri_rule r;
+    if (make_collapsed_rule(order, &r)) return -1;
+    for (int i = 0; i < r.n; i++) { b1[i] = r.b1[i]; b2[i] = r.b2[i]; w[i] = r.w[i]; }
+    return r.n;
+}
