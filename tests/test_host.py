@@ -1,0 +1,140 @@
+"""Host-side tests (no GPU): meshes, spaces, the C ABI surface of libbmi.so."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from beastgpu_amd import _bmi, meshes, spaces
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ----------------------------------------------------------------- meshes
+
+@pytest.mark.parametrize("level,nv,nt", [(0, 12, 20), (1, 42, 80), (3, 642, 1280), (5, 10242, 20480)])
+def test_icosphere_counts_and_orientation(level, nv, nt):
+    m = meshes.icosphere(level)
+    assert (m.nv, m.nt) == (nv, nt)
+    v = m.vertices[m.faces]
+    n = np.cross(v[:, 0] - v[:, 2], v[:, 1] - v[:, 2])
+    assert (np.einsum("ij,ij->i", n, v.mean(axis=1)) > 0).all()
+    assert np.allclose(np.linalg.norm(m.vertices, axis=1), 1.0)
+    assert abs(m.areas().sum() - 4 * np.pi) < 0.5 * 4 ** -level * 4 * np.pi
+
+
+def test_perturbation_amplitude_and_seed():
+    m = meshes.icosphere(3)
+    a = meshes.perturb_radial(m, 2, 0.03)
+    b = meshes.perturb_radial(m, 2, 0.03)
+    c = meshes.perturb_radial(m, 3, 0.03)
+    r = np.linalg.norm(a.vertices, axis=1)
+    assert abs(np.abs(r - 1.0).max() - 0.03) < 1e-12
+    assert np.array_equal(a.vertices, b.vertices)
+    assert not np.array_equal(a.vertices, c.vertices)
+
+
+@pytest.mark.parametrize("d", [1, 2, 24])
+def test_meshcuboid_degree_pattern(d):
+    m = meshes.meshcuboid(1.0, 1.0, 1.0, 0.5 / d)
+    n = 2 * d
+    assert m.nt == 12 * n * n
+    assert m.nv == 6 * n * n + 2
+    deg = np.bincount(m.vertex_degrees())
+    assert deg[3] == 2 and deg[5] == 6 and deg[6] == m.nv - 8
+    if d == 24:
+        # reference's recorded common-vertex count (data/GPU/Saut-Swab.txt:50-55)
+        assert int((m.vertex_degrees().astype(np.int64) ** 2).sum() - 9 * m.nt) == 248784
+
+
+def test_ellipsoid_60k():
+    m = meshes.ellipsoid()
+    assert m.nt == 60000
+    X = spaces.raviartthomas(m)
+    assert X.nfun == 90000
+    assert (X.idx >= 0).all()
+    r = (m.vertices / np.array([1.0, 0.7, 0.5])) ** 2
+    assert np.allclose(r.sum(axis=1), 1.0)
+    assert m.areas().min() > 0.2 * m.areas().mean()
+
+
+# ----------------------------------------------------------------- spaces
+
+def test_rwg_space_structure():
+    m = meshes.icosphere(2)
+    X = spaces.raviartthomas(m)
+    assert X.nfun == 3 * m.nt // 2
+    cnt_plus = np.bincount(X.idx[X.coeff > 0], minlength=X.nfun)
+    cnt_minus = np.bincount(X.idx[X.coeff < 0], minlength=X.nfun)
+    assert (cnt_plus == 1).all() and (cnt_minus == 1).all()
+    # local shape i sits on the edge opposite vertex i
+    for K in range(10):
+        f = m.faces[K]
+        for i in range(3):
+            e = X.idx[K, i]
+            other = np.argwhere(X.idx == e)
+            K2, i2 = [tuple(o) for o in other if o[0] != K][0]
+            assert set(m.faces[K2]) - {m.faces[K2][i2]} == set(f) - {f[i]}
+
+
+def test_rwg_open_mesh_excludes_boundary_edges():
+    m = meshes.Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float),
+                    np.array([[0, 1, 2], [1, 3, 2]], np.int32))
+    X = spaces.raviartthomas(m)
+    assert X.nfun == 1
+    assert (X.idx >= 0).sum() == 2
+
+
+def test_lagrange_space():
+    m = meshes.icosphere(1)
+    X = spaces.lagrangec0d1(m)
+    assert X.nfun == m.nv
+    assert np.array_equal(X.idx, m.faces)
+    assert (X.coeff == 1.0).all()
+
+
+# ------------------------------------------------------------------ C ABI
+
+def _declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "bmi.h")).read()
+    return sorted(set(re.findall(r"\b(bmi_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _bmi.lib()
+    declared = _declared_entry_points()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(_bmi.EXPORTS) == declared
+
+
+def test_options_default_and_version():
+    o = _bmi.default_options()
+    assert (o.far_test_order, o.far_trial_order, list(o.ss_order), o.near_order) == (2, 3, [5, 5, 5], 4)
+    assert o.precision == _bmi.PREC_FP64 and o.near_factor == 2.0
+    assert b"gfx950" in _bmi.lib().bmi_version()
+
+
+def test_struct_layouts_match_header():
+    assert ctypes.sizeof(_bmi.Operator) == 4 + 4 + 6 * 8  # int32 + padding + 3 complex
+    assert ctypes.sizeof(_bmi.Options) == 6 * 4 + 4 + 4 + 8
+    assert ctypes.sizeof(_bmi.Stats) == 4 * 8 + 6 * 8 + 3 * 4 + 4
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_no_device_fails_loudly():
+    from beastgpu_amd.api import Context
+
+    with pytest.raises(_bmi.BMIError) as e:
+        Context(0)
+    assert e.value.status in (_bmi.EINVAL, _bmi.EDEVICE)
+
+
+def test_null_arguments_are_rejected():
+    L = _bmi.lib()
+    assert L.bmi_context_create(0, None) == _bmi.EINVAL
+    assert L.bmi_assemble(None, None) == _bmi.EINVAL
+    assert b"NULL" in L.bmi_last_error()
+    assert L.bmi_matrix_free(None) == _bmi.OK

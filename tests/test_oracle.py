@@ -1,0 +1,319 @@
+"""Pinning tests of the CPU oracle (no GPU).
+
+Each oracle function is checked against something that does not share its
+code: closed-form integrals, independent high-order cubature, the analytic
+potential of a triangle, mathematical invariants, or the counts the reference
+recorded for its own meshes.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from beastgpu_amd import Helmholtz3D, Maxwell3D, meshes, spaces
+from oracle import oracle as O
+from tests import _indep as ind
+
+
+# ------------------------------------------------------------------ rules
+
+@pytest.mark.parametrize("order,degree", [(1, 1), (2, 2), (3, 3)])
+def test_triangle_rules_exact(order, degree):
+    b1, b2, w = O.far_rule(order)
+    for a in range(degree + 1):
+        for b in range(degree + 1 - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.sum(w * b1 ** a * b2 ** b) - exact) < 1e-15
+    # one degree higher is not integrated exactly by the 3- and 4-point rules
+    if order in (2, 3):
+        errs = [abs(np.sum(w * b1 ** a * b2 ** (degree + 1 - a)) -
+                    math.factorial(a) * math.factorial(degree + 1 - a) / math.factorial(degree + 3))
+                for a in range(degree + 2)]
+        assert max(errs) > 1e-6
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 5, 8, 12])
+def test_gauss_legendre_exact(n):
+    x, w = O.gauss_legendre01(n)
+    xs, ws = ind.gl01(n)
+    assert np.allclose(x, xs, atol=1e-15, rtol=0) and np.allclose(w, ws, atol=1e-15, rtol=0)
+    for k in range(2 * n):
+        assert abs(np.sum(w * x ** k) - 1.0 / (k + 1)) < 1e-14
+
+
+def test_collapsed_rule_exact():
+    b1, b2, w = O.collapsed_rule(4)
+    for a in range(6):
+        for b in range(6 - a):
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.sum(w * b1 ** a * b2 ** b) - exact) < 1e-15
+
+
+# -------------------------------------------------- polynomial kernels
+
+def _poly_reference(op, X, Y, p, q, kernel, lin, n=10):
+    """int_K int_L P(x - y) f_i(x) . g_j(y) with the high-order collapsed rule
+    (exact for these polynomial integrands), independent of the oracle."""
+    tv = X.mesh.vertices[X.mesh.faces[p]]
+    sv = Y.mesh.vertices[Y.mesh.faces[q]]
+    b1, b2, w = ind.collapsed(n)
+    x = ind.cell_points(tv, b1, b2)
+    y = ind.cell_points(sv, b1, b2)
+    At, As = ind.area(tv), ind.area(sv)
+    wx, wy = w * 2 * At, w * 2 * As
+    d = x[:, None, :] - y[None, :, :]
+    if kernel == O.KERNEL_ONE:
+        G = np.ones(d.shape[:2])
+    elif kernel == O.KERNEL_LINEAR:
+        G = 1.0 + d @ np.asarray(lin)
+    else:
+        G = np.einsum("abk,abk->ab", d, d)
+    W = wx[:, None] * wy[None, :] * G
+    Z = np.zeros((3, 3), dtype=complex)
+    if X.kind == spaces.P1:
+        fx = np.stack([b1, b2, 1 - b1 - b2], axis=1)
+        Z = op.alpha * fx.T @ W @ fx
+    else:
+        for i in range(3):
+            fi = (x - tv[i]) / (2 * At)
+            for j in range(3):
+                gj = (y - sv[j]) / (2 * As)
+                Z[i, j] = op.alpha * np.sum(W * (fi @ gj.T)) + op.beta * np.sum(W) / (At * As)
+    return Z
+
+
+def _pairs_by_class(op, X):
+    cls = O.classify_all(op, X, X)
+    out = {}
+    for c in (O.FAR, O.CV, O.CE, O.CF):
+        p, q = np.argwhere(cls == c)[0]
+        out[c] = (int(p), int(q))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["efie", "hh3d"])
+@pytest.mark.parametrize("kernel", [O.KERNEL_ONE, O.KERNEL_LINEAR, O.KERNEL_QUAD])
+def test_ss_polynomial_kernels(kind, kernel):
+    m = meshes.perturb_radial(meshes.icosphere(1), 5, 0.1)
+    if kind == "efie":
+        op, X = Maxwell3D.singlelayer(wavenumber=1.3), spaces.raviartthomas(m)
+    else:
+        op, X = Helmholtz3D.singlelayer(wavenumber=1.3, alpha=0.7 - 0.2j), spaces.lagrangec0d1(m)
+    lin = (0.3, -1.1, 0.7)
+    pairs = _pairs_by_class(op, X)
+    for c in (O.CV, O.CE, O.CF):
+        p, q = pairs[c]
+        Zo, cls = O.pair_block(op, X, X, p, q, kernel=kernel, lin=lin, ss_order=(8, 8, 8))
+        assert cls == c
+        Zr = _poly_reference(op, X, X, p, q, kernel, lin)
+        assert np.abs(Zo - Zr).max() <= 1e-13 * np.abs(Zr).max(), (c, Zo, Zr)
+
+
+@pytest.mark.parametrize("kind", ["efie", "hh3d"])
+@pytest.mark.parametrize("kernel", [O.KERNEL_ONE, O.KERNEL_LINEAR])
+def test_far_rule_polynomial_kernels(kind, kernel):
+    """Far rule: linear shapes times a kernel of degree <= 1 is within the
+    3-point (degree 2) and 4-point (degree 3) rules' exactness."""
+    m = meshes.perturb_radial(meshes.icosphere(1), 5, 0.1)
+    if kind == "efie":
+        op, X = Maxwell3D.singlelayer(wavenumber=1.3), spaces.raviartthomas(m)
+    else:
+        op, X = Helmholtz3D.singlelayer(wavenumber=1.3), spaces.lagrangec0d1(m)
+    p, q = _pairs_by_class(op, X)[O.FAR]
+    lin = (0.3, -1.1, 0.7)
+    Zo, cls = O.pair_block(op, X, X, p, q, kernel=kernel, lin=lin)
+    assert cls == O.FAR
+    Zr = _poly_reference(op, X, X, p, q, kernel, lin)
+    assert np.abs(Zo - Zr).max() <= 1e-13 * np.abs(Zr).max()
+
+
+def _green_reference(op, X, p, q, n=24):
+    tv = X.mesh.vertices[X.mesh.faces[p]]
+    sv = X.mesh.vertices[X.mesh.faces[q]]
+    b1, b2, w = ind.collapsed(n)
+    x, y = ind.cell_points(tv, b1, b2), ind.cell_points(sv, b1, b2)
+    At, As = ind.area(tv), ind.area(sv)
+    R = np.linalg.norm(x[:, None, :] - y[None, :, :], axis=2)
+    W = (w * 2 * At)[:, None] * (w * 2 * As)[None, :] * np.exp(-op.gamma * R) / (4 * np.pi * R)
+    Z = np.zeros((3, 3), dtype=complex)
+    for i in range(3):
+        fi = (x - tv[i]) / (2 * At)
+        for j in range(3):
+            gj = (y - sv[j]) / (2 * As)
+            Z[i, j] = op.alpha * np.sum(W * (fi @ gj.T)) + op.beta * np.sum(W) / (At * As)
+    return Z
+
+
+def test_far_rule_vs_bruteforce():
+    """Well separated pair: the 3x4 rule is within its quadrature error of a
+    576x576-point reference; a wrong weight, point or shape would be O(1)."""
+    m = meshes.icosphere(2)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.singlelayer(wavenumber=2.0)
+    c = m.vertices[m.faces].mean(axis=1)
+    p = 0
+    q = int(np.argmax(np.linalg.norm(c - c[p], axis=1)))  # antipodal cell
+    Zo, cls = O.pair_block(op, X, X, p, q)
+    assert cls == O.FAR
+    Zr = _green_reference(op, X, p, q)
+    assert np.abs(Zo - Zr).max() <= 2e-4 * np.abs(Zr).max()
+
+
+# ------------------------------------------------ static singular integrals
+
+def _hh_static(m):
+    X = spaces.lagrangec0d1(m)
+    op = Helmholtz3D.singlelayer(gamma=0.0)
+    return op, X
+
+
+@pytest.mark.parametrize("cls_want", [O.CV, O.CE, O.CF])
+def test_ss_static_vs_analytic(cls_want):
+    """gamma = 0, P1: sum_j Z_ij = (1/4pi) int_K lambda_i(x) Phi_L(x) dx with
+    Phi_L the closed-form potential of the trial cell, integrated by a
+    composite rule; likewise sum_i Z_ij with the roles swapped.  This pins
+    the vertex correspondence, the region maps and the Jacobians."""
+    m = meshes.perturb_radial(meshes.icosphere(1), 3, 0.05)
+    op, X = _hh_static(m)
+    p, q = _pairs_by_class(op, X)[cls_want]
+    Zo, cls = O.pair_block(op, X, X, p, q, ss_order=(8, 8, 8))
+    assert cls == cls_want
+    tv = m.vertices[m.faces[p]]
+    sv = m.vertices[m.faces[q]]
+
+    def outer(a, b, level):
+        # composite rule; its error falls as 4^-level (log-singular derivative
+        # of the potential at the shared vertex/edge), so two levels are
+        # Richardson-extrapolated
+        b1, b2, w = ind.subdivided(10, level)
+        pts = ind.cell_points(a, b1, b2)
+        phi = ind.triangle_potential(b, pts)
+        lam = np.stack([b1, b2, 1 - b1 - b2], axis=1)
+        return (w * 2 * ind.area(a) * phi) @ lam / (4 * np.pi)
+
+    row = (4 * outer(tv, sv, 4) - outer(tv, sv, 3)) / 3
+    col = (4 * outer(sv, tv, 4) - outer(sv, tv, 3)) / 3
+    assert np.allclose(Zo.sum(axis=1).real, row, rtol=3e-8, atol=0), (Zo.sum(axis=1), row)
+    assert np.allclose(Zo.sum(axis=0).real, col, rtol=3e-8, atol=0), (Zo.sum(axis=0), col)
+    assert np.abs(Zo.imag).max() == 0.0
+
+
+@pytest.mark.parametrize("kind", ["efie", "hh3d"])
+def test_ss_convergence(kind):
+    """Exponential convergence in the Gauss-Legendre order towards n = 12."""
+    m = meshes.perturb_radial(meshes.icosphere(1), 4, 0.05)
+    if kind == "efie":
+        op, X = Maxwell3D.singlelayer(wavenumber=3.0), spaces.raviartthomas(m)
+    else:
+        op, X = Helmholtz3D.singlelayer(wavenumber=3.0), spaces.lagrangec0d1(m)
+    pairs = _pairs_by_class(op, X)
+    for c in (O.CV, O.CE, O.CF):
+        p, q = pairs[c]
+        ref, _ = O.pair_block(op, X, X, p, q, ss_order=(12, 12, 12))
+        errs = []
+        for n in (3, 5, 8):
+            Z, _ = O.pair_block(op, X, X, p, q, ss_order=(n, n, n))
+            errs.append(np.abs(Z - ref).max() / np.abs(ref).max())
+        assert errs[2] < errs[1] < errs[0]
+        assert errs[1] < 5e-4 and errs[2] < 5e-6, errs
+
+
+# ------------------------------------------------------------- classifier
+
+def test_classifier_euler_counts():
+    for level in (1, 2, 3):
+        m = meshes.icosphere(level)
+        X = spaces.raviartthomas(m)
+        op = Maxwell3D.singlelayer(wavenumber=1.0)
+        cls = O.classify_all(op, X, X)
+        deg = m.vertex_degrees().astype(np.int64)
+        assert (cls == O.CF).sum() == m.nt
+        assert (cls == O.CE).sum() == 3 * m.nt
+        assert (cls == O.CV).sum() == int((deg ** 2).sum() - 9 * m.nt)
+        assert (cls == O.NEAR).sum() == 0
+
+
+@pytest.mark.slow
+def test_classifier_reference_meshcuboid_counts():
+    """Touching-pair counts the reference recorded for meshcuboid(1,1,1,0.5/24)
+    (data/GPU/Saut-Swab.txt:50-55): CV 248,784, CE 82,944, CF 27,648."""
+    m = meshes.meshcuboid(1.0, 1.0, 1.0, 0.5 / 24)
+    X = spaces.lagrangec0d1(m)
+    op = Helmholtz3D.singlelayer(wavenumber=1.0)
+    cls = O.classify_all(op, X, X)
+    assert (cls == O.CF).sum() == 27648
+    assert (cls == O.CE).sum() == 82944
+    assert (cls == O.CV).sum() == 248784
+    assert (cls == O.NEAR).sum() == 0
+
+
+def test_classifier_near_bruteforce():
+    """Two parallel, slightly offset copies of a coarse sphere give near pairs;
+    compare with a direct evaluation of max(d^2 k^2, d^2/(16 A)) < 0.2^2."""
+    a = meshes.icosphere(1)
+    b = meshes.Mesh(a.vertices * 1.02 + np.array([0.0, 0.0, 0.001]), a.faces)
+    X = spaces.lagrangec0d1(a)
+    Y = spaces.lagrangec0d1(b)
+    for k in (0.5, 4.0):
+        op = Helmholtz3D.singlelayer(wavenumber=k)
+        cls = O.classify_all(op, X, Y)
+        tv = a.vertices[a.faces]
+        sv = b.vertices[b.faces]
+        d2 = ((tv[:, None, :, None, :] - sv[None, :, None, :, :]) ** 2).sum(-1).reshape(a.nt, b.nt, 9).min(-1)
+        A = b.areas()[None, :]
+        near = np.maximum(d2 * k * k, d2 / (16 * A)) < 0.04
+        assert near.any()
+        assert np.array_equal(cls == O.NEAR, near)
+
+
+# ---------------------------------------------------------------- scatter
+
+def test_scatter_closed_form_constant_kernel():
+    """G = 1: Z_mn = alpha (int f_m).(int f_n) + beta (int div f_m)(int div f_n),
+    with int_K f_i = (c_K - v_i)/2 and int div f_i = 1 per cell."""
+    m = meshes.perturb_radial(meshes.icosphere(2), 9, 0.05)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.singlelayer(wavenumber=1.0)
+    Z, _ = O.assemble(op, X, X, kernel=O.KERNEL_ONE)
+    v = m.vertices[m.faces]
+    c = v.mean(axis=1)
+    u = np.zeros((X.nfun, 3))
+    dv = np.zeros(X.nfun)
+    for K in range(m.nt):
+        for i in range(3):
+            f = X.idx[K, i]
+            if f >= 0:
+                u[f] += X.coeff[K, i] * (c[K] - v[K, i]) / 2
+                dv[f] += X.coeff[K, i]
+    Zr = op.alpha * (u @ u.T) + op.beta * np.outer(dv, dv)
+    # the beta terms of the two cells of an edge cancel: scale the bound by |beta|
+    assert np.abs(Z - Zr).max() <= 1e-13 * (np.abs(Zr).max() + abs(op.beta))
+    assert np.abs(dv).max() < 1e-12  # RWG divergence integrates to zero
+
+
+def test_scatter_renumbering_invariance():
+    m = meshes.perturb_radial(meshes.icosphere(1), 2, 0.05)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.singlelayer(wavenumber=1.5)
+    Z, _ = O.assemble(op, X, X)
+    rng = np.random.default_rng(0)
+    pc = rng.permutation(m.nt)
+    pf = rng.permutation(X.nfun)
+    m2 = meshes.Mesh(m.vertices, m.faces[pc])
+    inv = np.empty_like(pf)
+    inv[pf] = np.arange(X.nfun)
+    idx2 = np.where(X.idx[pc] >= 0, inv[np.maximum(X.idx[pc], 0)], -1)
+    X2 = spaces.Space(m2, spaces.RWG, idx2, X.coeff[pc], X.nfun)
+    Z2, _ = O.assemble(op, X2, X2)
+    assert np.abs(Z2 - Z[np.ix_(pf, pf)]).max() <= 1e-14 * np.abs(Z).max()
+
+
+def test_rows_subset_matches_full():
+    m = meshes.icosphere(2)
+    X = spaces.lagrangec0d1(m)
+    op = Helmholtz3D.singlelayer(wavenumber=2.0)
+    Z, _ = O.assemble(op, X, X)
+    rows = np.array([3, 77, 12, 150])
+    Zr, _ = O.assemble(op, X, X, rows=rows)
+    # same terms; the colouring (hence the summation order) differs
+    assert np.abs(Zr - Z[rows]).max() <= 1e-15 * np.abs(Z).max()
