@@ -28,6 +28,7 @@ EXPORTS = (
     "bmi_plan_create", "bmi_plan_free",
     "bmi_matrix_create", "bmi_matrix_free", "bmi_matrix_device_ptr",
     "bmi_assemble", "bmi_plan_stats", "bmi_matrix_to_host",
+    "bmi_matrix_apply", "bmi_matrix_compare",
 )
 
 
@@ -114,6 +115,8 @@ def lib():
     L.bmi_assemble.argtypes = [vp, vp]
     L.bmi_plan_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.bmi_matrix_to_host.argtypes = [vp, vp, i32]
+    L.bmi_matrix_apply.argtypes = [vp, vp, vp, i32]
+    L.bmi_matrix_compare.argtypes = [vp, vp, vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("bmi_version", "bmi_last_error", "bmi_options_default"):

@@ -158,6 +158,23 @@ class DeviceMatrix(_Handle):
         _bmi.check(_bmi.lib().bmi_matrix_device_ptr(self.h, ctypes.byref(p), ctypes.byref(ld)))
         return p.value
 
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        """y = Z x on the device; x is (cols,) or (cols, nrhs) with nrhs <= 8."""
+        x = np.asarray(x, dtype=np.complex128)
+        vec = x.ndim == 1
+        xf = np.asfortranarray(x.reshape(self.cols, -1))
+        nrhs = xf.shape[1]
+        y = np.empty((self.rows, nrhs), dtype=np.complex128, order="F")
+        _bmi.check(_bmi.lib().bmi_matrix_apply(self.h, xf.ctypes.data, y.ctypes.data, nrhs))
+        return y[:, 0] if vec else y
+
+    def compare(self, ref: "DeviceMatrix") -> dict:
+        """||self - ref||_F, ||ref||_F, max |self - ref| computed on the device."""
+        out = (ctypes.c_double * 3)()
+        _bmi.check(_bmi.lib().bmi_matrix_compare(self.h, ref.h, out))
+        return {"diff_fro": out[0], "ref_fro": out[1], "max_abs": out[2],
+                "rel_fro": out[0] / out[1] if out[1] > 0 else float("nan")}
+
     def to_host(self, dtype: int | None = None) -> np.ndarray:
         dtype = self.dtype if dtype is None else dtype
         out = np.empty((self.rows, self.cols), dtype=np.complex128 if dtype == _bmi.C64 else np.complex64, order="F")

@@ -60,23 +60,6 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def rel_fro(A, B, cols=2048):
-    num = den = 0.0
-    for j in range(0, B.shape[1], cols):
-        a = A[:, j:j + cols].astype(np.complex128)
-        b = B[:, j:j + cols]
-        num += float(np.linalg.norm(a - b) ** 2)
-        den += float(np.linalg.norm(b) ** 2)
-    return (num / den) ** 0.5
-
-
-def matvec(A, x, cols=2048):
-    y = np.zeros(A.shape[0], dtype=np.complex128)
-    for j in range(0, A.shape[1], cols):
-        y += A[:, j:j + cols].astype(np.complex128) @ x[j:j + cols]
-    return y
-
-
 def time_steps(asm, steps, warmup):
     for _ in range(warmup):
         asm.assemble()
@@ -197,29 +180,26 @@ def main():
     }
 
     if not args.no_mixed:
-        Z64 = asm.matrix.to_host()
         rng = np.random.default_rng(33)
         x = np.sqrt(rng.uniform(size=n)) * np.exp(2j * np.pi * rng.uniform(size=n))
-        y64 = matvec(Z64, x)
+        y64 = asm.matrix.apply(x)
         mixed = {}
         for out in ("c32", "c64"):
             am = Assembler(op, X, X, GPUConfiguration(precision="mixed", output=out), ctx=ctx)
             tm, fm, sm, cm, stm = time_steps(am, args.steps, args.warmup)
-            Zm = am.matrix.to_host()
-            ym = matvec(Zm, x)
+            cmp = am.matrix.compare(asm.matrix)
+            ym = am.matrix.apply(x)
             mixed[out] = {
                 "value": entries / tm,
                 "ms_per_step": tm * 1e3,
                 "speedup_vs_fp64": t_step / tm,
-                "rel_fro_err_vs_fp64": rel_fro(Zm, Z64),
+                "rel_fro_err_vs_fp64": cmp["rel_fro"],
                 "rel_zx_err_vs_fp64": float(np.linalg.norm(ym - y64) / np.linalg.norm(y64)),
                 "phases_ms": {"classify": cm, "special": sm, "far_blocks": fm},
                 "far_fp64_pairs": stm["pairs"]["far_fp64"],
             }
             print(f"[bench] mixed {out}: {mixed[out]}", file=sys.stderr, flush=True)
-            del Zm
             am.close()
-        del Z64
         result["mixed"] = mixed
 
     asm.close()

@@ -155,6 +155,19 @@ bmi_status bmi_plan_stats(const bmi_plan *plan, bmi_stats *stats);
  * Replaces copy_to_CPU (utils/copy_to_CPU.jl:5-61). */
 bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype);
 
+/* y = Z x for nrhs (1..8) right-hand sides on the device-resident matrix: x is
+ * cols x nrhs, y is rows x nrhs, both caller-owned host ComplexF64 arrays in
+ * column-major order; products and sums in FP64, deterministic order.
+ * Replaces the host-side application of the returned Matrix{ComplexF64}
+ * (the solver's Z*x after assemble_gpu, assamble_gpu.jl:135), without the
+ * 15 GB device-to-host copy. */
+bmi_status bmi_matrix_apply(const bmi_matrix *m, const double *x, double *y, int32_t nrhs);
+
+/* out[0] = ||a - ref||_F, out[1] = ||ref||_F, out[2] = max |a_ij - ref_ij|, computed
+ * on the device (a and ref of equal shape, any element types).
+ * Replaces test(M, M_ref) of test/difference.jl:1-6, which compared one column. */
+bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif
