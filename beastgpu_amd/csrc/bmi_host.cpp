@@ -11,6 +11,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -131,6 +132,9 @@ uint64_t spread3(uint64_t v) {
 struct bmi_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // pinned staging ring for device-to-host copies of matrices
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 struct bmi_mesh {
@@ -149,13 +153,27 @@ struct bmi_space {
     int ncolour = 0;
     bool uncovered = false;  // some function has no local shape
     std::vector<int> patch_tri, row_ptr, row_fn, row_slot_ptr, patch_colour;
+    std::vector<double> patch_sph, patch_amax;
     std::vector<unsigned char> row_first, slot_kl;
     std::vector<double> slot_cf;
     std::vector<std::vector<int>> by_colour;
     DevBuf<int> d_patch_tri, d_row_ptr, d_row_fn, d_row_slot_ptr, d_by_colour;
     DevBuf<unsigned char> d_row_first, d_slot_kl;
-    DevBuf<double> d_slot_cf;
+    DevBuf<double> d_slot_cf, d_patch_sph, d_patch_amax;
     std::vector<int> colour_off;  // offsets of each colour's list in d_by_colour
+    // test role: panels of BMI_PANEL_ROWS functions in sub-batches of BMI_PATCH cells
+    int npanel = 0, nsb = 0;
+    std::vector<int> sb_off, sb_cells, sb_row_ptr;
+    std::vector<unsigned char> sb_slot_kl;
+    std::vector<double> sb_slot_cf, sb_sph, sb_amax;
+    DevBuf<int> d_sb_off, d_sb_cells;
+    DevBuf<double> d_sb_sph, d_sb_amax;
+    // fixed-size records: test role (per sub-batch) and trial role (per patch)
+    DevBuf<int> d_trs, d_bnc, d_bfn, d_bcs, d_tslot, d_bslot;
+    DevBuf<double> d_tscf, d_bscf;
+    int rounds_t = 1, rounds_b = 1, ncmax = 1;
+    DevBuf<unsigned char> d_tkl, d_bfirst, d_bkl;
+    DevBuf<double> d_tcf, d_bcf;
 };
 
 struct bmi_matrix {
@@ -173,7 +191,7 @@ struct bmi_plan {
     int op_kind = 0;
     OpConst opc{};
     RuleConst rule{};
-    DevBuf<double> d_xpts, d_ypts;
+    DevBuf<double> d_xpts, d_ypts, d_tgeo, d_bgeo;
     int npa = 0, npb = 0;
     long long nblocks = 0;
     DevBuf<int> d_blk_count, d_blk_off, d_partials, d_total;
@@ -199,6 +217,8 @@ DevSpace dev_view(const bmi_space *s, const double *pts) {
     d.area = s->mesh->d_area.p;
     d.pts = pts;
     d.patch_tri = s->d_patch_tri.p;
+    d.patch_sph = s->d_patch_sph.p;
+    d.patch_amax = s->d_patch_amax.p;
     d.row_ptr = s->d_row_ptr.p;
     d.row_fn = s->d_row_fn.p;
     d.row_first = s->d_row_first.p;
@@ -208,11 +228,188 @@ DevSpace dev_view(const bmi_space *s, const double *pts) {
     return d;
 }
 
+// The test role of a space: cells grouped by panel sub-batch.
+DevSpace dev_view_test(const bmi_space *s, const double *pts) {
+    DevSpace d = dev_view(s, pts);
+    d.npatch = s->nsb;
+    d.patch_tri = s->d_sb_cells.p;
+    d.patch_sph = s->d_sb_sph.p;
+    d.patch_amax = s->d_sb_amax.p;
+    d.row_ptr = nullptr;
+    d.row_fn = nullptr;
+    d.row_first = nullptr;
+    d.row_slot_ptr = nullptr;
+    d.slot_kl = nullptr;
+    d.slot_cf = nullptr;
+    return d;
+}
+
+// Bounding sphere (centre of the vertex bounding box) and largest area of a
+// group of cells.
+void group_sphere(const bmi_mesh *m, const int *cells, int n, double *sph, double *amax) {
+    double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
+    *amax = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const int K = cells[k];
+        if (K < 0) continue;
+        *amax = std::max(*amax, m->area[K]);
+        for (int i = 0; i < 3; ++i)
+            for (int d = 0; d < 3; ++d) {
+                blo[d] = std::min(blo[d], m->vtx[9 * (size_t)K + 3 * i + d]);
+                bhi[d] = std::max(bhi[d], m->vtx[9 * (size_t)K + 3 * i + d]);
+            }
+    }
+    double r2 = 0.0;
+    for (int d = 0; d < 3; ++d) sph[d] = 0.5 * (blo[d] + bhi[d]);
+    for (int k = 0; k < n; ++k) {
+        const int K = cells[k];
+        if (K < 0) continue;
+        for (int i = 0; i < 3; ++i) {
+            double q2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const double x = m->vtx[9 * (size_t)K + 3 * i + d] - sph[d];
+                q2 += x * x;
+            }
+            r2 = std::max(r2, q2);
+        }
+    }
+    sph[3] = std::sqrt(r2) * (1.0 + 1e-12);
+}
+
+// Panels: functions [64 P, 64 P + 64) and every cell carrying one of them,
+// ordered by Morton rank, cut into sub-batches of BMI_PATCH cells.  For each
+// sub-batch and each of the 64 rows: the slots (local cell * 3 + shape,
+// coefficient) of that row's function on the sub-batch's cells.
+bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, const std::vector<int> &morton_rank) {
+    const bmi_mesh *m = s->mesh;
+    const int64_t nt = m->nt;
+    const int R = BMI_PANEL_ROWS;
+    const int np = (int)((s->nfun + R - 1) / R);
+    s->npanel = np;
+    std::vector<int> cnt(np + 1, 0);
+    for (int64_t K = 0; K < nt; ++K) {
+        int seen[3] = {-1, -1, -1};
+        for (int i = 0; i < 3; ++i) {
+            const int64_t f = gidx[3 * K + i];
+            if (f < 0) continue;
+            const int P = (int)(f / R);
+            if (P != seen[0] && P != seen[1] && P != seen[2]) {
+                seen[i] = P;
+                cnt[P + 1]++;
+            }
+        }
+    }
+    for (int P = 0; P < np; ++P) cnt[P + 1] += cnt[P];
+    std::vector<int> cells(cnt[np]), pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t K = 0; K < nt; ++K) {
+        int seen[3] = {-1, -1, -1};
+        for (int i = 0; i < 3; ++i) {
+            const int64_t f = gidx[3 * K + i];
+            if (f < 0) continue;
+            const int P = (int)(f / R);
+            if (P != seen[0] && P != seen[1] && P != seen[2]) {
+                seen[i] = P;
+                cells[pos[P]++] = (int)K;
+            }
+        }
+    }
+    s->sb_off.assign(np + 1, 0);
+    s->sb_cells.clear();
+    s->sb_row_ptr.clear();
+    s->sb_slot_kl.clear();
+    s->sb_slot_cf.clear();
+    struct Slot { int r, kl; double cf; };
+    std::vector<Slot> sl;
+    for (int P = 0; P < np; ++P) {
+        std::sort(cells.begin() + cnt[P], cells.begin() + cnt[P + 1],
+                  [&](int a, int b) { return morton_rank[a] < morton_rank[b]; });
+        const int nc = cnt[P + 1] - cnt[P];
+        const int nsb = (nc + BMI_PATCH - 1) / BMI_PATCH;
+        for (int b = 0; b < nsb; ++b) {
+            sl.clear();
+            for (int k = 0; k < BMI_PATCH; ++k) {
+                const int idx = b * BMI_PATCH + k;
+                const int K = idx < nc ? cells[cnt[P] + idx] : -1;
+                s->sb_cells.push_back(K);
+                if (K < 0) continue;
+                for (int i = 0; i < 3; ++i) {
+                    const int64_t f = gidx[3 * (int64_t)K + i];
+                    if (f < 0 || f / R != P) continue;
+                    sl.push_back({(int)(f - (int64_t)P * R), 3 * k + i, coeff[3 * (int64_t)K + i]});
+                }
+            }
+            std::stable_sort(sl.begin(), sl.end(), [](const Slot &x, const Slot &y) {
+                return x.r != y.r ? x.r < y.r : x.kl < y.kl;
+            });
+            size_t u = 0;
+            for (int r = 0; r <= R; ++r) {
+                s->sb_row_ptr.push_back((int)s->sb_slot_kl.size());
+                while (u < sl.size() && sl[u].r == r) {
+                    s->sb_slot_kl.push_back((unsigned char)sl[u].kl);
+                    s->sb_slot_cf.push_back(sl[u].cf);
+                    ++u;
+                }
+            }
+        }
+        s->sb_off[P + 1] = s->sb_off[P] + nsb;
+    }
+    s->nsb = s->sb_off[np];
+    s->sb_sph.assign((size_t)s->nsb * 4, 0.0);
+    s->sb_amax.assign(s->nsb, 0.0);
+    for (int b = 0; b < s->nsb; ++b) group_sphere(m, &s->sb_cells[(size_t)b * BMI_PATCH], BMI_PATCH, &s->sb_sph[4 * b], &s->sb_amax[b]);
+    // fixed-size slot records per sub-batch
+    std::vector<int> trs((size_t)s->nsb * (R + 1), 0);
+    std::vector<unsigned char> tkl((size_t)s->nsb * BMI_MAX_ROWS, 0);
+    std::vector<double> tcf((size_t)s->nsb * BMI_MAX_ROWS, 0.0);
+    for (int b = 0; b < s->nsb; ++b) {
+        const int *rp = &s->sb_row_ptr[(size_t)b * (R + 1)];
+        const int base = rp[0];
+        if (rp[R] - base > BMI_MAX_ROWS) return fail(BMI_EINVAL, "more than 48 slots in a sub-batch");
+        for (int r = 0; r <= R; ++r) trs[(size_t)b * (R + 1) + r] = rp[r] - base;
+        for (int u = base; u < rp[R]; ++u) {
+            tkl[(size_t)b * BMI_MAX_ROWS + (u - base)] = s->sb_slot_kl[u];
+            tcf[(size_t)b * BMI_MAX_ROWS + (u - base)] = s->sb_slot_cf[u];
+        }
+    }
+    // rank-round records: row of each cell slot in its panel and its rank
+    // among the sub-batch's slots of that row
+    std::vector<int> tslot((size_t)s->nsb * BMI_MAX_ROWS, 0xff);
+    std::vector<double> tscf((size_t)s->nsb * BMI_MAX_ROWS, 0.0);
+    s->rounds_t = 1;
+    for (int P = 0; P < np; ++P)
+        for (int b = s->sb_off[P]; b < s->sb_off[P + 1]; ++b) {
+            int seen[BMI_PANEL_ROWS] = {0};
+            for (int k = 0; k < BMI_PATCH; ++k) {
+                const int K = s->sb_cells[(size_t)b * BMI_PATCH + k];
+                if (K < 0) continue;
+                for (int i = 0; i < 3; ++i) {
+                    const int64_t f = gidx[3 * (int64_t)K + i];
+                    if (f < 0 || f / R != P) continue;
+                    const int r = (int)(f - (int64_t)P * R);
+                    const int rank = seen[r]++;
+                    if (rank > 255) return fail(BMI_EUNSUPPORTED, "function with more than 256 cells");
+                    tslot[(size_t)b * BMI_MAX_ROWS + 3 * k + i] = r | (rank << 8);
+                    tscf[(size_t)b * BMI_MAX_ROWS + 3 * k + i] = coeff[3 * (int64_t)K + i];
+                    s->rounds_t = std::max(s->rounds_t, rank + 1);
+                }
+            }
+        }
+    hipError_t e;
+    if ((e = s->d_tslot.upload(tslot)) != hipSuccess || (e = s->d_tscf.upload(tscf)) != hipSuccess)
+        return fail(BMI_EDEVICE, std::string("panel upload: ") + hipGetErrorString(e));
+    if ((e = s->d_sb_off.upload(s->sb_off)) != hipSuccess || (e = s->d_sb_cells.upload(s->sb_cells)) != hipSuccess ||
+        (e = s->d_sb_sph.upload(s->sb_sph)) != hipSuccess || (e = s->d_sb_amax.upload(s->sb_amax)) != hipSuccess ||
+        (e = s->d_trs.upload(trs)) != hipSuccess || (e = s->d_tkl.upload(tkl)) != hipSuccess ||
+        (e = s->d_tcf.upload(tcf)) != hipSuccess)
+        return fail(BMI_EDEVICE, std::string("panel upload: ") + hipGetErrorString(e));
+    return BMI_OK;
+}
+
 // Patches: runs of BMI_PATCH cells in Morton order of their centroids.
 // Rows of a patch: the functions its cells carry, ascending.  Patches that
 // share a function get different colours (greedy); the first-touch flag of a
 // (patch, function) row is set on the patch with the smallest colour.
-bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff) {
+bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff, std::vector<int> &morton_rank) {
     const bmi_mesh *m = s->mesh;
     const int64_t nt = m->nt;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -236,11 +433,18 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff)
     std::vector<int> order(nt);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    morton_rank.assign(nt, 0);
+    for (int64_t i = 0; i < nt; ++i) morton_rank[order[i]] = (int)i;
 
     const int np = (int)((nt + BMI_PATCH - 1) / BMI_PATCH);
     s->npatch = np;
     s->patch_tri.assign((size_t)np * BMI_PATCH, -1);
     for (int64_t i = 0; i < nt; ++i) s->patch_tri[i] = order[i];
+
+    s->patch_sph.assign((size_t)np * 4, 0.0);
+    s->patch_amax.assign(np, 0.0);
+    for (int p = 0; p < np; ++p)
+        group_sphere(m, &s->patch_tri[(size_t)p * BMI_PATCH], BMI_PATCH, &s->patch_sph[4 * p], &s->patch_amax[p]);
 
     s->row_ptr.assign(np + 1, 0);
     s->row_fn.clear();
@@ -330,9 +534,79 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff)
         (e = s->d_row_slot_ptr.upload(s->row_slot_ptr)) != hipSuccess ||
         (e = s->d_row_first.upload(s->row_first)) != hipSuccess ||
         (e = s->d_slot_kl.upload(s->slot_kl)) != hipSuccess || (e = s->d_slot_cf.upload(s->slot_cf)) != hipSuccess ||
-        (e = s->d_by_colour.upload(flat)) != hipSuccess)
+        (e = s->d_by_colour.upload(flat)) != hipSuccess || (e = s->d_patch_sph.upload(s->patch_sph)) != hipSuccess ||
+        (e = s->d_patch_amax.upload(s->patch_amax)) != hipSuccess)
+        return fail(BMI_EDEVICE, std::string("space upload: ") + hipGetErrorString(e));
+    // fixed-size column records per patch
+    std::vector<int> bnc(np, 0), bfn((size_t)np * BMI_MAX_ROWS, -1), bcs((size_t)np * (BMI_MAX_ROWS + 1), 0);
+    std::vector<unsigned char> bfirst((size_t)np * BMI_MAX_ROWS, 0), bkl((size_t)np * BMI_MAX_ROWS, 0);
+    std::vector<double> bcf((size_t)np * BMI_MAX_ROWS, 0.0);
+    for (int p = 0; p < np; ++p) {
+        const int r0 = s->row_ptr[p], nr = s->row_ptr[p + 1] - r0;
+        const int base = s->row_slot_ptr[r0];
+        bnc[p] = nr;
+        for (int r = 0; r < nr; ++r) {
+            bfn[(size_t)p * BMI_MAX_ROWS + r] = s->row_fn[r0 + r];
+            bfirst[(size_t)p * BMI_MAX_ROWS + r] = s->row_first[r0 + r];
+            bcs[(size_t)p * (BMI_MAX_ROWS + 1) + r] = s->row_slot_ptr[r0 + r] - base;
+        }
+        for (int r = nr; r <= BMI_MAX_ROWS; ++r)
+            bcs[(size_t)p * (BMI_MAX_ROWS + 1) + r] = s->row_slot_ptr[r0 + nr] - base;
+        for (int u = base; u < s->row_slot_ptr[r0 + nr]; ++u) {
+            bkl[(size_t)p * BMI_MAX_ROWS + (u - base)] = s->slot_kl[u];
+            bcf[(size_t)p * BMI_MAX_ROWS + (u - base)] = s->slot_cf[u];
+        }
+    }
+    // rank-round records: column of each cell slot in its patch and its rank
+    std::vector<int> bslot((size_t)np * BMI_MAX_ROWS, 0xff);
+    std::vector<double> bscf((size_t)np * BMI_MAX_ROWS, 0.0);
+    s->rounds_b = 1;
+    s->ncmax = 1;
+    for (int p = 0; p < np; ++p) {
+        const int r0 = s->row_ptr[p], nr = s->row_ptr[p + 1] - r0;
+        s->ncmax = std::max(s->ncmax, nr);
+        int seen[BMI_MAX_ROWS] = {0};
+        for (int l = 0; l < BMI_PATCH; ++l) {
+            const int L = s->patch_tri[(size_t)p * BMI_PATCH + l];
+            if (L < 0) continue;
+            for (int j = 0; j < 3; ++j) {
+                const int64_t f = gidx[3 * (int64_t)L + j];
+                if (f < 0) continue;
+                const int c = (int)(std::lower_bound(s->row_fn.begin() + r0, s->row_fn.begin() + r0 + nr, (int)f) -
+                                    (s->row_fn.begin() + r0));
+                const int rank = seen[c]++;
+                bslot[(size_t)p * BMI_MAX_ROWS + 3 * l + j] = c | (rank << 8);
+                bscf[(size_t)p * BMI_MAX_ROWS + 3 * l + j] = coeff[3 * (int64_t)L + j];
+                s->rounds_b = std::max(s->rounds_b, rank + 1);
+            }
+        }
+    }
+    if ((e = s->d_bslot.upload(bslot)) != hipSuccess || (e = s->d_bscf.upload(bscf)) != hipSuccess)
+        return fail(BMI_EDEVICE, std::string("space upload: ") + hipGetErrorString(e));
+    if ((e = s->d_bnc.upload(bnc)) != hipSuccess || (e = s->d_bfn.upload(bfn)) != hipSuccess ||
+        (e = s->d_bcs.upload(bcs)) != hipSuccess || (e = s->d_bfirst.upload(bfirst)) != hipSuccess ||
+        (e = s->d_bkl.upload(bkl)) != hipSuccess || (e = s->d_bcf.upload(bcf)) != hipSuccess)
         return fail(BMI_EDEVICE, std::string("space upload: ") + hipGetErrorString(e));
     return BMI_OK;
+}
+
+// Packed geometry (BMI_GEO doubles) of groups of BMI_PATCH cells.
+std::vector<double> geo_records(const bmi_mesh *m, const std::vector<double> &pts, const std::vector<int> &cells,
+                                int ngroups) {
+    std::vector<double> g((size_t)ngroups * BMI_GEO, 0.0);
+    for (int b = 0; b < ngroups; ++b) {
+        double *r = &g[(size_t)b * BMI_GEO];
+        for (int c = 0; c < BMI_PATCH; ++c) {
+            const int K = cells[(size_t)b * BMI_PATCH + c];
+            if (K < 0) continue;
+            for (int e = 0; e < 3 * BMI_MAX_RULE; ++e)
+                r[BMI_GEO_PTS + 3 * BMI_MAX_RULE * c + e] = pts[(size_t)K * 3 * BMI_MAX_RULE + e];
+            for (int d = 0; d < 3; ++d) r[BMI_GEO_CEN + 3 * c + d] = m->cen[3 * (size_t)K + d];
+            r[BMI_GEO_AREA + c] = m->area[K];
+            for (int e = 0; e < 9; ++e) r[BMI_GEO_VREL + 9 * c + e] = m->vtx[9 * (size_t)K + e] - m->cen[3 * (size_t)K + e % 3];
+        }
+    }
+    return g;
 }
 
 std::vector<double> rule_points(const bmi_mesh *m, int n, const double l[BMI_MAX_RULE][3]) {
@@ -383,6 +657,10 @@ bmi_status bmi_context_create(int32_t device, bmi_context **out) {
 bmi_status bmi_context_destroy(bmi_context *ctx) {
     if (!ctx) return BMI_OK;
     (void)hipSetDevice(ctx->device);
+    for (int i = 0; i < 2; ++i) {
+        if (ctx->stage[i]) (void)hipHostFree(ctx->stage[i]);
+        if (ctx->stage_ev[i]) (void)hipEventDestroy(ctx->stage_ev[i]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BMI_OK;
@@ -457,7 +735,10 @@ bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, c
     s->mesh = mesh;
     s->kind = kind;
     s->nfun = nfun;
-    bmi_status st = build_patches(s.get(), gidx, coeff);
+    std::vector<int> rank;
+    bmi_status st = build_patches(s.get(), gidx, coeff, rank);
+    if (st != BMI_OK) return st;
+    st = build_panels(s.get(), gidx, coeff, rank);
     if (st != BMI_OK) return st;
     *out = s.release();
     return BMI_OK;
@@ -565,11 +846,15 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
         for (int J = 0; J < BMI_MAX_RULE; ++J) R.w4pi[I][J] = wt[I] * wb[J] / (4.0 * kPi);
 
     hipError_t e;
-    if ((e = P->d_xpts.upload(rule_points(test->mesh, nt, lt))) != hipSuccess ||
-        (e = P->d_ypts.upload(rule_points(trial->mesh, nb, lb))) != hipSuccess)
-        return fail(BMI_EDEVICE, std::string("rule points: ") + hipGetErrorString(e));
+    {
+        const std::vector<double> xp = rule_points(test->mesh, nt, lt), yp = rule_points(trial->mesh, nb, lb);
+        if ((e = P->d_xpts.upload(xp)) != hipSuccess || (e = P->d_ypts.upload(yp)) != hipSuccess ||
+            (e = P->d_tgeo.upload(geo_records(test->mesh, xp, test->sb_cells, test->nsb))) != hipSuccess ||
+            (e = P->d_bgeo.upload(geo_records(trial->mesh, yp, trial->patch_tri, trial->npatch))) != hipSuccess)
+            return fail(BMI_EDEVICE, std::string("rule points: ") + hipGetErrorString(e));
+    }
 
-    P->npa = test->npatch;
+    P->npa = test->nsb;
     P->npb = trial->npatch;
     P->nblocks = (long long)P->npa * P->npb;
     if (P->nblocks >= (1LL << 31) - 1) return fail(BMI_EUNSUPPORTED, "too many patch blocks");
@@ -580,13 +865,14 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
         return fail(BMI_ENOMEM, std::string("plan buffers: ") + hipGetErrorString(e));
 
     ClassifyArgs &C = P->cargs;
-    C.X = dev_view(test, P->d_xpts.p);
+    C.X = dev_view_test(test, P->d_xpts.p);
     C.Y = dev_view(trial, P->d_ypts.p);
     C.npa = P->npa;
     C.npb = P->npb;
     C.k2x16 = 16.0 * (op->gamma[0] * op->gamma[0] + op->gamma[1] * op->gamma[1]);
     const double h = std::max(test->mesh->hmax, trial->mesh->hmax);
     C.near_d2 = o.precision == BMI_PREC_MIXED ? (o.near_factor * h) * (o.near_factor * h) : -1.0;
+    C.near_d = o.precision == BMI_PREC_MIXED ? o.near_factor * h : -1.0;
     C.blk_count = P->d_blk_count.p;
     C.blk_off = P->d_blk_off.p;
 
@@ -609,10 +895,23 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     C.spec_loc = P->d_spec_loc.p;
     C.capacity = total;
     BMI_HIP(launch_classify(C, 1, s));
+    // pair statistics: a cell of several panels is classified once per panel,
+    // so count each (test cell, trial cell) pair once
     std::vector<unsigned char> cls(cap, 0);
-    if (total > 0) BMI_HIP(hipMemcpyAsync(cls.data(), P->d_spec_cls.p, total, hipMemcpyDeviceToHost, s));
+    std::vector<int> sk(cap, 0), sl(cap, 0);
+    if (total > 0) {
+        BMI_HIP(hipMemcpyAsync(cls.data(), P->d_spec_cls.p, total, hipMemcpyDeviceToHost, s));
+        BMI_HIP(hipMemcpyAsync(sk.data(), P->d_spec_K.p, sizeof(int) * total, hipMemcpyDeviceToHost, s));
+        BMI_HIP(hipMemcpyAsync(sl.data(), P->d_spec_L.p, sizeof(int) * total, hipMemcpyDeviceToHost, s));
+    }
     BMI_HIP(hipStreamSynchronize(s));
-    for (int i = 0; i < total; ++i) P->pairs[cls[i] < 6 ? cls[i] : 0]++;
+    {
+        std::vector<std::pair<int64_t, unsigned char>> keys(total);
+        for (int i = 0; i < total; ++i) keys[i] = {(int64_t)sk[i] * trial->mesh->nt + sl[i], cls[i]};
+        std::sort(keys.begin(), keys.end());
+        for (int i = 0; i < total; ++i)
+            if (i == 0 || keys[i].first != keys[i - 1].first) P->pairs[keys[i].second < 6 ? keys[i].second : 0]++;
+    }
     int64_t special = 0;
     for (int c = 1; c < 6; ++c) special += P->pairs[c];
     P->pairs[0] = test->mesh->nt * trial->mesh->nt - special;
@@ -682,9 +981,28 @@ bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
     if (P->X->uncovered || P->Y->uncovered)
         BMI_HIP(hipMemsetAsync(out->d, 0, (size_t)out->rows * out->cols * (out->dtype == BMI_C64 ? 16 : 8), s));
 
-    BlockArgs B{};
-    B.X = P->cargs.X;
-    B.Y = P->cargs.Y;
+    PanelArgs B{};
+    B.npanel = P->X->npanel;
+    B.nrows = (int)P->X->nfun;
+    B.sb_off = P->X->d_sb_off.p;
+    B.tgeo = P->d_tgeo.p;
+    B.trs = P->X->d_trs.p;
+    B.tkl = P->X->d_tkl.p;
+    B.tcf = P->X->d_tcf.p;
+    B.bgeo = P->d_bgeo.p;
+    B.bnc = P->Y->d_bnc.p;
+    B.bfn = P->Y->d_bfn.p;
+    B.bfirst = P->Y->d_bfirst.p;
+    B.bcs = P->Y->d_bcs.p;
+    B.bkl = P->Y->d_bkl.p;
+    B.bcf = P->Y->d_bcf.p;
+    B.tslot = P->X->d_tslot.p;
+    B.tscf = P->X->d_tscf.p;
+    B.bslot = P->Y->d_bslot.p;
+    B.bscf = P->Y->d_bscf.p;
+    B.rounds_t = P->X->rounds_t;
+    B.rounds_b = P->Y->rounds_b;
+    B.ncm = P->Y->ncmax;
     B.npb = P->npb;
     B.blk_off = P->d_blk_off.p;
     B.spec_loc = P->d_spec_loc.p;
@@ -695,15 +1013,11 @@ bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
     B.op = P->opc;
     B.rule = P->rule;
     int launches = 0;
-    for (int ct = 0; ct < P->X->ncolour; ++ct) {
-        const int ntp = P->X->colour_off[ct + 1] - P->X->colour_off[ct];
-        for (int cb = 0; cb < P->Y->ncolour; ++cb) {
-            B.tp_list = P->X->d_by_colour.p + P->X->colour_off[ct];
-            B.bq_list = P->Y->d_by_colour.p + P->Y->colour_off[cb];
-            B.nbq = P->Y->colour_off[cb + 1] - P->Y->colour_off[cb];
-            BMI_HIP(launch_blocks(B, ntp, P->op_kind, mixed, out->dtype == BMI_C32, s));
-            ++launches;
-        }
+    for (int cb = 0; cb < P->Y->ncolour; ++cb) {
+        B.bq_list = P->Y->d_by_colour.p + P->Y->colour_off[cb];
+        B.nbq = P->Y->colour_off[cb + 1] - P->Y->colour_off[cb];
+        BMI_HIP(launch_panels(B, P->op_kind, mixed, out->dtype == BMI_C32, s));
+        ++launches;
     }
     BMI_HIP(hipEventRecord(P->ev[3], s));
     BMI_HIP(hipStreamSynchronize(s));
@@ -718,7 +1032,7 @@ bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
     P->last.total_ms = t03;
     for (int i = 0; i < 6; ++i) P->last.pairs[i] = P->pairs[i];
     P->last.far_launches = launches;
-    P->last.n_colours_test = P->X->ncolour;
+    P->last.n_colours_test = 1;
     P->last.n_colours_trial = P->Y->ncolour;
     return BMI_OK;
 }
@@ -727,36 +1041,120 @@ bmi_status bmi_plan_stats(const bmi_plan *P, bmi_stats *st) {
     if (!P || !st) return fail(BMI_EINVAL, "NULL argument");
     *st = P->last;
     for (int i = 0; i < 6; ++i) st->pairs[i] = P->pairs[i];
-    st->n_colours_test = P->X->ncolour;
+    st->n_colours_test = 1;
     st->n_colours_trial = P->Y->ncolour;
     return BMI_OK;
+}
+
+// Host copy of a source chunk into the destination, converting the element
+// type if needed, split over a few host threads.
+static void consume_chunk(const void *src, int src32, void *dst, int dst32, size_t first, size_t count) {
+    const unsigned nth = count > (1u << 20) ? 4u : 1u;
+    auto work = [&](size_t a, size_t b) {
+        if (src32 == dst32) {
+            const size_t es = src32 ? 8 : 16;
+            std::memcpy((unsigned char *)dst + (first + a) * es, (const unsigned char *)src + a * es, (b - a) * es);
+        } else if (!src32) {
+            const double *s = (const double *)src;
+            float *d = (float *)dst + 2 * first;
+            for (size_t i = 2 * a; i < 2 * b; ++i) d[i] = (float)s[i];
+        } else {
+            const float *s = (const float *)src;
+            double *d = (double *)dst + 2 * first;
+            for (size_t i = 2 * a; i < 2 * b; ++i) d[i] = (double)s[i];
+        }
+    };
+    if (nth == 1) {
+        work(0, count);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (count + nth - 1) / nth;
+    for (unsigned t = 0; t < nth; ++t) {
+        const size_t a = t * per, b = std::min(count, a + per);
+        if (a < b) th.emplace_back(work, a, b);
+    }
+    for (auto &x : th) x.join();
 }
 
 bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype) {
     if (!m || !dst) return fail(BMI_EINVAL, "NULL argument");
     if (dtype != BMI_C64 && dtype != BMI_C32) return fail(BMI_EINVAL, "bad dtype");
-    BMI_HIP(hipSetDevice(m->ctx->device));
+    bmi_context *ctx = m->ctx;
+    BMI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t kStage = 64u << 20;  // bytes per pinned staging buffer
+    for (int i = 0; i < 2; ++i) {
+        if (!ctx->stage[i]) BMI_HIP(hipHostMalloc(&ctx->stage[i], kStage, hipHostMallocDefault));
+        if (!ctx->stage_ev[i]) BMI_HIP(hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
+    }
+    const int src32 = m->dtype == BMI_C32;
+    const size_t es = src32 ? 8 : 16;
     const size_t n = (size_t)m->rows * (size_t)m->cols;
-    if (dtype == m->dtype) {
-        BMI_HIP(hipMemcpy(dst, m->d, n * (dtype == BMI_C64 ? 16 : 8), hipMemcpyDeviceToHost));
-        return BMI_OK;
+    const size_t ce = kStage / es;
+    const size_t nk = (n + ce - 1) / ce;
+    auto issue = [&](size_t k) -> hipError_t {
+        const size_t o = k * ce, c = std::min(ce, n - o);
+        hipError_t e = hipMemcpyAsync(ctx->stage[k & 1], (const unsigned char *)m->d + o * es, c * es,
+                                      hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        return hipEventRecord(ctx->stage_ev[k & 1], s);
+    };
+    // two-deep pipeline: chunk k+1 is in flight while chunk k is copied out
+    BMI_HIP(issue(0));
+    for (size_t k = 0; k < nk; ++k) {
+        if (k + 1 < nk) BMI_HIP(issue(k + 1));
+        BMI_HIP(hipEventSynchronize(ctx->stage_ev[k & 1]));
+        const size_t o = k * ce, c = std::min(ce, n - o);
+        consume_chunk(ctx->stage[k & 1], src32, dst, dtype == BMI_C32, o, c);
     }
-    const size_t chunk = 1 << 22;  // complex values per staging copy
-    const size_t es = m->dtype == BMI_C64 ? 16 : 8;
-    std::vector<unsigned char> stage(chunk * es);
-    for (size_t o = 0; o < n; o += chunk) {
-        const size_t c = std::min(chunk, n - o);
-        BMI_HIP(hipMemcpy(stage.data(), (const unsigned char *)m->d + o * es, c * es, hipMemcpyDeviceToHost));
-        if (m->dtype == BMI_C64) {
-            const double *src = (const double *)stage.data();
-            float *d = (float *)dst + 2 * o;
-            for (size_t i = 0; i < 2 * c; ++i) d[i] = (float)src[i];
-        } else {
-            const float *src = (const float *)stage.data();
-            double *d = (double *)dst + 2 * o;
-            for (size_t i = 0; i < 2 * c; ++i) d[i] = (double)src[i];
-        }
+    return BMI_OK;
+}
+
+bmi_status bmi_matrix_apply(const bmi_matrix *m, const double *x, double *y, int32_t nrhs) {
+    if (!m || !x || !y) return fail(BMI_EINVAL, "NULL argument");
+    if (nrhs < 1 || nrhs > BMI_MAX_RHS) return fail(BMI_EINVAL, "nrhs must be 1..8");
+    if (m->rows > (1LL << 31) - 1 || m->cols > (1LL << 31) - 1) return fail(BMI_EUNSUPPORTED, "matrix too large");
+    BMI_HIP(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    const int rows = (int)m->rows, cols = (int)m->cols;
+    const int nchunk = (cols + BMI_APPLY_CHUNK - 1) / BMI_APPLY_CHUNK;
+    DevBuf<double2> dx, dy, part;
+    hipError_t e;
+    if ((e = dx.alloc((size_t)cols * nrhs)) != hipSuccess || (e = dy.alloc((size_t)rows * nrhs)) != hipSuccess ||
+        (e = part.alloc((size_t)nchunk * rows * nrhs)) != hipSuccess)
+        return fail(BMI_ENOMEM, std::string("apply buffers: ") + hipGetErrorString(e));
+    BMI_HIP(hipMemcpyAsync(dx.p, x, sizeof(double2) * cols * nrhs, hipMemcpyHostToDevice, s));
+    BMI_HIP(launch_apply(m->d, m->dtype == BMI_C32, m->rows, rows, cols, dx.p, nrhs, part.p, dy.p, s));
+    BMI_HIP(hipMemcpyAsync(y, dy.p, sizeof(double2) * rows * nrhs, hipMemcpyDeviceToHost, s));
+    BMI_HIP(hipStreamSynchronize(s));
+    return BMI_OK;
+}
+
+bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double out[3]) {
+    if (!a || !ref || !out) return fail(BMI_EINVAL, "NULL argument");
+    if (a->rows != ref->rows || a->cols != ref->cols) return fail(BMI_EINVAL, "matrix shapes differ");
+    if (a->ctx != ref->ctx) return fail(BMI_EINVAL, "matrices belong to different contexts");
+    BMI_HIP(hipSetDevice(a->ctx->device));
+    hipStream_t s = a->ctx->stream;
+    const int groups = 4096;
+    DevBuf<double> part;
+    hipError_t e = part.alloc(3 * groups);
+    if (e != hipSuccess) return fail(BMI_ENOMEM, "compare buffer");
+    BMI_HIP(launch_compare(a->d, a->dtype == BMI_C32, ref->d, ref->dtype == BMI_C32, a->rows * a->cols, groups,
+                           part.p, s));
+    std::vector<double> h(3 * groups);
+    BMI_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    BMI_HIP(hipStreamSynchronize(s));
+    double d2 = 0.0, r2 = 0.0, dm = 0.0;
+    for (int g = 0; g < groups; ++g) {
+        d2 += h[3 * g];
+        r2 += h[3 * g + 1];
+        dm = std::max(dm, h[3 * g + 2]);
     }
+    out[0] = std::sqrt(d2);
+    out[1] = std::sqrt(r2);
+    out[2] = std::sqrt(dm);
     return BMI_OK;
 }
 

@@ -5,15 +5,22 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-// Triangles per patch.  A work-group of 256 threads owns one
-// (test patch) x (trial patch) block of 16 x 16 cell pairs.
+// Cells per trial patch and per test sub-batch: a work-group of 256 threads
+// integrates 16 x 16 cell pairs at a time.
 #define BMI_PATCH 16
 #define BMI_BLOCK_THREADS 256
-// at most 3 local shapes per cell -> at most 48 rows (or columns) per patch
+// at most 3 local shapes per cell -> at most 48 columns per trial patch
 #define BMI_MAX_ROWS (3 * BMI_PATCH)
+// output rows owned by one test panel (consecutive functions); a column of
+// a panel is one contiguous 1 KB (ComplexF64) segment of the matrix
+#define BMI_PANEL_ROWS 64
+// accumulators per thread: 64 rows x 48 columns / 256 threads
+#define BMI_PANEL_ACC (BMI_PANEL_ROWS * BMI_MAX_ROWS / BMI_BLOCK_THREADS)
 #define BMI_MAX_RULE 4      // far-rule points per cell
 #define BMI_MAX_GL 8        // Gauss-Legendre points per hypercube dimension
 #define BMI_MAX_NEAR 64     // points of the collapsed near rule per cell
+#define BMI_MAX_RHS 8       // right-hand sides per bmi_matrix_apply call
+#define BMI_APPLY_CHUNK 512 // columns per work-group of the apply kernel
 
 enum { OPK_HH3D = 1, OPK_MW = 2 };
 enum { CLS_FAR = 0, CLS_CV = 1, CLS_CE = 2, CLS_CF = 3, CLS_NEAR = 4, CLS_FAR64 = 5 };
@@ -27,6 +34,8 @@ struct DevSpace {
     const double *area;  // nt
     const double *pts;   // nt x BMI_MAX_RULE x 3: far-rule points relative to the centroid
     const int *patch_tri;        // npatch x BMI_PATCH, -1 padded
+    const double *patch_sph;     // npatch x 4: bounding sphere (centre, radius) of the cells' vertices
+    const double *patch_amax;    // npatch: largest cell area
     const int *row_ptr;          // npatch + 1: rows of each patch
     const int *row_fn;           // global function of each row
     const unsigned char *row_first;  // 1 if this patch is the first (in launch order) to write the row
@@ -50,16 +59,54 @@ struct RuleConst {
     double w4pi[BMI_MAX_RULE][BMI_MAX_RULE];  // w_I * w_J / (4 pi), reference weights
 };
 
-struct BlockArgs {
-    DevSpace X, Y;
-    const int *tp_list;  // test patches of this launch's colour
-    const int *bq_list;  // trial patches of this launch's colour
+// Packed geometry of a group of BMI_PATCH cells (a test sub-batch or a trial
+// patch) for one far rule, BMI_GEO doubles per group, loaded with one
+// coalesced read:
+#define BMI_GEO_PTS 0                                  // 16 x 12 rule points, relative to the centroid
+#define BMI_GEO_CEN (BMI_PATCH * 3 * BMI_MAX_RULE)     // 16 x 3 centroids
+#define BMI_GEO_AREA (BMI_GEO_CEN + 3 * BMI_PATCH)     // 16 areas (0 = no cell)
+#define BMI_GEO_VREL (BMI_GEO_AREA + BMI_PATCH)        // 16 x 9 vertices, relative to the centroid
+#define BMI_GEO (BMI_GEO_VREL + 9 * BMI_PATCH)         // 400
+
+// Far-pair assembly.  Test side: panels of BMI_PANEL_ROWS consecutive
+// functions, each with the cells that carry them (cells of several panels are
+// repeated), cut into sub-batches of BMI_PATCH cells; per sub-batch a slot
+// table: for each of the 64 rows the slots (local cell * 3 + shape,
+// coefficient) of that row's function.  Trial side: Morton patches of
+// BMI_PATCH cells with their columns (functions) and slot tables.  All
+// per-group records have fixed sizes, so a work-group needs no dependent
+// loads to find them.
+struct PanelArgs {
+    int npanel;
+    int nrows;                   // functions of the test space
+    const int *sb_off;           // npanel + 1: sub-batches of each panel
+    const double *tgeo;          // nsb x BMI_GEO
+    const int *trs;              // nsb x (BMI_PANEL_ROWS + 1): slot range of each row (0..48)
+    const unsigned char *tkl;    // nsb x 48
+    const double *tcf;           // nsb x 48
+    const double *bgeo;          // npb x BMI_GEO
+    const int *bnc;              // npb: columns of each trial patch
+    const int *bfn;              // npb x 48: function of each column
+    const unsigned char *bfirst; // npb x 48: 1 if the column is stored (first trial patch in launch order)
+    const int *bcs;              // npb x 49: slot range of each column (0..48)
+    const unsigned char *bkl;    // npb x 48
+    const double *bcf;           // npb x 48
+    // rank-round records (one per cell slot, 48 per group): bits 0-7 row (test)
+    // or column (trial) index inside the work-group's tile, 0xff = none;
+    // bits 8-15 rank of the slot among the group's slots of that row/column
+    const int *tslot;            // nsb x 48
+    const double *tscf;          // nsb x 48
+    const int *bslot;            // npb x 48
+    const double *bscf;          // npb x 48
+    int rounds_t, rounds_b;      // largest rank + 1 on each side
+    int ncm;                     // largest number of columns of a trial patch
+    const int *bq_list;          // trial patches of this launch's colour
     int nbq;
-    int npb;             // trial patches in total (block id = p * npb + q)
-    const int *blk_off;  // special-pair offsets per block
+    int npb;                     // trial patches in total (block id = sub-batch * npb + q)
+    const int *blk_off;          // special-pair offsets per block
     const unsigned char *spec_loc;
     const double2 *spec_val;
-    int capacity;        // special-pair entries allocated
+    int capacity;                // special-pair entries allocated
     void *Z;
     long long ld;
     OpConst op;
@@ -71,11 +118,11 @@ struct ClassifyArgs {
     int npa, npb;
     double k2x16;        // 16 |gamma|^2
     double near_d2;      // (near_factor * max edge)^2, mixed precision only; < 0 disables
+    double near_d;       // near_factor * max edge, mixed precision only; < 0 disables
     int *blk_count;      // pass 0 output
     const int *blk_off;  // pass 1 input
     int *spec_K, *spec_L;
     unsigned char *spec_cls, *spec_loc;
-    unsigned long long *class_count;  // 6 counters
     int capacity;
 };
 
@@ -98,5 +145,9 @@ struct SpecialArgs {
 hipError_t launch_classify(const ClassifyArgs &a, int pass, hipStream_t s);
 hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *total, hipStream_t s);
 hipError_t launch_special(const SpecialArgs &a, int capacity, int mixed, int op_kind, hipStream_t s);
-hipError_t launch_blocks(const BlockArgs &a, int ntp, int op_kind, int mixed, int c32, hipStream_t s);
+hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hipStream_t s);
 int scan_partials_needed(int n);
+hipError_t launch_apply(const void *Z, int c32, long long ld, int rows, int cols, const double2 *x, int nrhs,
+                        double2 *partial, double2 *y, hipStream_t s);
+hipError_t launch_compare(const void *a, int a32, const void *b, int b32, long long n, int groups, double *out,
+                          hipStream_t s);

@@ -1,5 +1,5 @@
 // gfx950 kernels of the BEM assembly: pair classification, special-pair
-// integrals (Sauter-Schwab, near, FP64-kept far pairs) and the far-pair block
+// integrals (Sauter-Schwab, near, FP64-kept far pairs) and the far-pair panel
 // kernel with its owner-computes write-back.
 //
 // Reference path replaced (file:line in the reference):
@@ -8,8 +8,8 @@
 //   test_toll! + SS kernel           pulledback_integrand_gpu.jl:22-228,
 //                                    SauterSchwab.jl:212-298,
 //                                    Integrand_gpu.jl:58-132          -> k_special
-//   doubleQuadRuleMainCaseKernel!    doubleQuadRuleMainCase.jl:60-105 -> k_blocks (phase 1)
-//   store_with_kernel_register!      store_with_kernel.jl:88-97       -> k_blocks (phase 3)
+//   doubleQuadRuleMainCaseKernel!    doubleQuadRuleMainCase.jl:60-105 -> k_panels (phase 1)
+//   store_with_kernel_register!      store_with_kernel.jl:88-97       -> k_panels (phase 3, write-back)
 //   make_complex                     assamble_gpu.jl:141-144          -> not needed: complex is written directly
 #include "bmi_internal.h"
 
@@ -319,93 +319,138 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
 
 // ------------------------------------------------------- classification
 
-// One work-group per (test patch, trial patch) block, one thread per pair.
-// Class as quadrule_determine.jl:54-77: hits = #vertex pairs closer than
-// 1e3 eps; otherwise near iff dmin^2 max(16|gamma|^2, 1/A_trial) < 0.64.
-// The mixed path also keeps far pairs whose centroids are closer than
-// near_factor * max edge in FP64 (class 5).
-// pass 0 counts special pairs per block; pass 1 writes them, in thread order,
-// at the block's offset.
+// One work-group per test patch p.  For each trial patch q the bounding
+// spheres give a lower bound D on the distance between any cell of p and any
+// cell of q; a block can hold a special pair only if D is below the largest
+// distance at which a pair can touch (0), be near (dmin < 0.8 sqrt(A_trial),
+// from the near test below) or, in the mixed path, be kept in FP64 (centroid
+// distance < near_factor * max edge).  Blocks that pass are classified pair
+// by pair (256 threads = 16 x 16 pairs), as quadrule_determine.jl:54-77:
+// hits = vertex pairs closer than 1e3 eps (compared as squared distances);
+// otherwise near iff dmin^2 max(16|gamma|^2, 1/A_trial) < 0.64; the mixed
+// path marks far pairs with close centroids as class 5.
+// pass 0 writes the number of special pairs of every block; pass 1 writes the
+// pairs, in thread order, at the block's offset.
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(ClassifyArgs a, int pass) {
     __shared__ double tv[BMI_PATCH][9], bv[BMI_PATCH][9];
     __shared__ double tc[BMI_PATCH][3], bc[BMI_PATCH][3];
     __shared__ double barea[BMI_PATCH];
     __shared__ int tt[BMI_PATCH], bt[BMI_PATCH];
+    __shared__ int nearq[BMI_BLOCK_THREADS];
     __shared__ int wsum[BMI_BLOCK_THREADS / 64];
 
-    const int b = blockIdx.x;
-    const int p = b / a.npb, q = b - p * a.npb;
+    const int p = blockIdx.x;
     const int t = threadIdx.x;
+    const int lane = t & 63, wv = t >> 6;
     if (t < BMI_PATCH) tt[t] = a.X.patch_tri[p * BMI_PATCH + t];
-    else if (t < 2 * BMI_PATCH) bt[t - BMI_PATCH] = a.Y.patch_tri[q * BMI_PATCH + t - BMI_PATCH];
     __syncthreads();
     for (int i = t; i < BMI_PATCH * 9; i += BMI_BLOCK_THREADS) {
         const int c = i / 9, e = i - 9 * (i / 9);
-        const int K = tt[c], L = bt[c];
-        tv[c][e] = K >= 0 ? a.X.vtx[9 * K + e] : 0.0;
-        bv[c][e] = L >= 0 ? a.Y.vtx[9 * L + e] : 0.0;
+        tv[c][e] = tt[c] >= 0 ? a.X.vtx[9 * tt[c] + e] : 0.0;
     }
     if (t < BMI_PATCH * 3) {
         const int c = t / 3, e = t - 3 * (t / 3);
         tc[c][e] = tt[c] >= 0 ? a.X.cen[3 * tt[c] + e] : 0.0;
-        bc[c][e] = bt[c] >= 0 ? a.Y.cen[3 * bt[c] + e] : 0.0;
     }
-    if (t < BMI_PATCH) barea[t] = bt[t] >= 0 ? a.Y.area[bt[t]] : 1.0;
-    __syncthreads();
+    const double px = a.X.patch_sph[4 * p], py = a.X.patch_sph[4 * p + 1], pz = a.X.patch_sph[4 * p + 2];
+    const double pr = a.X.patch_sph[4 * p + 3];
+    const double dtol2 = DTOL * DTOL;
 
-    const int kl = t >> 4, ll = t & 15;
-    const int K = tt[kl], L = bt[ll];
-    int cls = -1;
-    if (K >= 0 && L >= 0) {
-        int hits = 0;
-        double dmin2 = 1.79769313486231570815e308;
+    for (int q0 = 0; q0 < a.npb; q0 += BMI_BLOCK_THREADS) {
+        // cull this chunk of trial patches
+        const int q = q0 + t;
+        int cand = 0;
+        if (q < a.npb) {
+            const double dx = px - a.Y.patch_sph[4 * q], dy = py - a.Y.patch_sph[4 * q + 1],
+                         dz = pz - a.Y.patch_sph[4 * q + 2];
+            const double D = sqrt(dx * dx + dy * dy + dz * dz) - pr - a.Y.patch_sph[4 * q + 3];
+            double reach = fmax(1e-9, 0.8 * sqrt(a.Y.patch_amax[q]));
+            if (a.near_d >= 0.0) reach = fmax(reach, a.near_d);
+            cand = D < 1.0001 * reach + 1e-12;
+            if (pass == 0 && !cand) a.blk_count[(long long)p * a.npb + q] = 0;
+        }
+        const unsigned long long m = __ballot(cand);
+        if (lane == 0) wsum[wv] = __popcll(m);
+        __syncthreads();
+        int base = 0, ncand = 0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const double dx = tv[kl][3 * i] - bv[ll][3 * j];
-                const double dy = tv[kl][3 * i + 1] - bv[ll][3 * j + 1];
-                const double dz = tv[kl][3 * i + 2] - bv[ll][3 * j + 2];
-                const double d2 = dx * dx + dy * dy + dz * dz;
-                dmin2 = fmin(dmin2, d2);
-                hits += sqrt(d2) < DTOL;
+        for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
+            base += i < wv ? wsum[i] : 0;
+            ncand += wsum[i];
+        }
+        if (cand) nearq[base + __popcll(m & ((1ull << lane) - 1ull))] = q;
+        __syncthreads();
+
+        for (int u = 0; u < ncand; ++u) {
+            const int qq = nearq[u];
+            const long long b = (long long)p * a.npb + qq;
+            if (t < BMI_PATCH) bt[t] = a.Y.patch_tri[qq * BMI_PATCH + t];
+            __syncthreads();
+            for (int i = t; i < BMI_PATCH * 9; i += BMI_BLOCK_THREADS) {
+                const int c = i / 9, e = i - 9 * (i / 9);
+                bv[c][e] = bt[c] >= 0 ? a.Y.vtx[9 * bt[c] + e] : 0.0;
             }
-        if (hits > 0) {
-            cls = hits > 3 ? 3 : hits;
-        } else {
-            const double m = fmax(a.k2x16, 1.0 / barea[ll]);
-            if (dmin2 * m < 0.2 * 0.2 * 16) {
-                cls = CLS_NEAR;
-            } else {
-                cls = CLS_FAR;
-                if (a.near_d2 >= 0.0) {
-                    const double cx = tc[kl][0] - bc[ll][0], cy = tc[kl][1] - bc[ll][1], cz = tc[kl][2] - bc[ll][2];
-                    if (cx * cx + cy * cy + cz * cz < a.near_d2) cls = CLS_FAR64;
+            if (t < BMI_PATCH * 3) {
+                const int c = t / 3, e = t - 3 * (t / 3);
+                bc[c][e] = bt[c] >= 0 ? a.Y.cen[3 * bt[c] + e] : 0.0;
+            }
+            if (t < BMI_PATCH) barea[t] = bt[t] >= 0 ? a.Y.area[bt[t]] : 1.0;
+            __syncthreads();
+
+            const int kl = t >> 4, ll = t & 15;
+            const int K = tt[kl], L = bt[ll];
+            int cls = -1;
+            if (K >= 0 && L >= 0) {
+                int hits = 0;
+                double dmin2 = 1.79769313486231570815e308;
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const double dx = tv[kl][3 * i] - bv[ll][3 * j];
+                        const double dy = tv[kl][3 * i + 1] - bv[ll][3 * j + 1];
+                        const double dz = tv[kl][3 * i + 2] - bv[ll][3 * j + 2];
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        dmin2 = fmin(dmin2, d2);
+                        hits += d2 < dtol2;
+                    }
+                if (hits > 0) {
+                    cls = hits > 3 ? 3 : hits;
+                } else if (dmin2 * fmax(a.k2x16, 1.0 / barea[ll]) < 0.2 * 0.2 * 16) {
+                    cls = CLS_NEAR;
+                } else {
+                    cls = CLS_FAR;
+                    if (a.near_d2 >= 0.0) {
+                        const double cx = tc[kl][0] - bc[ll][0], cy = tc[kl][1] - bc[ll][1],
+                                     cz = tc[kl][2] - bc[ll][2];
+                        if (cx * cx + cy * cy + cz * cz < a.near_d2) cls = CLS_FAR64;
+                    }
                 }
             }
-        }
-    }
-    const int special = cls > 0;
-    const unsigned long long mask = __ballot(special);
-    const int lane = t & 63, wv = t >> 6;
-    const int before = __popcll(mask & ((1ull << lane) - 1ull));
-    if (lane == 0) wsum[wv] = __popcll(mask);
-    __syncthreads();
-    int base = 0, total = 0;
+            const int special = cls > 0;
+            const unsigned long long sm = __ballot(special);
+            const int before = __popcll(sm & ((1ull << lane) - 1ull));
+            __syncthreads();  // wsum reuse
+            if (lane == 0) wsum[wv] = __popcll(sm);
+            __syncthreads();
+            int sbase = 0, total = 0;
 #pragma unroll
-    for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
-        base += i < wv ? wsum[i] : 0;
-        total += wsum[i];
-    }
-    if (pass == 0) {
-        if (t == 0) a.blk_count[b] = total;
-    } else if (special) {
-        const int e = a.blk_off[b] + base + before;
-        if (e < a.capacity) {
-            a.spec_K[e] = K;
-            a.spec_L[e] = L;
-            a.spec_cls[e] = (unsigned char)cls;
-            a.spec_loc[e] = (unsigned char)t;
+            for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
+                sbase += i < wv ? wsum[i] : 0;
+                total += wsum[i];
+            }
+            if (pass == 0) {
+                if (t == 0) a.blk_count[b] = total;
+            } else if (special) {
+                const int e = a.blk_off[b] + sbase + before;
+                if (e < a.capacity) {
+                    a.spec_K[e] = K;
+                    a.spec_L[e] = L;
+                    a.spec_cls[e] = (unsigned char)cls;
+                    a.spec_loc[e] = (unsigned char)t;
+                }
+            }
+            __syncthreads();  // bt, bv, wsum reused by the next block
         }
     }
 }
@@ -496,8 +541,7 @@ hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *to
 }
 
 hipError_t launch_classify(const ClassifyArgs &a, int pass, hipStream_t s) {
-    const long long nb = (long long)a.npa * a.npb;
-    hipLaunchKernelGGL(k_classify, dim3((unsigned)nb), dim3(BMI_BLOCK_THREADS), 0, s, a, pass);
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)a.npa), dim3(BMI_BLOCK_THREADS), 0, s, a, pass);
     return hipGetLastError();
 }
 
@@ -629,7 +673,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_special(SpecialArgs a) {
                 for (int j = 0; j < 3; ++j) {
                     const double dx = tvx[3 * i] - svx[3 * j], dy = tvx[3 * i + 1] - svx[3 * j + 1],
                                  dz = tvx[3 * i + 2] - svx[3 * j + 2];
-                    if (sqrt(dx * dx + dy * dy + dz * dz) < DTOL && ne < 3) {
+                    if (dx * dx + dy * dy + dz * dz < DTOL * DTOL && ne < 3) {
                         mi[ne] = i;
                         mj[ne] = j;
                         ++ne;
@@ -813,159 +857,458 @@ __device__ __forceinline__ void out_write(float2 *Z, long long off, double re, d
     Z[off] = v;
 }
 
-// One work-group per (test patch p, trial patch q) block of 16 x 16 cell
-// pairs.  Launches go colour pair by colour pair: within one launch no two
-// work-groups share an output entry, so the write-back needs no atomics.
-//   phase 1: thread (kl, ll) integrates cell pair (p[kl], q[ll]) -> LDS
-//   phase 2: special pairs of the block (singular, near, FP64-kept) overwrite
-//            their slots with the values computed by k_special / k_far64
-//   phase 3: each output entry (row of p, column of q) gathers its slot
-//            contributions in a fixed order and is stored, or added to what
-//            an earlier launch stored (first-touch flags decide).
+// One work-group per (test panel P, trial patch q).  The panel owns output
+// rows [64 P, 64 P + 64) completely: its cells (all cells carrying one of
+// those functions) are integrated against the 16 cells of q in sub-batches of
+// 16, and every output entry (row of P, column of q) is accumulated in a
+// register of the thread that owns it.  Launches go trial colour by trial
+// colour: inside one launch no two work-groups share an output entry (rows
+// belong to one panel, columns of one colour to one patch), so there are no
+// atomics; a column is stored by the launch of its first trial patch and
+// read-modify-written by the launch of its second.
+//   per sub-batch:  phase 1  thread (k, l) integrates cell pair (sub-batch[k], q[l]) -> LDS
+//                   phase 2  special pairs of the block overwrite their slots
+//                   phase 3  each owned entry adds its slot contributions, in a fixed order
+//   at the end:     each wave writes whole 64-row column segments (1 KB, contiguous)
+// The records of sub-batch s + 1 are read into registers while s is being
+// integrated.  Thread t owns row t % 64 and the columns t / 64 + 4 j, j < 12.
 template <int OP, int MIXED, typename OutT>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_blocks(BlockArgs a) {
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
     __shared__ double2 V[9 * BMI_BLOCK_THREADS];  // V[v * 256 + kl * 16 + ll]
-    __shared__ double sxp[BMI_PATCH][3 * BMI_MAX_RULE], syp[BMI_PATCH][3 * BMI_MAX_RULE];
-    __shared__ double sxa[BMI_PATCH][9], sya[BMI_PATCH][9];
-    __shared__ double sxc[BMI_PATCH][3], syc[BMI_PATCH][3];
-    __shared__ double sxs[BMI_PATCH], sys[BMI_PATCH];
-    __shared__ int stt[BMI_PATCH], sbt[BMI_PATCH];
-    __shared__ int s_rfn[BMI_MAX_ROWS], s_cfn[BMI_MAX_ROWS];
-    __shared__ unsigned char s_rfirst[BMI_MAX_ROWS], s_cfirst[BMI_MAX_ROWS];
-    __shared__ int s_rs[BMI_MAX_ROWS + 1], s_cs[BMI_MAX_ROWS + 1];
-    __shared__ unsigned char s_rkl[BMI_MAX_ROWS], s_ckl[BMI_MAX_ROWS];
-    __shared__ double s_rcf[BMI_MAX_ROWS], s_ccf[BMI_MAX_ROWS];
+    __shared__ double sg[BMI_GEO];                // test sub-batch geometry
+    __shared__ double bg[BMI_GEO];                // trial patch geometry
+    __shared__ int s_cfn[BMI_MAX_ROWS];
+    __shared__ unsigned char s_cfirst[BMI_MAX_ROWS];
+    __shared__ int s_cs[BMI_MAX_ROWS + 1];
+    __shared__ unsigned char s_ckl[BMI_MAX_ROWS];
+    __shared__ double s_ccf[BMI_MAX_ROWS];
+    __shared__ int s_rs[BMI_PANEL_ROWS + 1];
+    __shared__ unsigned char s_rkl[BMI_MAX_ROWS];
+    __shared__ double s_rcf[BMI_MAX_ROWS];
 
     const int t = threadIdx.x;
     const int w = blockIdx.x;
-    const int wp = w / a.nbq;
-    const int p = a.tp_list[wp];
-    const int q = a.bq_list[w - wp * a.nbq];
+    const int P = w / a.nbq;
+    const int q = a.bq_list[w - P * a.nbq];
+    const int r0 = P * BMI_PANEL_ROWS;
+    const int nr = min(BMI_PANEL_ROWS, a.nrows - r0);
+    const int nc = a.bnc[q];
+    const int myr = t & (BMI_PANEL_ROWS - 1);
+    const int myc = t >> 6;
+    const int kl = t >> 4, ll = t & 15;
+    const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
 
-    const int r0 = a.X.row_ptr[p], nr = a.X.row_ptr[p + 1] - r0;
-    const int c0 = a.Y.row_ptr[q], nc = a.Y.row_ptr[q + 1] - c0;
-    const int rs0 = a.X.row_slot_ptr[r0], nrs = a.X.row_slot_ptr[r0 + nr] - rs0;
-    const int cs0 = a.Y.row_slot_ptr[c0], ncs = a.Y.row_slot_ptr[c0 + nc] - cs0;
-
-    if (t < BMI_PATCH) stt[t] = a.X.patch_tri[p * BMI_PATCH + t];
-    else if (t < 2 * BMI_PATCH) sbt[t - BMI_PATCH] = a.Y.patch_tri[q * BMI_PATCH + t - BMI_PATCH];
-    if (t < nr) {
-        s_rfn[t] = a.X.row_fn[r0 + t];
-        s_rfirst[t] = a.X.row_first[r0 + t];
-        s_rs[t] = a.X.row_slot_ptr[r0 + t] - rs0;
-    }
-    if (t == 0) s_rs[nr] = nrs;
-    if (t < nc) {
-        s_cfn[t] = a.Y.row_fn[c0 + t];
-        s_cfirst[t] = a.Y.row_first[c0 + t];
-        s_cs[t] = a.Y.row_slot_ptr[c0 + t] - cs0;
-    }
-    if (t == 0) s_cs[nc] = ncs;
-    if (t < nrs) {
-        s_rkl[t] = a.X.slot_kl[rs0 + t];
-        s_rcf[t] = a.X.slot_cf[rs0 + t];
-    }
-    if (t < ncs) {
-        s_ckl[t] = a.Y.slot_kl[cs0 + t];
-        s_ccf[t] = a.Y.slot_cf[cs0 + t];
-    }
-    __syncthreads();
-    for (int i = t; i < BMI_PATCH * 3 * BMI_MAX_RULE; i += BMI_BLOCK_THREADS) {
-        const int c = i / (3 * BMI_MAX_RULE), e = i - c * (3 * BMI_MAX_RULE);
-        const int K = stt[c], L = sbt[c];
-        sxp[c][e] = K >= 0 ? a.X.pts[3 * BMI_MAX_RULE * (long long)K + e] : 0.0;
-        syp[c][e] = L >= 0 ? a.Y.pts[3 * BMI_MAX_RULE * (long long)L + e] : 0.0;
-    }
-    if (t < BMI_PATCH * 3) {
-        const int c = t / 3, e = t - 3 * (t / 3);
-        const int K = stt[c], L = sbt[c];
-        sxc[c][e] = K >= 0 ? a.X.cen[3 * K + e] : 0.0;
-        syc[c][e] = L >= 0 ? a.Y.cen[3 * L + e] : 0.0;
-    }
-    if (t < BMI_PATCH) {
-        sxs[t] = stt[t] >= 0 ? a.X.area[stt[t]] : 0.0;
-        sys[t] = sbt[t] >= 0 ? a.Y.area[sbt[t]] : 0.0;
-    }
-    __syncthreads();
-    if (OP == OPK_MW && t < BMI_PATCH * 9) {
-        const int c = t / 9, e = t - 9 * (t / 9);
-        const int K = stt[c], L = sbt[c];
-        sxa[c][e] = K >= 0 ? a.X.vtx[9 * K + e] - sxc[c][e % 3] : 0.0;
-        sya[c][e] = L >= 0 ? a.Y.vtx[9 * L + e] - syc[c][e % 3] : 0.0;
-    }
-    __syncthreads();
-
-    // phase 1
+    // trial patch records (once per work-group)
     {
-        const int kl = t >> 4, ll = t & 15;
+        const double *g = a.bgeo + (long long)q * BMI_GEO;
+        bg[t] = g[t];
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) bg[BMI_BLOCK_THREADS + t] = g[BMI_BLOCK_THREADS + t];
+        if (t < BMI_MAX_ROWS) {
+            s_cfn[t] = a.bfn[q * BMI_MAX_ROWS + t];
+            s_cfirst[t] = a.bfirst[q * BMI_MAX_ROWS + t];
+            s_ckl[t] = a.bkl[q * BMI_MAX_ROWS + t];
+            s_ccf[t] = a.bcf[q * BMI_MAX_ROWS + t];
+        }
+        if (t <= BMI_MAX_ROWS) s_cs[t] = a.bcs[q * (BMI_MAX_ROWS + 1) + t];
+    }
+
+    // registers holding the next sub-batch's records
+    double pg0 = 0.0, pg1 = 0.0, pcf = 0.0;
+    int prs = 0, pkl = 0, pe0 = 0, pe1 = 0;
+    auto fetch = [&](int s) {
+        const double *g = a.tgeo + (long long)s * BMI_GEO;
+        pg0 = g[t];
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) pg1 = g[BMI_BLOCK_THREADS + t];
+        if (t <= BMI_PANEL_ROWS) prs = a.trs[(long long)s * (BMI_PANEL_ROWS + 1) + t];
+        if (t < BMI_MAX_ROWS) {
+            pkl = a.tkl[(long long)s * BMI_MAX_ROWS + t];
+            pcf = a.tcf[(long long)s * BMI_MAX_ROWS + t];
+        }
+        const long long b = (long long)s * a.npb + q;
+        pe0 = a.blk_off[b];
+        pe1 = a.blk_off[b + 1];
+    };
+    if (sb0 < sb1) fetch(sb0);
+
+    double accr[BMI_PANEL_ACC], acci[BMI_PANEL_ACC];
+#pragma unroll
+    for (int j = 0; j < BMI_PANEL_ACC; ++j) accr[j] = acci[j] = 0.0;
+
+    for (int s = sb0; s < sb1; ++s) {
+        __syncthreads();  // the previous sub-batch is done with sg, V and the row tables
+        sg[t] = pg0;
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) sg[BMI_BLOCK_THREADS + t] = pg1;
+        if (t <= BMI_PANEL_ROWS) s_rs[t] = prs;
+        if (t < BMI_MAX_ROWS) {
+            s_rkl[t] = (unsigned char)pkl;
+            s_rcf[t] = pcf;
+        }
+        const int e0 = pe0, e1 = min(pe1, a.capacity);
+        if (s + 1 < sb1) fetch(s + 1);
+        __syncthreads();
+
+        // phase 1
+        {
+            double2 loc[9];
+            const double xs = sg[BMI_GEO_AREA + kl], ys = bg[BMI_GEO_AREA + ll];
+            if (xs > 0.0 && ys > 0.0) {
+                const double *xp = &sg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * kl];
+                const double *xc = &sg[BMI_GEO_CEN + 3 * kl];
+                const double *xa = &sg[BMI_GEO_VREL + 9 * kl];
+                const double *yp = &bg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * ll];
+                const double *yc = &bg[BMI_GEO_CEN + 3 * ll];
+                const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
+                if (MIXED) pair_mixed<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
+                else pair_fp64<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int v = 0; v < 9; ++v) V[v * BMI_BLOCK_THREADS + t] = loc[v];
+        }
+
+        // phase 2 (blocks next to the diagonal only)
+        if (e1 > e0) {
+            __syncthreads();
+            for (int e = e0 + t; e < e1; e += BMI_BLOCK_THREADS) {
+                const int lo = a.spec_loc[e];
+#pragma unroll
+                for (int v = 0; v < 9; ++v) V[v * BMI_BLOCK_THREADS + lo] = a.spec_val[9 * (long long)e + v];
+            }
+        }
+        __syncthreads();
+
+        // phase 3
+        const int u0 = s_rs[myr], u1 = s_rs[myr + 1];
+        if (u0 < u1) {
+#pragma unroll
+            for (int j = 0; j < BMI_PANEL_ACC; ++j) {
+                const int c = myc + 4 * j;
+                if (c < nc) {
+                    double sr = accr[j], si = acci[j];
+                    const int v0 = s_cs[c], v1 = s_cs[c + 1];
+                    for (int u = u0; u < u1; ++u) {
+                        const int ki = s_rkl[u];
+                        const int k = ki / 3, i = ki - 3 * k;
+                        const double cf = s_rcf[u];
+                        for (int v = v0; v < v1; ++v) {
+                            const int lj = s_ckl[v];
+                            const int l = lj / 3, jj = lj - 3 * l;
+                            const double cc = cf * s_ccf[v];
+                            const double2 z = V[(3 * i + jj) * BMI_BLOCK_THREADS + k * BMI_PATCH + l];
+                            sr = fma(cc, z.x, sr);
+                            si = fma(cc, z.y, si);
+                        }
+                    }
+                    accr[j] = sr;
+                    acci[j] = si;
+                }
+            }
+        }
+    }
+
+    // write-back: a wave stores rows r0 .. r0 + 63 of one column
+    __syncthreads();
+    if (myr < nr) {
+        OutT *Z = (OutT *)a.Z;
+#pragma unroll
+        for (int j = 0; j < BMI_PANEL_ACC; ++j) {
+            const int c = myc + 4 * j;
+            if (c < nc) {
+                const long long off = (long long)(r0 + myr) + (long long)s_cfn[c] * a.ld;
+                out_write(Z, off, accr[j], acci[j], s_cfirst[c]);
+            }
+        }
+    }
+}
+
+// Variant of k_panels without the LDS pair buffer and the gather: each
+// thread adds its pair's 9 values straight into an LDS tile of the
+// work-group's 64 rows x ncm columns.  Two pairs add to the same entry only if
+// their test cells share the entry's row and their trial cells share its
+// column; giving every slot its rank among the slots of its group with the
+// same row (column), the additions go in rounds (rank_t, rank_b) with a
+// barrier between rounds, so no entry is touched twice inside a round and
+// the summation order is fixed.  RWG functions have at most two cells, so
+// there are at most 2 x 2 rounds.
+template <int OP, int MIXED, typename OutT>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a) {
+    extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x ncm
+    __shared__ double sg[BMI_GEO];
+    __shared__ double bg[BMI_GEO];
+    __shared__ int s_tslot[BMI_MAX_ROWS];
+    __shared__ double s_tscf[BMI_MAX_ROWS];
+    __shared__ int s_cfn[BMI_MAX_ROWS];
+    __shared__ unsigned char s_cfirst[BMI_MAX_ROWS];
+    __shared__ int s_sidx[BMI_BLOCK_THREADS];
+
+    const int t = threadIdx.x;
+    const int w = blockIdx.x;
+    const int P = w / a.nbq;
+    const int q = a.bq_list[w - P * a.nbq];
+    const int r0 = P * BMI_PANEL_ROWS;
+    const int nr = min(BMI_PANEL_ROWS, a.nrows - r0);
+    const int nc = a.bnc[q];
+    const int ncm = a.ncm;
+    const int kl = t >> 4, ll = t & 15;
+    const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
+
+    for (int i = t; i < BMI_PANEL_ROWS * ncm; i += BMI_BLOCK_THREADS) tile[i] = make_double2(0.0, 0.0);
+    {
+        const double *g = a.bgeo + (long long)q * BMI_GEO;
+        bg[t] = g[t];
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) bg[BMI_BLOCK_THREADS + t] = g[BMI_BLOCK_THREADS + t];
+        if (t < BMI_MAX_ROWS) {
+            s_cfn[t] = a.bfn[q * BMI_MAX_ROWS + t];
+            s_cfirst[t] = a.bfirst[q * BMI_MAX_ROWS + t];
+        }
+    }
+    // the three column slots of this thread's trial cell, for the whole work-group
+    int bsl[3];
+    double bcf[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        bsl[j] = a.bslot[q * BMI_MAX_ROWS + 3 * ll + j];
+        bcf[j] = a.bscf[q * BMI_MAX_ROWS + 3 * ll + j];
+    }
+
+    double pg0 = 0.0, pg1 = 0.0, pcf = 0.0;
+    int psl = 0xff, pe0 = 0, pe1 = 0;
+    auto fetch = [&](int s) {
+        const double *g = a.tgeo + (long long)s * BMI_GEO;
+        pg0 = g[t];
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) pg1 = g[BMI_BLOCK_THREADS + t];
+        if (t < BMI_MAX_ROWS) {
+            psl = a.tslot[(long long)s * BMI_MAX_ROWS + t];
+            pcf = a.tscf[(long long)s * BMI_MAX_ROWS + t];
+        }
+        const long long b = (long long)s * a.npb + q;
+        pe0 = a.blk_off[b];
+        pe1 = a.blk_off[b + 1];
+    };
+    if (sb0 < sb1) fetch(sb0);
+
+    for (int s = sb0; s < sb1; ++s) {
+        __syncthreads();  // previous sub-batch is done with sg and the slot records
+        sg[t] = pg0;
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) sg[BMI_BLOCK_THREADS + t] = pg1;
+        if (t < BMI_MAX_ROWS) {
+            s_tslot[t] = psl;
+            s_tscf[t] = pcf;
+        }
+        const int e0 = pe0, e1 = min(pe1, a.capacity);
+        if (s + 1 < sb1) fetch(s + 1);
+        __syncthreads();
+
         double2 loc[9];
-        if (stt[kl] >= 0 && sbt[ll] >= 0) {
-            if (MIXED)
-                pair_mixed<OP>(a.op, a.rule, sxp[kl], sxa[kl], sxc[kl], sxs[kl], syp[ll], sya[ll], syc[ll], sys[ll], loc);
-            else
-                pair_fp64<OP>(a.op, a.rule, sxp[kl], sxa[kl], sxc[kl], sxs[kl], syp[ll], sya[ll], syc[ll], sys[ll], loc);
+        const double xs = sg[BMI_GEO_AREA + kl], ys = bg[BMI_GEO_AREA + ll];
+        if (xs > 0.0 && ys > 0.0) {
+            const double *xp = &sg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * kl];
+            const double *xc = &sg[BMI_GEO_CEN + 3 * kl];
+            const double *xa = &sg[BMI_GEO_VREL + 9 * kl];
+            const double *yp = &bg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * ll];
+            const double *yc = &bg[BMI_GEO_CEN + 3 * ll];
+            const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
+            if (MIXED) pair_mixed<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
+            else pair_fp64<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
         } else {
 #pragma unroll
             for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
         }
+        // special pairs of this block take their precomputed values
+        if (e1 > e0) {
+            s_sidx[t] = -1;
+            __syncthreads();
+            for (int e = e0 + t; e < e1; e += BMI_BLOCK_THREADS) s_sidx[a.spec_loc[e]] = e;
+            __syncthreads();
+            const int e = s_sidx[t];
+            if (e >= 0) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) V[v * BMI_BLOCK_THREADS + t] = loc[v];
-    }
-    __syncthreads();
-
-    // phase 2
-    {
-        const int b = p * a.npb + q;
-        const int e0 = a.blk_off[b], e1 = min(a.blk_off[b + 1], a.capacity);
-        for (int e = e0 + t; e < e1; e += BMI_BLOCK_THREADS) {
-            const int loc = a.spec_loc[e];
-#pragma unroll
-            for (int v = 0; v < 9; ++v) V[v * BMI_BLOCK_THREADS + loc] = a.spec_val[9 * (long long)e + v];
-        }
-    }
-    __syncthreads();
-
-    // phase 3
-    OutT *Z = (OutT *)a.Z;
-    const int ne = nr * nc;
-    for (int e = t; e < ne; e += BMI_BLOCK_THREADS) {
-        const int c = e / nr, r = e - c * nr;
-        double sr = 0.0, si = 0.0;
-        for (int u = s_rs[r]; u < s_rs[r + 1]; ++u) {
-            const int ki = s_rkl[u];
-            const int k = ki / 3, i = ki - 3 * k;
-            const double cf = s_rcf[u];
-            for (int v = s_cs[c]; v < s_cs[c + 1]; ++v) {
-                const int lj = s_ckl[v];
-                const int l = lj / 3, j = lj - 3 * l;
-                const double cc = cf * s_ccf[v];
-                const double2 z = V[(3 * i + j) * BMI_BLOCK_THREADS + k * BMI_PATCH + l];
-                sr = fma(cc, z.x, sr);
-                si = fma(cc, z.y, si);
+                for (int v = 0; v < 9; ++v) loc[v] = a.spec_val[9 * (long long)e + v];
             }
         }
-        const long long off = (long long)s_rfn[r] + (long long)s_cfn[c] * a.ld;
-        out_write(Z, off, sr, si, s_rfirst[r] && s_cfirst[c]);
+        int tsl[3];
+        double tcf[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            tsl[i] = s_tslot[3 * kl + i];
+            tcf[i] = s_tscf[3 * kl + i];
+        }
+        for (int ra = 0; ra < a.rounds_t; ++ra) {
+            for (int rb = 0; rb < a.rounds_b; ++rb) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    if ((tsl[i] & 0xff) == 0xff || (tsl[i] >> 8) != ra) continue;
+                    const int row = tsl[i] & 0xff;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        if ((bsl[j] & 0xff) == 0xff || (bsl[j] >> 8) != rb) continue;
+                        const double cc = tcf[i] * bcf[j];
+                        double2 &z = tile[row * ncm + (bsl[j] & 0xff)];
+                        z.x = fma(cc, loc[3 * i + j].x, z.x);
+                        z.y = fma(cc, loc[3 * i + j].y, z.y);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+
+    // write-back: consecutive threads take consecutive rows of one column
+    __syncthreads();
+    OutT *Z = (OutT *)a.Z;
+    for (int e = t; e < BMI_PANEL_ROWS * nc; e += BMI_BLOCK_THREADS) {
+        const int c = e / BMI_PANEL_ROWS, r = e - c * BMI_PANEL_ROWS;
+        if (r < nr) {
+            const double2 v = tile[r * ncm + c];
+            out_write(Z, (long long)(r0 + r) + (long long)s_cfn[c] * a.ld, v.x, v.y, s_cfirst[c]);
+        }
     }
 }
 
 template <int OP, int MIXED>
-static void launch_blocks_t(const BlockArgs &a, int ntp, int c32, hipStream_t s) {
-    const unsigned grid = (unsigned)((long long)ntp * a.nbq);
+static void launch_panels_t(const PanelArgs &a, int c32, hipStream_t s) {
+    const unsigned grid = (unsigned)((long long)a.npanel * a.nbq);
+    if (OP == OPK_MW) {
+        // RWG: rank rounds into an LDS tile
+        const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * a.ncm;
+        if (c32)
+            hipLaunchKernelGGL((k_panels_rr<OP, MIXED, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_panels_rr<OP, MIXED, double2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
+        return;
+    }
     if (c32)
-        hipLaunchKernelGGL((k_blocks<OP, MIXED, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_panels<OP, MIXED, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
     else
-        hipLaunchKernelGGL((k_blocks<OP, MIXED, double2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_panels<OP, MIXED, double2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
 }
 
-hipError_t launch_blocks(const BlockArgs &a, int ntp, int op_kind, int mixed, int c32, hipStream_t s) {
-    if (ntp <= 0 || a.nbq <= 0) return hipSuccess;
+hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hipStream_t s) {
+    if (a.npanel <= 0 || a.nbq <= 0) return hipSuccess;
     if (op_kind == OPK_MW) {
-        if (mixed) launch_blocks_t<OPK_MW, 1>(a, ntp, c32, s);
-        else launch_blocks_t<OPK_MW, 0>(a, ntp, c32, s);
+        if (mixed) launch_panels_t<OPK_MW, 1>(a, c32, s);
+        else launch_panels_t<OPK_MW, 0>(a, c32, s);
     } else {
-        if (mixed) launch_blocks_t<OPK_HH3D, 1>(a, ntp, c32, s);
-        else launch_blocks_t<OPK_HH3D, 0>(a, ntp, c32, s);
+        if (mixed) launch_panels_t<OPK_HH3D, 1>(a, c32, s);
+        else launch_panels_t<OPK_HH3D, 0>(a, c32, s);
     }
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------- apply and compare
+
+// y = Z x for up to BMI_MAX_RHS right-hand sides.  Grid: (row blocks of 256,
+// column chunks); each thread owns one row of one chunk, reads its column
+// entries coalesced across the wave, and writes a partial sum; k_apply_reduce
+// adds the chunk partials in chunk order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_apply(const T *Z, long long ld, int rows, int cols,
+                                                             const double2 *x, int nrhs, int chunk,
+                                                             double2 *partial) {
+    __shared__ double2 sx[BMI_APPLY_CHUNK * BMI_MAX_RHS];
+    const int row = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
+    const int c0 = blockIdx.y * chunk;
+    const int c1 = min(c0 + chunk, cols);
+    for (int i = threadIdx.x; i < (c1 - c0) * nrhs; i += BMI_BLOCK_THREADS) {
+        const int c = i / nrhs, r = i - c * nrhs;
+        sx[i] = x[(long long)r * cols + c0 + c];
+    }
+    __syncthreads();
+    if (row >= rows) return;
+    double ar[BMI_MAX_RHS], ai[BMI_MAX_RHS];
+#pragma unroll
+    for (int r = 0; r < BMI_MAX_RHS; ++r) ar[r] = ai[r] = 0.0;
+    for (int c = c0; c < c1; ++c) {
+        const T z = Z[row + (long long)c * ld];
+        const double zr = (double)z.x, zi = (double)z.y;
+        const double2 *xc = &sx[(c - c0) * nrhs];
+#pragma unroll
+        for (int r = 0; r < BMI_MAX_RHS; ++r) {
+            if (r >= nrhs) break;
+            const double2 v = xc[r];
+            ar[r] = fma(zr, v.x, fma(-zi, v.y, ar[r]));
+            ai[r] = fma(zr, v.y, fma(zi, v.x, ai[r]));
+        }
+    }
+    for (int r = 0; r < nrhs; ++r)
+        partial[((long long)blockIdx.y * nrhs + r) * rows + row] = make_double2(ar[r], ai[r]);
+}
+
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_apply_reduce(const double2 *partial, int nchunk, int rows,
+                                                                    int nrhs, double2 *y) {
+    const long long i = (long long)blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
+    if (i >= (long long)rows * nrhs) return;
+    double sr = 0.0, si = 0.0;
+    for (int k = 0; k < nchunk; ++k) {
+        const double2 v = partial[(long long)k * rows * nrhs + i];
+        sr += v.x;
+        si += v.y;
+    }
+    y[i] = make_double2(sr, si);
+}
+
+hipError_t launch_apply(const void *Z, int c32, long long ld, int rows, int cols, const double2 *x, int nrhs,
+                        double2 *partial, double2 *y, hipStream_t s) {
+    const int nchunk = (cols + BMI_APPLY_CHUNK - 1) / BMI_APPLY_CHUNK;
+    dim3 grid((rows + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, nchunk);
+    if (c32)
+        hipLaunchKernelGGL(k_apply<float2>, grid, dim3(BMI_BLOCK_THREADS), 0, s, (const float2 *)Z, ld, rows, cols, x,
+                           nrhs, BMI_APPLY_CHUNK, partial);
+    else
+        hipLaunchKernelGGL(k_apply<double2>, grid, dim3(BMI_BLOCK_THREADS), 0, s, (const double2 *)Z, ld, rows, cols,
+                           x, nrhs, BMI_APPLY_CHUNK, partial);
+    const long long n = (long long)rows * nrhs;
+    hipLaunchKernelGGL(k_apply_reduce, dim3((unsigned)((n + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS)),
+                       dim3(BMI_BLOCK_THREADS), 0, s, partial, nchunk, rows, nrhs, y);
+    return hipGetLastError();
+}
+
+// Sums of |a - b|^2, |b|^2 and max |a - b| over a matrix, per work-group;
+// the host adds the per-group values in order.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_compare(const TA *a, const TB *b, long long n,
+                                                               double *out /* 3 per WG */) {
+    __shared__ double s0[BMI_BLOCK_THREADS], s1[BMI_BLOCK_THREADS], s2[BMI_BLOCK_THREADS];
+    const long long per = (n + (long long)gridDim.x - 1) / gridDim.x;
+    const long long i0 = (long long)blockIdx.x * per;
+    const long long i1 = min(i0 + per, n);
+    double d2 = 0.0, r2 = 0.0, dm = 0.0;
+    for (long long i = i0 + threadIdx.x; i < i1; i += BMI_BLOCK_THREADS) {
+        const double ax = (double)a[i].x, ay = (double)a[i].y;
+        const double bx = (double)b[i].x, by = (double)b[i].y;
+        const double dx = ax - bx, dy = ay - by;
+        const double dd = dx * dx + dy * dy;
+        d2 += dd;
+        r2 += bx * bx + by * by;
+        dm = fmax(dm, dd);
+    }
+    s0[threadIdx.x] = d2;
+    s1[threadIdx.x] = r2;
+    s2[threadIdx.x] = dm;
+    __syncthreads();
+    for (int o = BMI_BLOCK_THREADS / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s0[threadIdx.x] += s0[threadIdx.x + o];
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[3 * blockIdx.x] = s0[0];
+        out[3 * blockIdx.x + 1] = s1[0];
+        out[3 * blockIdx.x + 2] = s2[0];
+    }
+}
+
+hipError_t launch_compare(const void *a, int a32, const void *b, int b32, long long n, int groups, double *out,
+                          hipStream_t s) {
+    dim3 g(groups), t(BMI_BLOCK_THREADS);
+    if (!a32 && !b32) hipLaunchKernelGGL((k_compare<double2, double2>), g, t, 0, s, (const double2 *)a, (const double2 *)b, n, out);
+    else if (a32 && !b32) hipLaunchKernelGGL((k_compare<float2, double2>), g, t, 0, s, (const float2 *)a, (const double2 *)b, n, out);
+    else if (!a32 && b32) hipLaunchKernelGGL((k_compare<double2, float2>), g, t, 0, s, (const double2 *)a, (const float2 *)b, n, out);
+    else hipLaunchKernelGGL((k_compare<float2, float2>), g, t, 0, s, (const float2 *)a, (const float2 *)b, n, out);
     return hipGetLastError();
 }
