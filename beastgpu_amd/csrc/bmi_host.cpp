@@ -323,13 +323,45 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
     for (int P = 0; P < np; ++P) {
         std::sort(cells.begin() + cnt[P], cells.begin() + cnt[P + 1],
                   [&](int a, int b) { return morton_rank[a] < morton_rank[b]; });
-        const int nc = cnt[P + 1] - cnt[P];
-        const int nsb = (nc + BMI_PATCH - 1) / BMI_PATCH;
+        // Round-robin packing: the panel's cells, in Morton order, are dealt
+        // over ceil(n/16) sub-batches (neighbours land in different ones);
+        // a cell that would share a function of the panel with a cell already
+        // in its sub-batch goes to the next sub-batch that has room and no
+        // such conflict (one more sub-batch if none has).  So every row has at
+        // most one cell per sub-batch (its rank in the sub-batch is 0).
+        const int ncell = cnt[P + 1] - cnt[P];
+        std::vector<std::vector<int>> bins;
+        std::vector<uint64_t> used;
+        for (int nb = std::max(1, (ncell + BMI_PATCH - 1) / BMI_PATCH);; ++nb) {
+            bins.assign(nb, {});
+            used.assign(nb, 0);
+            bool ok = true;
+            for (int i = 0; i < ncell && ok; ++i) {
+                const int K = cells[cnt[P] + i];
+                uint64_t rows = 0;
+                for (int j = 0; j < 3; ++j) {
+                    const int64_t f = gidx[3 * (int64_t)K + j];
+                    if (f >= 0 && f / R == P) rows |= 1ull << (f - (int64_t)P * R);
+                }
+                ok = false;
+                for (int d = 0; d < nb; ++d) {
+                    const int b = (i + d) % nb;
+                    if ((int)bins[b].size() < BMI_PATCH && !(used[b] & rows)) {
+                        bins[b].push_back(K);
+                        used[b] |= rows;
+                        ok = true;
+                        break;
+                    }
+                }
+            }
+            if (ok) break;
+        }
+        if (ncell == 0) bins.clear();
+        const int nsb = (int)bins.size();
         for (int b = 0; b < nsb; ++b) {
             sl.clear();
             for (int k = 0; k < BMI_PATCH; ++k) {
-                const int idx = b * BMI_PATCH + k;
-                const int K = idx < nc ? cells[cnt[P] + idx] : -1;
+                const int K = k < (int)bins[b].size() ? bins[b][k] : -1;
                 s->sb_cells.push_back(K);
                 if (K < 0) continue;
                 for (int i = 0; i < 3; ++i) {
@@ -436,10 +468,31 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
     morton_rank.assign(nt, 0);
     for (int64_t i = 0; i < nt; ++i) morton_rank[order[i]] = (int)i;
 
-    const int np = (int)((nt + BMI_PATCH - 1) / BMI_PATCH);
+    // runs of at most BMI_PATCH consecutive cells carrying at most
+    // BMI_MAX_COLS functions (bounds the LDS tile of the far kernel)
+    s->patch_tri.clear();
+    {
+        std::vector<int64_t> fs;
+        int64_t i = 0;
+        while (i < nt) {
+            fs.clear();
+            int n = 0;
+            while (i < nt && n < BMI_PATCH) {
+                const int K = order[i];
+                for (int j = 0; j < 3; ++j)
+                    if (gidx[3 * (int64_t)K + j] >= 0) fs.push_back(gidx[3 * (int64_t)K + j]);
+                std::sort(fs.begin(), fs.end());
+                fs.erase(std::unique(fs.begin(), fs.end()), fs.end());
+                if (n > 0 && (int)fs.size() > BMI_MAX_COLS) break;
+                s->patch_tri.push_back(K);
+                ++n;
+                ++i;
+            }
+            while (n++ < BMI_PATCH) s->patch_tri.push_back(-1);
+        }
+    }
+    const int np = (int)(s->patch_tri.size() / BMI_PATCH);
     s->npatch = np;
-    s->patch_tri.assign((size_t)np * BMI_PATCH, -1);
-    for (int64_t i = 0; i < nt; ++i) s->patch_tri[i] = order[i];
 
     s->patch_sph.assign((size_t)np * 4, 0.0);
     s->patch_amax.assign(np, 0.0);
@@ -1003,6 +1056,7 @@ bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
     B.rounds_t = P->X->rounds_t;
     B.rounds_b = P->Y->rounds_b;
     B.ncm = P->Y->ncmax;
+    B.rr = P->X->rounds_t * P->Y->rounds_b <= 6;
     B.npb = P->npb;
     B.blk_off = P->d_blk_off.p;
     B.spec_loc = P->d_spec_loc.p;

@@ -9,11 +9,14 @@
 // integrates 16 x 16 cell pairs at a time.
 #define BMI_PATCH 16
 #define BMI_BLOCK_THREADS 256
-// at most 3 local shapes per cell -> at most 48 columns per trial patch
+// at most 3 local shapes per cell -> at most 48 slots per group of cells
 #define BMI_MAX_ROWS (3 * BMI_PATCH)
+// trial patches are cut to carry at most this many functions (columns)
+#define BMI_MAX_COLS 40
 // output rows owned by one test panel (consecutive functions); a column of
 // a panel is one contiguous 1 KB (ComplexF64) segment of the matrix
 #define BMI_PANEL_ROWS 64
+static_assert(BMI_PANEL_ROWS <= 64, "a panel's rows are tracked in a 64-bit mask");
 // accumulators per thread: 64 rows x 48 columns / 256 threads
 #define BMI_PANEL_ACC (BMI_PANEL_ROWS * BMI_MAX_ROWS / BMI_BLOCK_THREADS)
 #define BMI_MAX_RULE 4      // far-rule points per cell
@@ -100,6 +103,7 @@ struct PanelArgs {
     const double *bscf;          // npb x 48
     int rounds_t, rounds_b;      // largest rank + 1 on each side
     int ncm;                     // largest number of columns of a trial patch
+    int rr;                      // 1: rank-round kernel, 0: pair buffer + gather
     const int *bq_list;          // trial patches of this launch's colour
     int nbq;
     int npb;                     // trial patches in total (block id = sub-batch * npb + q)

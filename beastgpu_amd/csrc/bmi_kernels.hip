@@ -1035,10 +1035,10 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
 template <int OP, int MIXED, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a) {
     extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x ncm
-    __shared__ double sg[BMI_GEO];
-    __shared__ double bg[BMI_GEO];
-    __shared__ int s_tslot[BMI_MAX_ROWS];
-    __shared__ double s_tscf[BMI_MAX_ROWS];
+    __shared__ double sg[2][BMI_GEO];             // test sub-batch geometry, double-buffered
+    __shared__ double bg[BMI_GEO];                // trial patch geometry
+    __shared__ int s_tslot[2][BMI_MAX_ROWS];
+    __shared__ double s_tscf[2][BMI_MAX_ROWS];
     __shared__ int s_cfn[BMI_MAX_ROWS];
     __shared__ unsigned char s_cfirst[BMI_MAX_ROWS];
     __shared__ int s_sidx[BMI_BLOCK_THREADS];
@@ -1073,6 +1073,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
         bcf[j] = a.bscf[q * BMI_MAX_ROWS + 3 * ll + j];
     }
 
+    // registers with the records of the next sub-batch
     double pg0 = 0.0, pg1 = 0.0, pcf = 0.0;
     int psl = 0xff, pe0 = 0, pe1 = 0;
     auto fetch = [&](int s) {
@@ -1087,26 +1088,32 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
         pe0 = a.blk_off[b];
         pe1 = a.blk_off[b + 1];
     };
-    if (sb0 < sb1) fetch(sb0);
+    auto stash = [&](int buf) {
+        sg[buf][t] = pg0;
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) sg[buf][BMI_BLOCK_THREADS + t] = pg1;
+        if (t < BMI_MAX_ROWS) {
+            s_tslot[buf][t] = psl;
+            s_tscf[buf][t] = pcf;
+        }
+    };
+    int e0 = 0, e1 = 0;
+    if (sb0 < sb1) {
+        fetch(sb0);
+        stash(0);
+        e0 = pe0;
+        e1 = min(pe1, a.capacity);
+        if (sb0 + 1 < sb1) fetch(sb0 + 1);
+    }
+    __syncthreads();
 
     for (int s = sb0; s < sb1; ++s) {
-        __syncthreads();  // previous sub-batch is done with sg and the slot records
-        sg[t] = pg0;
-        if (t < BMI_GEO - BMI_BLOCK_THREADS) sg[BMI_BLOCK_THREADS + t] = pg1;
-        if (t < BMI_MAX_ROWS) {
-            s_tslot[t] = psl;
-            s_tscf[t] = pcf;
-        }
-        const int e0 = pe0, e1 = min(pe1, a.capacity);
-        if (s + 1 < sb1) fetch(s + 1);
-        __syncthreads();
-
+        const int cur = (s - sb0) & 1;
         double2 loc[9];
-        const double xs = sg[BMI_GEO_AREA + kl], ys = bg[BMI_GEO_AREA + ll];
+        const double xs = sg[cur][BMI_GEO_AREA + kl], ys = bg[BMI_GEO_AREA + ll];
         if (xs > 0.0 && ys > 0.0) {
-            const double *xp = &sg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * kl];
-            const double *xc = &sg[BMI_GEO_CEN + 3 * kl];
-            const double *xa = &sg[BMI_GEO_VREL + 9 * kl];
+            const double *xp = &sg[cur][BMI_GEO_PTS + 3 * BMI_MAX_RULE * kl];
+            const double *xc = &sg[cur][BMI_GEO_CEN + 3 * kl];
+            const double *xa = &sg[cur][BMI_GEO_VREL + 9 * kl];
             const double *yp = &bg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * ll];
             const double *yc = &bg[BMI_GEO_CEN + 3 * ll];
             const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
@@ -1132,8 +1139,16 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
         double tcf[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            tsl[i] = s_tslot[3 * kl + i];
-            tcf[i] = s_tscf[3 * kl + i];
+            tsl[i] = s_tslot[cur][3 * kl + i];
+            tcf[i] = s_tscf[cur][3 * kl + i];
+        }
+        // the next sub-batch goes to the other buffer (read last in the
+        // previous iteration); the round barriers below publish it
+        if (s + 1 < sb1) {
+            stash(cur ^ 1);
+            e0 = pe0;
+            e1 = min(pe1, a.capacity);
+            if (s + 2 < sb1) fetch(s + 2);
         }
         for (int ra = 0; ra < a.rounds_t; ++ra) {
             for (int rb = 0; rb < a.rounds_b; ++rb) {
@@ -1156,7 +1171,6 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
     }
 
     // write-back: consecutive threads take consecutive rows of one column
-    __syncthreads();
     OutT *Z = (OutT *)a.Z;
     for (int e = t; e < BMI_PANEL_ROWS * nc; e += BMI_BLOCK_THREADS) {
         const int c = e / BMI_PANEL_ROWS, r = e - c * BMI_PANEL_ROWS;
@@ -1170,8 +1184,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
 template <int OP, int MIXED>
 static void launch_panels_t(const PanelArgs &a, int c32, hipStream_t s) {
     const unsigned grid = (unsigned)((long long)a.npanel * a.nbq);
-    if (OP == OPK_MW) {
-        // RWG: rank rounds into an LDS tile
+    if (a.rr) {
+        // few rounds (RWG: at most 2): rank rounds into an LDS tile
         const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * a.ncm;
         if (c32)
             hipLaunchKernelGGL((k_panels_rr<OP, MIXED, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
