@@ -82,14 +82,46 @@ __device__ __forceinline__ void sincos_turns32(float f, float &s, float &c) {
     c = ((qi + 1) & 2) ? -b : b;
 }
 
+// sin and cos of 2*pi*t, FP64, with a table of the 64 angles 2*pi*m/64:
+// t = m/64 + f exactly (|f| <= 1/128, Sterbenz), x = 2*pi*f, |x| <= pi/64,
+// Taylor polynomials to x^7 / x^8 (truncation below 5e-18), then the angle
+// addition formulas with tab[m mod 64] = (sin, cos).
+__device__ __forceinline__ void sincos_turns64_tab(double t, const double2 *tab, double &s, double &c) {
+    const double m = rint(64.0 * t);
+    const double f = fma(m, -1.0 / 64.0, t);
+    const double x = f * 6.283185307179586476925;
+    const double x2 = x * x;
+    double ps = fma(x2, 1.0 / 362880.0, -1.0 / 5040.0);
+    ps = fma(x2, ps, 1.0 / 120.0);
+    ps = fma(x2, ps, -1.0 / 6.0);
+    const double sx = fma(x * x2, ps, x);
+    double pc = fma(x2, 1.0 / 40320.0, -1.0 / 720.0);
+    pc = fma(x2, pc, 1.0 / 24.0);
+    pc = fma(x2, pc, -0.5);
+    const double cx = fma(x2, pc, 1.0);
+    const double2 sc = tab[((int)m) & 63];
+    s = fma(sc.x, cx, sc.y * sx);
+    c = fma(sc.y, cx, -(sc.x * sx));
+}
+
+// Fills tab[0..63] with (sin, cos)(2 pi m / 64); threads 0..63 of the block.
+__device__ __forceinline__ void fill_sincos_table(double2 *tab) {
+    if (threadIdx.x < 64) {
+        double s, c;
+        sincospi((double)threadIdx.x / 32.0, &s, &c);
+        tab[threadIdx.x] = make_double2(s, c);
+    }
+}
+
 // G * w for one point pair in FP64: h = w4pi * exp(-gamma R) / R, returned as (re, im).
 __device__ __forceinline__ void green64(const OpConst &op, double w4pi, double dx, double dy, double dz,
-                                        double &hr, double &hi) {
+                                        double &hr, double &hi, const double2 *tab = nullptr) {
     const double R2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double rinv = rsqrt64(R2);
     const double R = R2 * rinv;
     double s, c;
-    sincos_turns64(R * op.k2pi, s, c);
+    if (tab) sincos_turns64_tab(R * op.k2pi, tab, s, c);
+    else sincos_turns64(R * op.k2pi, s, c);
     double h = w4pi * rinv;
     if (op.lossy) h *= exp(-op.gamma_re * R);
     hr = h * c;
@@ -148,7 +180,8 @@ __device__ __forceinline__ void hh_epilogue(const OpConst &op, const double *pr,
 template <int OP>
 __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &rule, const double *xp,
                                           const double *xa, const double *xc, double xs, const double *yp,
-                                          const double *ya, const double *yc, double ys, double2 loc[9]) {
+                                          const double *ya, const double *yc, double ys, double2 loc[9],
+                                          const double2 *tab = nullptr) {
     const double dcx = xc[0] - yc[0], dcy = xc[1] - yc[1], dcz = xc[2] - yc[2];
     if (OP == OPK_MW) {
         MomMW m = {};
@@ -163,7 +196,7 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
                 if (J >= rule.nb) break;
                 const double y0 = yp[3 * J], y1 = yp[3 * J + 1], y2 = yp[3 * J + 2];
                 double hr, hi;
-                green64(op, rule.w4pi[I][J], ux - y0, uy - y1, uz - y2, hr, hi);
+                green64(op, rule.w4pi[I][J], ux - y0, uy - y1, uz - y2, hr, hi, tab);
                 sr += hr;
                 si += hi;
                 tr0 = fma(hr, y0, tr0); tr1 = fma(hr, y1, tr1); tr2 = fma(hr, y2, tr2);
@@ -189,7 +222,7 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
             for (int J = 0; J < BMI_MAX_RULE; ++J) {
                 if (J >= rule.nb) break;
                 double hr, hi;
-                green64(op, rule.w4pi[I][J], ux - yp[3 * J], uy - yp[3 * J + 1], uz - yp[3 * J + 2], hr, hi);
+                green64(op, rule.w4pi[I][J], ux - yp[3 * J], uy - yp[3 * J + 1], uz - yp[3 * J + 2], hr, hi, tab);
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
                     vr[j] = fma(hr, rule.lb[J][j], vr[j]);
@@ -219,8 +252,9 @@ __device__ __forceinline__ void green32(const OpConst &op, float w4pi, float dx,
     const float R = R2 * rinv;
     const double tt = (double)R * op.k2pi;
     const float f = (float)(tt - rint(tt));
-    float s, c;
-    sincos_turns32(f, s, c);
+    // v_sin_f32 / v_cos_f32 take the angle in turns; |f| <= 1/2
+    const float s = __builtin_amdgcn_sinf(f);
+    const float c = __builtin_amdgcn_cosf(f);
     float h = w4pi * rinv;
     if (op.lossy) h *= __expf(-(float)op.gamma_re * R);
     hr = h * c;
@@ -885,8 +919,10 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
     __shared__ int s_rs[BMI_PANEL_ROWS + 1];
     __shared__ unsigned char s_rkl[BMI_MAX_ROWS];
     __shared__ double s_rcf[BMI_MAX_ROWS];
+    __shared__ double2 s_sc[64];
 
     const int t = threadIdx.x;
+    if (!MIXED) fill_sincos_table(s_sc);
     const int w = blockIdx.x;
     const int P = w / a.nbq;
     const int q = a.bq_list[w - P * a.nbq];
@@ -959,7 +995,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
                 const double *yc = &bg[BMI_GEO_CEN + 3 * ll];
                 const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
                 if (MIXED) pair_mixed<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
-                else pair_fp64<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
+                else pair_fp64<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
             } else {
 #pragma unroll
                 for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
@@ -1041,7 +1077,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
     __shared__ double s_tscf[2][BMI_MAX_ROWS];
     __shared__ int s_cfn[BMI_MAX_ROWS];
     __shared__ unsigned char s_cfirst[BMI_MAX_ROWS];
-    __shared__ int s_sidx[BMI_BLOCK_THREADS];
+    __shared__ double2 s_sc[64];
 
     const int t = threadIdx.x;
     const int w = blockIdx.x;
@@ -1051,6 +1087,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
     const int nr = min(BMI_PANEL_ROWS, a.nrows - r0);
     const int nc = a.bnc[q];
     const int ncm = a.ncm;
+    if (!MIXED) fill_sincos_table(s_sc);
     const int kl = t >> 4, ll = t & 15;
     const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
 
@@ -1118,19 +1155,15 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
             const double *yc = &bg[BMI_GEO_CEN + 3 * ll];
             const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
             if (MIXED) pair_mixed<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
-            else pair_fp64<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
+            else pair_fp64<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
         } else {
 #pragma unroll
             for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
         }
-        // special pairs of this block take their precomputed values
-        if (e1 > e0) {
-            s_sidx[t] = -1;
-            __syncthreads();
-            for (int e = e0 + t; e < e1; e += BMI_BLOCK_THREADS) s_sidx[a.spec_loc[e]] = e;
-            __syncthreads();
-            const int e = s_sidx[t];
-            if (e >= 0) {
+        // special pairs of this block take their precomputed values (the
+        // block's entries are few; every thread scans them for its slot)
+        for (int e = e0; e < e1; ++e) {
+            if (a.spec_loc[e] == t) {
 #pragma unroll
                 for (int v = 0; v < 9; ++v) loc[v] = a.spec_val[9 * (long long)e + v];
             }
