@@ -1057,6 +1057,7 @@ bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
     B.rounds_b = P->Y->rounds_b;
     B.ncm = P->Y->ncmax;
     B.rr = P->X->rounds_t * P->Y->rounds_b <= 6;
+    B.fast = P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
     B.npb = P->npb;
     B.blk_off = P->d_blk_off.p;
     B.spec_loc = P->d_spec_loc.p;

@@ -104,6 +104,7 @@ struct PanelArgs {
     int rounds_t, rounds_b;      // largest rank + 1 on each side
     int ncm;                     // largest number of columns of a trial patch
     int rr;                      // 1: rank-round kernel, 0: pair buffer + gather
+    int fast;                    // 1: default 3 x 4 point far rule and a lossless operator (compile-time rule)
     const int *bq_list;          // trial patches of this launch's colour
     int nbq;
     int npb;                     // trial patches in total (block id = sub-batch * npb + q)

@@ -113,20 +113,60 @@ __device__ __forceinline__ void fill_sincos_table(double2 *tab) {
     }
 }
 
-// G * w for one point pair in FP64: h = w4pi * exp(-gamma R) / R, returned as (re, im).
-__device__ __forceinline__ void green64(const OpConst &op, double w4pi, double dx, double dy, double dz,
-                                        double &hr, double &hi, const double2 *tab = nullptr) {
+// G * w for one point pair in FP64: h = w4pi * exp(-gamma R) / R, returned as
+// (re, im).  TAB: sine and cosine from the 64-entry table in LDS, else from the
+// quarter-turn polynomials; lossy = false drops the attenuation factor.
+template <bool TAB>
+__device__ __forceinline__ void green64(const OpConst &op, bool lossy, double w4pi, double dx, double dy, double dz,
+                                        double &hr, double &hi, const double2 *tab) {
     const double R2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double rinv = rsqrt64(R2);
     const double R = R2 * rinv;
     double s, c;
-    if (tab) sincos_turns64_tab(R * op.k2pi, tab, s, c);
+    if (TAB) sincos_turns64_tab(R * op.k2pi, tab, s, c);
     else sincos_turns64(R * op.k2pi, s, c);
     double h = w4pi * rinv;
-    if (op.lossy) h *= exp(-op.gamma_re * R);
+    if (lossy) h *= exp(-op.gamma_re * R);
     hr = h * c;
     hi = -h * s;
 }
+
+// The far triangle rules of order 1..3 as compile-time values, equal to the
+// ones the host writes into RuleConst (weights on the reference simplex sum to
+// 1/2; w4pi = w_I w_J / (4 pi) in the same operation order).
+__host__ __device__ constexpr int frule_n(int o) { return o == 1 ? 1 : (o == 2 ? 3 : 4); }
+__host__ __device__ constexpr double frule_w(int o, int i) {
+    return o == 1 ? 0.5 : (o == 2 ? 1.0 / 6.0 : (i == 0 ? -27.0 / 96.0 : 25.0 / 96.0));
+}
+__host__ __device__ constexpr double frule_l(int o, int i, int d) {
+    return o == 1 ? 1.0 / 3.0
+                  : (o == 2 ? (i == d ? 2.0 / 3.0 : 1.0 / 6.0)
+                            : (i == 0 ? 1.0 / 3.0 : (i - 1 == d ? 3.0 / 5.0 : 1.0 / 5.0)));
+}
+#define BMI_FAST_TEST_ORDER 2
+#define BMI_FAST_TRIAL_ORDER 3
+
+// Far rule seen by the pair integrators: FAST = 1 is the default 3 x 4 point
+// rule of a lossless operator, fixed at compile time (straight-line code, no
+// rule loads); FAST = 0 reads the rule and the attenuation flag at run time.
+template <int FAST>
+struct FarRule {
+    const RuleConst &r;
+    const OpConst &op;
+    __device__ __forceinline__ int nt() const { return FAST ? frule_n(BMI_FAST_TEST_ORDER) : r.nt; }
+    __device__ __forceinline__ int nb() const { return FAST ? frule_n(BMI_FAST_TRIAL_ORDER) : r.nb; }
+    __device__ __forceinline__ bool lossy() const { return FAST ? false : op.lossy != 0; }
+    __device__ __forceinline__ double w4pi(int I, int J) const {
+        return FAST ? frule_w(BMI_FAST_TEST_ORDER, I) * frule_w(BMI_FAST_TRIAL_ORDER, J) / (4.0 * 3.14159265358979323846)
+                    : r.w4pi[I][J];
+    }
+    __device__ __forceinline__ double lt(int I, int d) const {
+        return FAST ? frule_l(BMI_FAST_TEST_ORDER, I, d) : r.lt[I][d];
+    }
+    __device__ __forceinline__ double lb(int J, int d) const {
+        return FAST ? frule_l(BMI_FAST_TRIAL_ORDER, J, d) : r.lb[J][d];
+    }
+};
 
 // ------------------------------------------------- moment-form epilogues
 
@@ -177,26 +217,28 @@ __device__ __forceinline__ void hh_epilogue(const OpConst &op, const double *pr,
 // (doubleQuadRuleMainCase.jl:83-101), FP64 throughout.
 //   xp, yp: rule points relative to the cell centroids (BMI_MAX_RULE x 3)
 //   xa, ya: vertices relative to the centroids (3 x 3)
-template <int OP>
-__device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &rule, const double *xp,
+template <int OP, int FAST, bool TAB>
+__device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &rule_, const double *xp,
                                           const double *xa, const double *xc, double xs, const double *yp,
                                           const double *ya, const double *yc, double ys, double2 loc[9],
-                                          const double2 *tab = nullptr) {
+                                          const double2 *tab) {
+    const FarRule<FAST> rule{rule_, op};
+    const bool lossy = rule.lossy();
     const double dcx = xc[0] - yc[0], dcy = xc[1] - yc[1], dcz = xc[2] - yc[2];
     if (OP == OPK_MW) {
         MomMW m = {};
 #pragma unroll
         for (int I = 0; I < BMI_MAX_RULE; ++I) {
-            if (I >= rule.nt) break;
+            if (I >= rule.nt()) break;
             const double x0 = xp[3 * I], x1 = xp[3 * I + 1], x2 = xp[3 * I + 2];
             const double ux = dcx + x0, uy = dcy + x1, uz = dcz + x2;
             double sr = 0, si = 0, tr0 = 0, tr1 = 0, tr2 = 0, ti0 = 0, ti1 = 0, ti2 = 0;
 #pragma unroll
             for (int J = 0; J < BMI_MAX_RULE; ++J) {
-                if (J >= rule.nb) break;
+                if (J >= rule.nb()) break;
                 const double y0 = yp[3 * J], y1 = yp[3 * J + 1], y2 = yp[3 * J + 2];
                 double hr, hi;
-                green64(op, rule.w4pi[I][J], ux - y0, uy - y1, uz - y2, hr, hi, tab);
+                green64<TAB>(op, lossy, rule.w4pi(I, J), ux - y0, uy - y1, uz - y2, hr, hi, tab);
                 sr += hr;
                 si += hi;
                 tr0 = fma(hr, y0, tr0); tr1 = fma(hr, y1, tr1); tr2 = fma(hr, y2, tr2);
@@ -215,26 +257,26 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
         double pr[9] = {}, pi[9] = {};
 #pragma unroll
         for (int I = 0; I < BMI_MAX_RULE; ++I) {
-            if (I >= rule.nt) break;
+            if (I >= rule.nt()) break;
             const double ux = dcx + xp[3 * I], uy = dcy + xp[3 * I + 1], uz = dcz + xp[3 * I + 2];
             double vr[3] = {}, vi[3] = {};
 #pragma unroll
             for (int J = 0; J < BMI_MAX_RULE; ++J) {
-                if (J >= rule.nb) break;
+                if (J >= rule.nb()) break;
                 double hr, hi;
-                green64(op, rule.w4pi[I][J], ux - yp[3 * J], uy - yp[3 * J + 1], uz - yp[3 * J + 2], hr, hi, tab);
+                green64<TAB>(op, lossy, rule.w4pi(I, J), ux - yp[3 * J], uy - yp[3 * J + 1], uz - yp[3 * J + 2], hr, hi, tab);
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    vr[j] = fma(hr, rule.lb[J][j], vr[j]);
-                    vi[j] = fma(hi, rule.lb[J][j], vi[j]);
+                    vr[j] = fma(hr, rule.lb(J, j), vr[j]);
+                    vi[j] = fma(hi, rule.lb(J, j), vi[j]);
                 }
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    pr[3 * i + j] = fma(rule.lt[I][i], vr[j], pr[3 * i + j]);
-                    pi[3 * i + j] = fma(rule.lt[I][i], vi[j], pi[3 * i + j]);
+                    pr[3 * i + j] = fma(rule.lt(I, i), vr[j], pr[3 * i + j]);
+                    pi[3 * i + j] = fma(rule.lt(I, i), vi[j], pi[3 * i + j]);
                 }
         }
         hh_epilogue(op, pr, pi, 4.0 * xs * ys, loc);
@@ -245,7 +287,7 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
 
 // FP32 evaluation with the phase reduced modulo one turn (2 pi) in FP64
 // before the FP32 sin/cos; per-pair sums in FP32, result widened to FP64.
-__device__ __forceinline__ void green32(const OpConst &op, float w4pi, float dx, float dy, float dz, float &hr,
+__device__ __forceinline__ void green32(const OpConst &op, bool lossy, float w4pi, float dx, float dy, float dz, float &hr,
                                         float &hi) {
     const float R2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = __builtin_amdgcn_rsqf(R2);
@@ -256,15 +298,17 @@ __device__ __forceinline__ void green32(const OpConst &op, float w4pi, float dx,
     const float s = __builtin_amdgcn_sinf(f);
     const float c = __builtin_amdgcn_cosf(f);
     float h = w4pi * rinv;
-    if (op.lossy) h *= __expf(-(float)op.gamma_re * R);
+    if (lossy) h *= __expf(-(float)op.gamma_re * R);
     hr = h * c;
     hi = -h * s;
 }
 
-template <int OP>
-__device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &rule, const double *xp,
+template <int OP, int FAST>
+__device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &rule_, const double *xp,
                                            const double *xa, const double *xc, double xs, const double *yp,
                                            const double *ya, const double *yc, double ys, double2 loc[9]) {
+    const FarRule<FAST> rule{rule_, op};
+    const bool lossy = rule.lossy();
     const float dcx = (float)(xc[0] - yc[0]), dcy = (float)(xc[1] - yc[1]), dcz = (float)(xc[2] - yc[2]);
     float ypf[3 * BMI_MAX_RULE];
 #pragma unroll
@@ -273,16 +317,16 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
         float m0r = 0, m0i = 0, mxr[3] = {}, mxi[3] = {}, myr[3] = {}, myi[3] = {}, mxyr = 0, mxyi = 0;
 #pragma unroll
         for (int I = 0; I < BMI_MAX_RULE; ++I) {
-            if (I >= rule.nt) break;
+            if (I >= rule.nt()) break;
             const float x0 = (float)xp[3 * I], x1 = (float)xp[3 * I + 1], x2 = (float)xp[3 * I + 2];
             const float ux = dcx + x0, uy = dcy + x1, uz = dcz + x2;
             float sr = 0, si = 0, tr0 = 0, tr1 = 0, tr2 = 0, ti0 = 0, ti1 = 0, ti2 = 0;
 #pragma unroll
             for (int J = 0; J < BMI_MAX_RULE; ++J) {
-                if (J >= rule.nb) break;
+                if (J >= rule.nb()) break;
                 const float y0 = ypf[3 * J], y1 = ypf[3 * J + 1], y2 = ypf[3 * J + 2];
                 float hr, hi;
-                green32(op, (float)rule.w4pi[I][J], ux - y0, uy - y1, uz - y2, hr, hi);
+                green32(op, lossy, (float)rule.w4pi(I, J), ux - y0, uy - y1, uz - y2, hr, hi);
                 sr += hr;
                 si += hi;
                 tr0 = fmaf(hr, y0, tr0); tr1 = fmaf(hr, y1, tr1); tr2 = fmaf(hr, y2, tr2);
@@ -319,26 +363,26 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
         float pr[9] = {}, pi[9] = {};
 #pragma unroll
         for (int I = 0; I < BMI_MAX_RULE; ++I) {
-            if (I >= rule.nt) break;
+            if (I >= rule.nt()) break;
             const float ux = dcx + (float)xp[3 * I], uy = dcy + (float)xp[3 * I + 1], uz = dcz + (float)xp[3 * I + 2];
             float vr[3] = {}, vi[3] = {};
 #pragma unroll
             for (int J = 0; J < BMI_MAX_RULE; ++J) {
-                if (J >= rule.nb) break;
+                if (J >= rule.nb()) break;
                 float hr, hi;
-                green32(op, (float)rule.w4pi[I][J], ux - ypf[3 * J], uy - ypf[3 * J + 1], uz - ypf[3 * J + 2], hr, hi);
+                green32(op, lossy, (float)rule.w4pi(I, J), ux - ypf[3 * J], uy - ypf[3 * J + 1], uz - ypf[3 * J + 2], hr, hi);
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    vr[j] = fmaf(hr, (float)rule.lb[J][j], vr[j]);
-                    vi[j] = fmaf(hi, (float)rule.lb[J][j], vi[j]);
+                    vr[j] = fmaf(hr, (float)rule.lb(J, j), vr[j]);
+                    vi[j] = fmaf(hi, (float)rule.lb(J, j), vi[j]);
                 }
             }
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    pr[3 * i + j] = fmaf((float)rule.lt[I][i], vr[j], pr[3 * i + j]);
-                    pi[3 * i + j] = fmaf((float)rule.lt[I][i], vi[j], pi[3 * i + j]);
+                    pr[3 * i + j] = fmaf((float)rule.lt(I, i), vr[j], pr[3 * i + j]);
+                    pi[3 * i + j] = fmaf((float)rule.lt(I, i), vi[j], pi[3 * i + j]);
                 }
         }
         const float sc = 4.0f * (float)(xs * ys);
@@ -607,7 +651,7 @@ __device__ __forceinline__ void spec_point(const OpConst &op, SpecAcc &acc, cons
         y[d] = Y.o[d] + Y.t1[d] * by1 + Y.t2[d] * by2;
     }
     double hr, hi;
-    green64(op, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1], dc[2] + x[2] - y[2], hr, hi);
+    green64<false>(op, op.lossy != 0, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1], dc[2] + x[2] - y[2], hr, hi, nullptr);
     if (OP == OPK_MW) {
         acc.v[0] += hr;
         acc.v[1] += hi;
@@ -843,7 +887,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(SpecialArgs a) {
         ya[v] = a.Y.vtx[9 * L + v] - yc[v % 3];
     }
     double2 loc[9];
-    pair_fp64<OP>(a.op, a.rule, xp, xa, xc, a.X.area[K], yp, ya, yc, a.Y.area[L], loc);
+    pair_fp64<OP, 0, false>(a.op, a.rule, xp, xa, xc, a.X.area[K], yp, ya, yc, a.Y.area[L], loc, nullptr);
 #pragma unroll
     for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
 }
@@ -906,7 +950,7 @@ __device__ __forceinline__ void out_write(float2 *Z, long long off, double re, d
 //   at the end:     each wave writes whole 64-row column segments (1 KB, contiguous)
 // The records of sub-batch s + 1 are read into registers while s is being
 // integrated.  Thread t owns row t % 64 and the columns t / 64 + 4 j, j < 12.
-template <int OP, int MIXED, typename OutT>
+template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
     __shared__ double2 V[9 * BMI_BLOCK_THREADS];  // V[v * 256 + kl * 16 + ll]
     __shared__ double sg[BMI_GEO];                // test sub-batch geometry
@@ -994,8 +1038,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
                 const double *yp = &bg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * ll];
                 const double *yc = &bg[BMI_GEO_CEN + 3 * ll];
                 const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
-                if (MIXED) pair_mixed<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
-                else pair_fp64<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
+                if (MIXED) pair_mixed<OP, FAST>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
+                else pair_fp64<OP, FAST, true>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
             } else {
 #pragma unroll
                 for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
@@ -1068,7 +1112,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
 // barrier between rounds, so no entry is touched twice inside a round and
 // the summation order is fixed.  RWG functions have at most two cells, so
 // there are at most 2 x 2 rounds.
-template <int OP, int MIXED, typename OutT>
+template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a) {
     extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x ncm
     __shared__ double sg[2][BMI_GEO];             // test sub-batch geometry, double-buffered
@@ -1154,8 +1198,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
             const double *yp = &bg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * ll];
             const double *yc = &bg[BMI_GEO_CEN + 3 * ll];
             const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
-            if (MIXED) pair_mixed<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
-            else pair_fp64<OP>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
+            if (MIXED) pair_mixed<OP, FAST>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
+            else pair_fp64<OP, FAST, true>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
         } else {
 #pragma unroll
             for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
@@ -1214,32 +1258,38 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a)
     }
 }
 
-template <int OP, int MIXED>
+template <int OP, int MIXED, int FAST>
 static void launch_panels_t(const PanelArgs &a, int c32, hipStream_t s) {
     const unsigned grid = (unsigned)((long long)a.npanel * a.nbq);
     if (a.rr) {
         // few rounds (RWG: at most 2): rank rounds into an LDS tile
         const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * a.ncm;
         if (c32)
-            hipLaunchKernelGGL((k_panels_rr<OP, MIXED, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
+            hipLaunchKernelGGL((k_panels_rr<OP, MIXED, FAST, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
         else
-            hipLaunchKernelGGL((k_panels_rr<OP, MIXED, double2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
+            hipLaunchKernelGGL((k_panels_rr<OP, MIXED, FAST, double2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
         return;
     }
     if (c32)
-        hipLaunchKernelGGL((k_panels<OP, MIXED, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_panels<OP, MIXED, FAST, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
     else
-        hipLaunchKernelGGL((k_panels<OP, MIXED, double2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_panels<OP, MIXED, FAST, double2>), dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+}
+
+template <int OP, int MIXED>
+static void launch_panels_m(const PanelArgs &a, int c32, hipStream_t s) {
+    if (a.fast) launch_panels_t<OP, MIXED, 1>(a, c32, s);
+    else launch_panels_t<OP, MIXED, 0>(a, c32, s);
 }
 
 hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hipStream_t s) {
     if (a.npanel <= 0 || a.nbq <= 0) return hipSuccess;
     if (op_kind == OPK_MW) {
-        if (mixed) launch_panels_t<OPK_MW, 1>(a, c32, s);
-        else launch_panels_t<OPK_MW, 0>(a, c32, s);
+        if (mixed) launch_panels_m<OPK_MW, 1>(a, c32, s);
+        else launch_panels_m<OPK_MW, 0>(a, c32, s);
     } else {
-        if (mixed) launch_panels_t<OPK_HH3D, 1>(a, c32, s);
-        else launch_panels_t<OPK_HH3D, 0>(a, c32, s);
+        if (mixed) launch_panels_m<OPK_HH3D, 1>(a, c32, s);
+        else launch_panels_m<OPK_HH3D, 0>(a, c32, s);
     }
     return hipGetLastError();
 }
