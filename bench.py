@@ -39,6 +39,17 @@ HBM_PEAK_GBS = 8000.0
 # div/sqrt/sin/cos/exp = 1 each
 FLOPS_FAR_PAIR = {"efie": 12 * 52 + 300, "hh3d": 12 * 62 + 27}
 FLOPS_SS_NODE = {"efie": 107, "hh3d": 117}
+FAR_KERNEL = {"efie": "k_panels_rr<MW, FP64, fixed 3x4 rule, C64> (far pairs + write-back)",
+              "hh3d": "k_panels_rr<HH3D, FP64, fixed 3x4 rule, C64> (far pairs + write-back)"}
+# HBM bytes per far-kernel launch (FETCH_SIZE + WRITE_SIZE over the launches of
+# one assembly, divided by the launches, each counter scaled by its reading on a
+# device copy of known size); from rocprofv3 --pmc runs of this script, kept
+# in profiles/k_panels_rr_efie_L5_pmc.md
+TRAFFIC = {
+    "efie-L5": {"bytes_per_launch": 4.939e9, "read_bytes_per_launch": 1.469e9, "write_bytes_per_launch": 3.470e9,
+                "algorithmic_bytes_per_launch": 2.517e9, "fetch_scale": 2.0, "write_scale": 1.0,
+                "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, FP64 C64 far-pair kernel, 6 launches"},
+}
 
 
 def workload(name: str):
@@ -111,6 +122,20 @@ def cpu_baseline(w, seconds=15.0):
     }
 
 
+def calibrate_copy(mib: int, reps: int = 3):
+    """Device copies of a known size: what FETCH_SIZE / WRITE_SIZE read for
+    them scales the counters of the assembly kernels to bytes."""
+    import torch
+
+    n = mib * (1 << 20)
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,7 +145,12 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: multi-GPU runs are not part of this build "
               "(assembly shards by test rows with no exchange)", file=sys.stderr)
@@ -167,14 +197,16 @@ def main():
         "roofline": {
             "bound": "mfma",
             "pipe": "VALU FP64 (no MFMA: sqrt/sincos evaluation); peak = MI355X FP64 78.6 TFLOP/s",
-            "kernel": "k_blocks (far pairs + write-back)",
+            "kernel": FAR_KERNEL[kind],
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC.get(args.workload),
             "flops_per_far_pair": FLOPS_FAR_PAIR[kind],
             "far_launches_per_step": st["far_launches"],
+            "avg_launch_ms": far_ms / max(1, st["far_launches"]),
+            "flops_per_launch": far_flops / max(1, st["far_launches"]),
             "write_gbs": entries * 16 / (far_ms * 1e-3) / 1e9,
         },
     }

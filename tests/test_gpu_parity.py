@@ -114,3 +114,40 @@ def test_config2_sampled_rows_and_mixed(gpu_ctx):
     assert np.linalg.norm(ym - y64) <= MIXED_TOL * np.linalg.norm(y64)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("kind", ["efie", "hh3d"])
+@pytest.mark.parametrize("far_test,far_trial", [(3, 2), (1, 3), (2, 1)])
+def test_nondefault_far_rules_match_oracle(gpu_ctx, kind, far_test, far_trial):
+    """Far-rule orders other than the default 3 x 4 points run the kernels
+    that read the rule at run time."""
+    op, X = _case(kind, 2, 1.5, 0.02, seed=5)
+    print(f"case rules {kind} {far_test}x{far_trial}", flush=True)
+    cfg = GPUConfiguration(far_test_order=far_test, far_trial_order=far_trial)
+    a = Assembler(op, X, X, cfg, ctx=gpu_ctx)
+    Z = a.assemble().to_host()
+    a.close()
+    Zref, _ = O.assemble(op, X, X, far_test=far_test, far_trial=far_trial)
+    err = relfro(Z, Zref)
+    print("rel err", err)
+    assert err <= FP64_TOL
+
+
+@pytest.mark.parametrize("kind", ["efie", "hh3d"])
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("mixed", MIXED_TOL)])
+def test_lossy_operator_matches_oracle(gpu_ctx, kind, precision, tol):
+    """Complex gamma (attenuating medium): the exp(-Re(gamma) R) factor."""
+    m = meshes.perturb_radial(meshes.icosphere(2), 6, 0.02)
+    g = complex(0.4, 1.7)
+    if kind == "efie":
+        op, X = Maxwell3D.singlelayer(gamma=g), spaces.raviartthomas(m)
+    else:
+        op, X = Helmholtz3D.singlelayer(gamma=g), spaces.lagrangec0d1(m)
+    print(f"case lossy {kind} {precision}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(precision=precision), ctx=gpu_ctx)
+    Z = a.assemble().to_host()
+    a.close()
+    Zref, _ = O.assemble(op, X, X)
+    err = relfro(Z, Zref)
+    print("rel err", err)
+    assert err <= tol
