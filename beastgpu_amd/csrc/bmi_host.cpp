@@ -157,9 +157,9 @@ struct bmi_space {
     std::vector<unsigned char> row_first, slot_kl;
     std::vector<double> slot_cf;
     std::vector<std::vector<int>> by_colour;
-    DevBuf<int> d_patch_tri, d_row_ptr, d_row_fn, d_row_slot_ptr, d_by_colour;
-    DevBuf<unsigned char> d_row_first, d_slot_kl;
-    DevBuf<double> d_slot_cf, d_patch_sph, d_patch_amax;
+    DevBuf<int> d_patch_tri, d_by_colour;
+    DevBuf<double> d_patch_sph, d_patch_amax;
+    DevBuf<int> d_home_patch, d_home_sb;  // cell -> its patch / its first sub-batch (-1: none)
     std::vector<int> colour_off;  // offsets of each colour's list in d_by_colour
     // test role: panels of BMI_PANEL_ROWS functions in sub-batches of BMI_PATCH cells
     int npanel = 0, nsb = 0;
@@ -219,12 +219,7 @@ DevSpace dev_view(const bmi_space *s, const double *pts) {
     d.patch_tri = s->d_patch_tri.p;
     d.patch_sph = s->d_patch_sph.p;
     d.patch_amax = s->d_patch_amax.p;
-    d.row_ptr = s->d_row_ptr.p;
-    d.row_fn = s->d_row_fn.p;
-    d.row_first = s->d_row_first.p;
-    d.row_slot_ptr = s->d_row_slot_ptr.p;
-    d.slot_kl = s->d_slot_kl.p;
-    d.slot_cf = s->d_slot_cf.p;
+    d.home = s->d_home_patch.p;
     return d;
 }
 
@@ -235,12 +230,7 @@ DevSpace dev_view_test(const bmi_space *s, const double *pts) {
     d.patch_tri = s->d_sb_cells.p;
     d.patch_sph = s->d_sb_sph.p;
     d.patch_amax = s->d_sb_amax.p;
-    d.row_ptr = nullptr;
-    d.row_fn = nullptr;
-    d.row_first = nullptr;
-    d.row_slot_ptr = nullptr;
-    d.slot_kl = nullptr;
-    d.slot_cf = nullptr;
+    d.home = s->d_home_sb.p;
     return d;
 }
 
@@ -426,8 +416,16 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
                 }
             }
         }
+    // home sub-batch of each cell: the first one that holds it
+    std::vector<int> home(nt, -1);
+    for (int b = 0; b < s->nsb; ++b)
+        for (int k = 0; k < BMI_PATCH; ++k) {
+            const int K = s->sb_cells[(size_t)b * BMI_PATCH + k];
+            if (K >= 0 && home[K] < 0) home[K] = b;
+        }
     hipError_t e;
-    if ((e = s->d_tslot.upload(tslot)) != hipSuccess || (e = s->d_tscf.upload(tscf)) != hipSuccess)
+    if ((e = s->d_tslot.upload(tslot)) != hipSuccess || (e = s->d_tscf.upload(tscf)) != hipSuccess ||
+        (e = s->d_home_sb.upload(home)) != hipSuccess)
         return fail(BMI_EDEVICE, std::string("panel upload: ") + hipGetErrorString(e));
     if ((e = s->d_sb_off.upload(s->sb_off)) != hipSuccess || (e = s->d_sb_cells.upload(s->sb_cells)) != hipSuccess ||
         (e = s->d_sb_sph.upload(s->sb_sph)) != hipSuccess || (e = s->d_sb_amax.upload(s->sb_amax)) != hipSuccess ||
@@ -581,12 +579,14 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
         s->colour_off[c + 1] = (int)flat.size();
     }
 
+    std::vector<int> home(nt, -1);
+    for (int p = 0; p < np; ++p)
+        for (int k = 0; k < BMI_PATCH; ++k) {
+            const int K = s->patch_tri[(size_t)p * BMI_PATCH + k];
+            if (K >= 0) home[K] = p;
+        }
     hipError_t e;
-    if ((e = s->d_patch_tri.upload(s->patch_tri)) != hipSuccess ||
-        (e = s->d_row_ptr.upload(s->row_ptr)) != hipSuccess || (e = s->d_row_fn.upload(s->row_fn)) != hipSuccess ||
-        (e = s->d_row_slot_ptr.upload(s->row_slot_ptr)) != hipSuccess ||
-        (e = s->d_row_first.upload(s->row_first)) != hipSuccess ||
-        (e = s->d_slot_kl.upload(s->slot_kl)) != hipSuccess || (e = s->d_slot_cf.upload(s->slot_cf)) != hipSuccess ||
+    if ((e = s->d_patch_tri.upload(s->patch_tri)) != hipSuccess || (e = s->d_home_patch.upload(home)) != hipSuccess ||
         (e = s->d_by_colour.upload(flat)) != hipSuccess || (e = s->d_patch_sph.upload(s->patch_sph)) != hipSuccess ||
         (e = s->d_patch_amax.upload(s->patch_amax)) != hipSuccess)
         return fail(BMI_EDEVICE, std::string("space upload: ") + hipGetErrorString(e));
@@ -960,7 +960,7 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     BMI_HIP(hipStreamSynchronize(s));
     {
         std::vector<std::pair<int64_t, unsigned char>> keys(total);
-        for (int i = 0; i < total; ++i) keys[i] = {(int64_t)sk[i] * trial->mesh->nt + sl[i], cls[i]};
+        for (int i = 0; i < total; ++i) keys[i] = {(int64_t)sk[i] * trial->mesh->nt + sl[i], cls[i] & 0x7f};
         std::sort(keys.begin(), keys.end());
         for (int i = 0; i < total; ++i)
             if (i == 0 || keys[i].first != keys[i - 1].first) P->pairs[keys[i].second < 6 ? keys[i].second : 0]++;
@@ -976,6 +976,8 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     S.spec_L = P->d_spec_L.p;
     S.spec_cls = P->d_spec_cls.p;
     S.n_special = P->d_total.p;
+    S.blk_off = P->d_blk_off.p;
+    S.npb = P->npb;
     S.capacity = total;
     S.spec_val = P->d_spec_val.p;
     S.op = P->opc;

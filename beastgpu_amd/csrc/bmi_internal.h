@@ -27,6 +27,9 @@ static_assert(BMI_PANEL_ROWS <= 64, "a panel's rows are tracked in a 64-bit mask
 
 enum { OPK_HH3D = 1, OPK_MW = 2 };
 enum { CLS_FAR = 0, CLS_CV = 1, CLS_CE = 2, CLS_CF = 3, CLS_NEAR = 4, CLS_FAR64 = 5 };
+// flag on a special-pair entry's class: the pair is integrated in the entry
+// of the test cell's home group and copied here
+#define SPEC_NOT_OWNER 0x80
 
 // Device view of one function space on one mesh.
 struct DevSpace {
@@ -39,12 +42,7 @@ struct DevSpace {
     const int *patch_tri;        // npatch x BMI_PATCH, -1 padded
     const double *patch_sph;     // npatch x 4: bounding sphere (centre, radius) of the cells' vertices
     const double *patch_amax;    // npatch: largest cell area
-    const int *row_ptr;          // npatch + 1: rows of each patch
-    const int *row_fn;           // global function of each row
-    const unsigned char *row_first;  // 1 if this patch is the first (in launch order) to write the row
-    const int *row_slot_ptr;     // rows + 1
-    const unsigned char *slot_kl;    // local cell * 3 + local shape
-    const double *slot_cf;       // coefficient of the slot
+    const int *home;             // nt: the group a cell calls home (its first group; -1 if in none)
 };
 
 struct OpConst {
@@ -136,6 +134,8 @@ struct SpecialArgs {
     const int *spec_K, *spec_L;
     const unsigned char *spec_cls;
     const int *n_special;
+    const int *blk_off;
+    int npb;
     int capacity;        // special-pair entries allocated
     double2 *spec_val;
     OpConst op;

@@ -13,6 +13,8 @@
 //   make_complex                     assamble_gpu.jl:141-144          -> not needed: complex is written directly
 #include "bmi_internal.h"
 
+#include <algorithm>
+
 #define DTOL (2.220446049250313e-16 * 1.0e3)
 
 // ------------------------------------------------------------------ math
@@ -524,7 +526,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(ClassifyArgs a, 
                 if (e < a.capacity) {
                     a.spec_K[e] = K;
                     a.spec_L[e] = L;
-                    a.spec_cls[e] = (unsigned char)cls;
+                    // bit 7: the pair is integrated in the test cell's home group, not here
+                    a.spec_cls[e] = (unsigned char)(cls | (a.X.home[K] == p ? 0 : SPEC_NOT_OWNER));
                     a.spec_loc[e] = (unsigned char)t;
                 }
             }
@@ -631,8 +634,6 @@ struct SpecAcc {
     double v[18];
 };
 
-// reference simplex vertices of the permuted charts (pulledback_integrand_gpu.jl:201-202)
-__constant__ double c_vref[3][2] = {{1.0, 0.0}, {0.0, 1.0}, {0.0, 0.0}};
 
 struct CellG {
     double o[3];   // v3 - centroid
@@ -641,7 +642,7 @@ struct CellG {
 };
 
 template <int OP>
-__device__ __forceinline__ void spec_point(const OpConst &op, SpecAcc &acc, const CellG &X, const CellG &Y,
+__device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const double2 *tab, SpecAcc &acc, const CellG &X, const CellG &Y,
                                            const double *dc, double bx1, double bx2, double by1, double by2,
                                            double w) {
     double x[3], y[3];
@@ -651,7 +652,8 @@ __device__ __forceinline__ void spec_point(const OpConst &op, SpecAcc &acc, cons
         y[d] = Y.o[d] + Y.t1[d] * by1 + Y.t2[d] * by2;
     }
     double hr, hi;
-    green64<false>(op, op.lossy != 0, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1], dc[2] + x[2] - y[2], hr, hi, nullptr);
+    green64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1], dc[2] + x[2] - y[2], hr, hi,
+                  tab);
     if (OP == OPK_MW) {
         acc.v[0] += hr;
         acc.v[1] += hi;
@@ -680,57 +682,147 @@ __device__ __forceinline__ void spec_point(const OpConst &op, SpecAcc &acc, cons
     }
 }
 
-struct RefChart {
-    double v3[2], t1[2], t2[2];
-};
 
-template <int OP>
-__device__ __forceinline__ void ss_region(const OpConst &op, SpecAcc &acc, const CellG &X, const CellG &Y,
-                                          const double *dc, const RefChart &c1, const RefChart &c2, double u1,
-                                          double u2, double v1, double v2, double mul) {
-    // calc_bary (Integrand_gpu.jl:51-55)
-    const double bx1 = c1.v3[0] + c1.t1[0] * u1 + c1.t2[0] * u2;
-    const double bx2 = c1.v3[1] + c1.t1[1] * u1 + c1.t2[1] * u2;
-    const double by1 = c2.v3[0] + c2.t1[0] * v1 + c2.t2[0] * v2;
-    const double by2 = c2.v3[1] + c2.t1[1] * v1 + c2.t2[1] * v2;
-    spec_point<OP>(op, acc, X, Y, dc, bx1, bx2, by1, by2, mul);
+
+// Hypercube maps of the Sauter-Schwab regions (SauterSchwab.jl:265-298):
+// region r of class cls at the node (e1, e2, e3, xi) with product weight w
+// gives the reference coordinates (u1, u2) of the test and (v1, v2) of the
+// trial chart and the weight times the region's Jacobian.
+__device__ __forceinline__ void ss_map(int cls, int r, double e1, double e2, double e3, double xi, double w,
+                                       double &u1, double &u2, double &v1, double &v2, double &m) {
+    const double xi3 = xi * xi * xi;
+    if (cls == CLS_CV) {
+        const double xe1 = xi * e1, xe2 = xi * e2;
+        m = w * xi3 * e2;
+        if (r == 0) { u1 = 1 - xi; u2 = xe1; v1 = 1 - xe2; v2 = xe2 * e3; }
+        else { u1 = 1 - xe2; u2 = xe2 * e3; v1 = 1 - xi; v2 = xe1; }
+    } else if (cls == CLS_CE) {
+        const double xe1 = xi * e1, e12 = e1 * e2, e23 = e2 * e3, e123 = e12 * e3;
+        m = w * xi3 * e1 * e1 * (r == 4 ? 1.0 : e2);
+        switch (r) {
+        case 0: u1 = 1 - xi; u2 = xe1; v1 = 1 - xi * (1 - e123); v2 = xe1 * e2 * (1 - e3); break;
+        case 1: u1 = 1 - xi * (1 - e12); u2 = xe1 * (1 - e2); v1 = 1 - xi; v2 = xe1 * e23; break;
+        case 2: u1 = 1 - xi * (1 - e123); u2 = xe1 * e2 * (1 - e3); v1 = 1 - xi; v2 = xe1; break;
+        case 3: u1 = 1 - xi * (1 - e123); u2 = xe1 * (1 - e23); v1 = 1 - xi; v2 = xe1 * e2; break;
+        default: u1 = 1 - xi; u2 = xe1 * e3; v1 = 1 - xi * (1 - e12); v2 = xe1 * (1 - e2); break;
+        }
+    } else {
+        const double p = xi * e1;
+        m = w * xi3 * e1 * e1 * e2;
+        switch (r) {
+        case 0: u1 = 1 - xi; u2 = xi - p + p * e2; v1 = 1 - (xi - p * e2 * e3); v2 = xi - p; break;
+        case 1: u1 = 1 - (xi - p * e2 * e3); u2 = xi - p; v1 = 1 - xi; v2 = xi - p + p * e2; break;
+        case 2: u1 = 1 - xi; u2 = p * (1 - e2 + e2 * e3); v1 = 1 - (xi - p * e2); v2 = p * (1 - e2); break;
+        case 3: u1 = 1 - (xi - p * e2); u2 = p * (1 - e2); v1 = 1 - xi; v2 = p * (1 - e2 + e2 * e3); break;
+        case 4: u1 = 1 - (xi - p * e2 * e3); u2 = p * (1 - e2 * e3); v1 = 1 - xi; v2 = p * (1 - e2); break;
+        default: u1 = 1 - xi; u2 = p * (1 - e2); v1 = 1 - (xi - p * e2 * e3); v2 = p * (1 - e2 * e3); break;
+        }
+    }
 }
 
-// One wave per special pair: Sauter-Schwab (classes 1-3) or the collapsed
-// near rule (class 4).  Lanes stride over quadrature nodes; a butterfly over
-// the wave reduces the accumulators.
+// One wave per touching or near pair (owner entries only:
+// a pair whose test cell belongs to several sub-batches is integrated once,
+// in the sub-batch the cell calls home).  Regions in turn, lanes striding
+// over the hypercube nodes, one Green evaluation each; a butterfly over the
+// wave reduces the accumulators.
 template <int OP>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_special(SpecialArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x) >> 6;
-    const int nwaves = gridDim.x * (BMI_BLOCK_THREADS / 64);
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_special(SpecialArgs a) {
+    __shared__ double s_glx[3][BMI_MAX_GL], s_glw[3][BMI_MAX_GL];
+    __shared__ double s_nb1[BMI_MAX_NEAR], s_nb2[BMI_MAX_NEAR], s_nw[BMI_MAX_NEAR];
+    __shared__ double2 s_sc[64];
+    __shared__ int s_items[BMI_BLOCK_THREADS];
+    __shared__ int s_wsum[BMI_BLOCK_THREADS / 64];
+    const int t = threadIdx.x;
+    fill_sincos_table(s_sc);
+    if (t < 3 * BMI_MAX_GL) {
+        s_glx[t / BMI_MAX_GL][t % BMI_MAX_GL] = a.glx[t / BMI_MAX_GL][t % BMI_MAX_GL];
+        s_glw[t / BMI_MAX_GL][t % BMI_MAX_GL] = a.glw[t / BMI_MAX_GL][t % BMI_MAX_GL];
+    }
+    if (t < BMI_MAX_NEAR) {
+        s_nb1[t] = a.nb1[t];
+        s_nb2[t] = a.nb2[t];
+        s_nw[t] = a.nw[t];
+    }
+    __syncthreads();
+    const int lane = t & 63, wv = t >> 6;
     const int ns = min(*a.n_special, a.capacity);
-    for (int e = wave; e < ns; e += nwaves) {
+    const bool lossy = a.op.lossy != 0;
+    // chunks of 256 entries: the threads pick out the touching and near
+    // entries this group owns, then each wave integrates every fourth of them
+    for (int c0 = blockIdx.x * BMI_BLOCK_THREADS; c0 < ns; c0 += gridDim.x * BMI_BLOCK_THREADS) {
+        const int ec = c0 + t;
+        int take = 0;
+        if (ec < ns) {
+            const int cl = a.spec_cls[ec];
+            take = !(cl & SPEC_NOT_OWNER) && cl >= CLS_CV && cl <= CLS_NEAR;
+        }
+        const unsigned long long m = __ballot(take);
+        __syncthreads();  // s_items, s_wsum of the previous chunk are consumed
+        if (lane == 0) s_wsum[wv] = __popcll(m);
+        __syncthreads();
+        int base = 0, nitem = 0;
+#pragma unroll
+        for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
+            base += i < wv ? s_wsum[i] : 0;
+            nitem += s_wsum[i];
+        }
+        if (take) s_items[base + __popcll(m & ((1ull << lane) - 1ull))] = ec;
+        __syncthreads();
+    for (int w = wv; w < nitem; w += BMI_BLOCK_THREADS / 64) {
+        const int e = s_items[w];
         const int cls = a.spec_cls[e];
-        if (cls < CLS_CV || cls > CLS_NEAR) continue;
         const int K = a.spec_K[e], L = a.spec_L[e];
-        double tvx[9], svx[9], tcn[3], scn[3];
+        // vertex permutations of the two cells (identity for near pairs):
+        // chart vertex c of the test cell is its vertex PT[c]
+        int PT[3] = {0, 1, 2}, PS[3] = {0, 1, 2};
+        if (cls != CLS_NEAR) {
+            // vertex correspondence (pulledback_integrand_gpu.jl:22-155)
+            int mi[3] = {0, 0, 0}, mj[3] = {0, 0, 0}, ne = 0;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            tvx[i] = a.X.vtx[9 * K + i];
-            svx[i] = a.Y.vtx[9 * L + i];
-        }
+            for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            tcn[d] = a.X.cen[3 * K + d];
-            scn[d] = a.Y.cen[3 * L + d];
+                for (int j = 0; j < 3; ++j) {
+                    const double dx = a.X.vtx[9 * K + 3 * i] - a.Y.vtx[9 * L + 3 * j];
+                    const double dy = a.X.vtx[9 * K + 3 * i + 1] - a.Y.vtx[9 * L + 3 * j + 1];
+                    const double dz = a.X.vtx[9 * K + 3 * i + 2] - a.Y.vtx[9 * L + 3 * j + 2];
+                    if (dx * dx + dy * dy + dz * dz < DTOL * DTOL && ne < 3) {
+                        mi[ne] = i;
+                        mj[ne] = j;
+                        ++ne;
+                    }
+                }
+            if (cls == CLS_CF) {
+                PS[0] = PS[1] = PS[2] = 0;
+                for (int k = 0; k < ne; ++k) PS[mi[k]] = mj[k];
+            } else if (cls == CLS_CE) {
+                // common edge between chart vertices 1 and 3
+                PT[0] = mi[1]; PT[1] = 3 - mi[0] - mi[1]; PT[2] = mi[0];
+                PS[0] = mj[1]; PS[1] = 3 - mj[0] - mj[1]; PS[2] = mj[0];
+            } else {
+                // common vertex at chart vertex 1
+                PT[0] = mi[0]; PT[1] = mi[0] == 0 ? 1 : 0; PT[2] = 3 - PT[0] - PT[1];
+                PS[0] = mj[0]; PS[1] = mj[0] == 0 ? 1 : 0; PS[2] = 3 - PS[0] - PS[1];
+            }
         }
+        // permuted charts relative to the centroids: a chart point (u1, u2)
+        // is u1 V[P0] + u2 V[P1] + (1 - u1 - u2) V[P2], the reference chart
+        // map of Integrand_gpu.jl:51-55 composed with the cell's own chart
         CellG X, Y;
         double dc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            X.o[d] = tvx[6 + d] - tcn[d];
-            X.t1[d] = tvx[d] - tvx[6 + d];
-            X.t2[d] = tvx[3 + d] - tvx[6 + d];
-            Y.o[d] = svx[6 + d] - scn[d];
-            Y.t1[d] = svx[d] - svx[6 + d];
-            Y.t2[d] = svx[3 + d] - svx[6 + d];
-            dc[d] = tcn[d] - scn[d];
+            const double cx = a.X.cen[3 * K + d], cy = a.Y.cen[3 * L + d];
+            const double x0 = a.X.vtx[9 * K + 3 * PT[0] + d], x1 = a.X.vtx[9 * K + 3 * PT[1] + d],
+                         x2 = a.X.vtx[9 * K + 3 * PT[2] + d];
+            const double y0 = a.Y.vtx[9 * L + 3 * PS[0] + d], y1 = a.Y.vtx[9 * L + 3 * PS[1] + d],
+                         y2 = a.Y.vtx[9 * L + 3 * PS[2] + d];
+            X.o[d] = x2 - cx;
+            X.t1[d] = x0 - x2;
+            X.t2[d] = x1 - x2;
+            Y.o[d] = y2 - cy;
+            Y.t1[d] = y0 - y2;
+            Y.t2[d] = y1 - y2;
+            dc[d] = cx - cy;
         }
         SpecAcc acc;
 #pragma unroll
@@ -738,172 +830,149 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_special(SpecialArgs a) {
 
         if (cls == CLS_NEAR) {
             const int np = a.near_np, tot = np * np;
+#pragma unroll 1
             for (int idx = lane; idx < tot; idx += 64) {
                 const int I = idx / np, J = idx - np * (idx / np);
-                spec_point<OP>(a.op, acc, X, Y, dc, a.nb1[I], a.nb2[I], a.nb1[J], a.nb2[J], a.nw[I] * a.nw[J]);
+                spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, s_nb1[I], s_nb2[I], s_nb1[J], s_nb2[J],
+                               s_nw[I] * s_nw[J]);
             }
         } else {
-            // vertex correspondence (pulledback_integrand_gpu.jl:22-155)
-            int mi[3] = {0, 0, 0}, mj[3] = {0, 0, 0}, ne = 0;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double dx = tvx[3 * i] - svx[3 * j], dy = tvx[3 * i + 1] - svx[3 * j + 1],
-                                 dz = tvx[3 * i + 2] - svx[3 * j + 2];
-                    if (dx * dx + dy * dy + dz * dz < DTOL * DTOL && ne < 3) {
-                        mi[ne] = i;
-                        mj[ne] = j;
-                        ++ne;
-                    }
-                }
-            int PT[3], PS[3];
-            if (cls == CLS_CF) {
-                PT[0] = 0; PT[1] = 1; PT[2] = 2;
-                PS[0] = PS[1] = PS[2] = 0;
-                for (int k = 0; k < ne; ++k) PS[mi[k]] = mj[k];
-            } else if (cls == CLS_CE) {
-                // common edge between reference vertices 1 and 3
-                PT[0] = mi[1]; PT[1] = 3 - mi[0] - mi[1]; PT[2] = mi[0];
-                PS[0] = mj[1]; PS[1] = 3 - mj[0] - mj[1]; PS[2] = mj[0];
-            } else {
-                // common vertex at reference vertex 1
-                PT[0] = mi[0]; PT[1] = mi[0] == 0 ? 1 : 0; PT[2] = 3 - PT[0] - PT[1];
-                PS[0] = mj[0]; PS[1] = mj[0] == 0 ? 1 : 0; PS[2] = 3 - PS[0] - PS[1];
-            }
-            RefChart c1, c2;
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                c1.v3[d] = c_vref[PT[2]][d];
-                c1.t1[d] = c_vref[PT[0]][d] - c_vref[PT[2]][d];
-                c1.t2[d] = c_vref[PT[1]][d] - c_vref[PT[2]][d];
-                c2.v3[d] = c_vref[PS[2]][d];
-                c2.t1[d] = c_vref[PS[0]][d] - c_vref[PS[2]][d];
-                c2.t2[d] = c_vref[PS[1]][d] - c_vref[PS[2]][d];
-            }
             const int g = cls - 1;
             const int n = a.ss_n[g];
             const int n2 = n * n, n4 = n2 * n2;
-            for (int idx = lane; idx < n4; idx += 64) {
-                const int i1 = idx / (n2 * n), r1 = idx - i1 * n2 * n;
-                const int i2 = r1 / n2, r2 = r1 - i2 * n2;
-                const int i3 = r2 / n, i4 = r2 - i3 * n;
-                const double e1 = a.glx[g][i1], e2 = a.glx[g][i2], e3 = a.glx[g][i3], xi = a.glx[g][i4];
-                const double w = a.glw[g][i1] * a.glw[g][i2] * a.glw[g][i3] * a.glw[g][i4];
-                const double xi3 = xi * xi * xi;
-                if (cls == CLS_CV) {
-                    // SauterSchwab.jl:265-272
-                    const double xe1 = xi * e1, xe2 = xi * e2;
-                    const double m = w * xi3 * e2;
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, xe1, 1 - xe2, xe2 * e3, m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xe2, xe2 * e3, 1 - xi, xe1, m);
-                } else if (cls == CLS_CE) {
-                    // SauterSchwab.jl:274-288
-                    const double xe1 = xi * e1, e12 = e1 * e2, e23 = e2 * e3, e123 = e12 * e3;
-                    const double m = w * xi3 * e1 * e1 * e2;
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, xe1, 1 - xi * (1 - e123), xe1 * e2 * (1 - e3), m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi * (1 - e12), xe1 * (1 - e2), 1 - xi, xe1 * e23, m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi * (1 - e123), xe1 * e2 * (1 - e3), 1 - xi, xe1, m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi * (1 - e123), xe1 * (1 - e23), 1 - xi, xe1 * e2, m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, xe1 * e3, 1 - xi * (1 - e12), xe1 * (1 - e2),
-                                  w * xi3 * e1 * e1);
-                } else {
-                    // SauterSchwab.jl:290-298
-                    const double m = w * xi3 * e1 * e1 * e2;
-                    const double p = xi * e1;
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, xi - p + p * e2, 1 - (xi - p * e2 * e3), xi - p, m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - (xi - p * e2 * e3), xi - p, 1 - xi, xi - p + p * e2, m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, p * (1 - e2 + e2 * e3), 1 - (xi - p * e2), p * (1 - e2), m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - (xi - p * e2), p * (1 - e2), 1 - xi, p * (1 - e2 + e2 * e3), m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - (xi - p * e2 * e3), p * (1 - e2 * e3), 1 - xi, p * (1 - e2), m);
-                    ss_region<OP>(a.op, acc, X, Y, dc, c1, c2, 1 - xi, p * (1 - e2), 1 - (xi - p * e2 * e3), p * (1 - e2 * e3), m);
+            const int nreg = cls == CLS_CV ? 2 : (cls == CLS_CE ? 5 : 6);
+            // region loop outermost: the region map is uniform over the wave
+            for (int r = 0; r < nreg; ++r) {
+#pragma unroll 1
+                for (int node = lane; node < n4; node += 64) {
+                    const int i1 = node / (n2 * n), r1 = node - i1 * n2 * n;
+                    const int i2 = r1 / n2, r2 = r1 - i2 * n2;
+                    const int i3 = r2 / n, i4 = r2 - i3 * n;
+                    const double wgt = s_glw[g][i1] * s_glw[g][i2] * s_glw[g][i3] * s_glw[g][i4];
+                    double u1, u2, v1, v2, m;
+                    ss_map(cls, r, s_glx[g][i1], s_glx[g][i2], s_glx[g][i3], s_glx[g][i4], wgt, u1, u2, v1, v2, m);
+                    spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m);
                 }
             }
         }
-        // butterfly over the 64 lanes
+        // butterfly over the 64 lanes, one accumulator at a time (a
+        // scheduling barrier keeps the 18 exchanges from being interleaved,
+        // which would hold all their temporaries at once)
 #pragma unroll
         for (int i = 0; i < 18; ++i) {
             double v = acc.v[i];
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
             acc.v[i] = v;
+            __builtin_amdgcn_sched_barrier(0);
         }
         if (lane == 0) {
             double2 loc[9];
             if (OP == OPK_MW) {
-                MomMW m;
-                m.m0r = acc.v[0]; m.m0i = acc.v[1];
+                MomMW mm;
+                mm.m0r = acc.v[0]; mm.m0i = acc.v[1];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    m.mxr[d] = acc.v[2 + d]; m.mxi[d] = acc.v[5 + d];
-                    m.myr[d] = acc.v[8 + d]; m.myi[d] = acc.v[11 + d];
+                    mm.mxr[d] = acc.v[2 + d]; mm.mxi[d] = acc.v[5 + d];
+                    mm.myr[d] = acc.v[8 + d]; mm.myi[d] = acc.v[11 + d];
                 }
-                m.mxyr = acc.v[14]; m.mxyi = acc.v[15];
+                mm.mxyr = acc.v[14]; mm.mxyi = acc.v[15];
                 double xa[9], ya[9];
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        xa[3 * i + d] = tvx[3 * i + d] - tcn[d];
-                        ya[3 * i + d] = svx[3 * i + d] - scn[d];
+                        xa[3 * i + d] = a.X.vtx[9 * K + 3 * i + d] - a.X.cen[3 * K + d];
+                        ya[3 * i + d] = a.Y.vtx[9 * L + 3 * i + d] - a.Y.cen[3 * L + d];
                     }
-                mw_epilogue(a.op, m, xa, ya, loc);
+                mw_epilogue(a.op, mm, xa, ya, loc);
             } else {
+                // accumulators are in chart order: (c, c') belongs to the
+                // local shapes (PT[c], PS[c'])
                 double pr[9], pi[9];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) {
-                    pr[v] = acc.v[2 * v];
-                    pi[v] = acc.v[2 * v + 1];
-                }
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int c2 = 0; c2 < 3; ++c2) {
+                        pr[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2)];
+                        pi[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2) + 1];
+                    }
                 hh_epilogue(a.op, pr, pi, 4.0 * a.X.area[K] * a.Y.area[L], loc);
             }
 #pragma unroll
             for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
         }
     }
+    }
 }
 
-// One thread per far pair that the mixed path keeps in FP64 (class 5).
+// One thread per far pair that the mixed path keeps in FP64 (class 5), owner
+// entries only.
 template <int OP>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(SpecialArgs a) {
-    const int e = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
-    if (e >= min(*a.n_special, a.capacity)) return;
-    if (a.spec_cls[e] != CLS_FAR64) return;
-    const int K = a.spec_K[e], L = a.spec_L[e];
-    double xp[3 * BMI_MAX_RULE], yp[3 * BMI_MAX_RULE], xa[9], ya[9], xc[3], yc[3];
+    const int ns = min(*a.n_special, a.capacity);
+    for (int e = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x; e < ns; e += gridDim.x * BMI_BLOCK_THREADS) {
+        if (a.spec_cls[e] != CLS_FAR64) continue;  // owner entries of class 5 only
+        const int K = a.spec_K[e], L = a.spec_L[e];
+        double xp[3 * BMI_MAX_RULE], yp[3 * BMI_MAX_RULE], xa[9], ya[9], xc[3], yc[3];
 #pragma unroll
-    for (int v = 0; v < 3 * BMI_MAX_RULE; ++v) {
-        xp[v] = a.X.pts[3 * BMI_MAX_RULE * (long long)K + v];
-        yp[v] = a.Y.pts[3 * BMI_MAX_RULE * (long long)L + v];
+        for (int v = 0; v < 3 * BMI_MAX_RULE; ++v) {
+            xp[v] = a.X.pts[3 * BMI_MAX_RULE * (long long)K + v];
+            yp[v] = a.Y.pts[3 * BMI_MAX_RULE * (long long)L + v];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xc[d] = a.X.cen[3 * K + d];
+            yc[d] = a.Y.cen[3 * L + d];
+        }
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            xa[v] = a.X.vtx[9 * K + v] - xc[v % 3];
+            ya[v] = a.Y.vtx[9 * L + v] - yc[v % 3];
+        }
+        double2 loc[9];
+        pair_fp64<OP, 0, false>(a.op, a.rule, xp, xa, xc, a.X.area[K], yp, ya, yc, a.Y.area[L], loc, nullptr);
+#pragma unroll
+        for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
     }
+}
+
+// One thread per non-owner entry: the same (test cell, trial cell) pair sits
+// in the block (home sub-batch of the test cell, patch of the trial cell);
+// copy its values.  A missing owner (which the classifier rules out: both
+// blocks classify the same pair with the same arithmetic) leaves NaN.
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(SpecialArgs a) {
+    const int ns = min(*a.n_special, a.capacity);
+    for (int e = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x; e < ns; e += gridDim.x * BMI_BLOCK_THREADS) {
+        if (!(a.spec_cls[e] & SPEC_NOT_OWNER)) continue;
+        const int K = a.spec_K[e], L = a.spec_L[e];
+        const long long b = (long long)a.X.home[K] * a.npb + a.Y.home[L];
+        const int o0 = a.blk_off[b], o1 = min(a.blk_off[b + 1], a.capacity);
+        int o = -1;
+        for (int u = o0; u < o1; ++u)
+            if (a.spec_K[u] == K && a.spec_L[u] == L) {
+                o = u;
+                break;
+            }
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        xc[d] = a.X.cen[3 * K + d];
-        yc[d] = a.Y.cen[3 * L + d];
+        for (int v = 0; v < 9; ++v)
+            a.spec_val[9 * (long long)e + v] =
+                o >= 0 ? a.spec_val[9 * (long long)o + v] : make_double2(__longlong_as_double(0x7ff8000000000000LL),
+                                                                          __longlong_as_double(0x7ff8000000000000LL));
     }
-#pragma unroll
-    for (int v = 0; v < 9; ++v) {
-        xa[v] = a.X.vtx[9 * K + v] - xc[v % 3];
-        ya[v] = a.Y.vtx[9 * L + v] - yc[v % 3];
-    }
-    double2 loc[9];
-    pair_fp64<OP, 0, false>(a.op, a.rule, xp, xa, xc, a.X.area[K], yp, ya, yc, a.Y.area[L], loc, nullptr);
-#pragma unroll
-    for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
 }
 
 hipError_t launch_special(const SpecialArgs &a, int capacity, int mixed, int op_kind, hipStream_t s) {
     if (capacity <= 0) return hipSuccess;
-    int waves_wg = BMI_BLOCK_THREADS / 64;
-    int grid = (capacity + waves_wg - 1) / waves_wg;
-    if (grid > 16384) grid = 16384;
+    const int grid = std::min((capacity + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, 4096);
     if (op_kind == OPK_MW) hipLaunchKernelGGL(k_special<OPK_MW>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
     else hipLaunchKernelGGL(k_special<OPK_HH3D>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+    const int g2 = std::min((capacity + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, 2048);
     if (mixed) {
-        const int g2 = (capacity + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
         if (op_kind == OPK_MW) hipLaunchKernelGGL(k_far64<OPK_MW>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
         else hipLaunchKernelGGL(k_far64<OPK_HH3D>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
     }
+    hipLaunchKernelGGL(k_link, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
     return hipGetLastError();
 }
 
