@@ -28,8 +28,11 @@ EXPORTS = (
     "bmi_plan_create", "bmi_plan_free",
     "bmi_matrix_create", "bmi_matrix_free", "bmi_matrix_device_ptr",
     "bmi_assemble", "bmi_plan_stats", "bmi_matrix_to_host",
-    "bmi_matrix_apply", "bmi_matrix_compare",
+    "bmi_matrix_apply", "bmi_matrix_compare", "bmi_device_memory",
+    "bmi_server_create", "bmi_server_destroy", "bmi_server_submit", "bmi_server_wait",
+    "bmi_server_release", "bmi_server_memory",
 )
+REQ_ASSEMBLE, REQ_ASSEMBLE_APPLY, REQ_APPLY = 0, 1, 2
 
 
 class BMIError(RuntimeError):
@@ -83,6 +86,47 @@ class Stats(ctypes.Structure):
         }
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int32), ("memory_budget", ctypes.c_int64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("xyz", ctypes.c_void_p),
+        ("nv", ctypes.c_int64),
+        ("tri", ctypes.c_void_p),
+        ("nt", ctypes.c_int64),
+        ("space_kind", ctypes.c_int32),
+        ("gidx", ctypes.c_void_p),
+        ("coeff", ctypes.c_void_p),
+        ("nfun", ctypes.c_int64),
+        ("op", Operator),
+        ("opts", Options),
+        ("dtype", ctypes.c_int32),
+        ("matrix", ctypes.c_uint64),
+        ("nrhs", ctypes.c_int32),
+        ("x", ctypes.c_void_p),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("message", ctypes.c_char * 160),
+        ("queue_ms", ctypes.c_double),
+        ("prep_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("entries", ctypes.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"status": self.status, "message": self.message.decode(errors="replace"),
+                "queue_ms": self.queue_ms, "prep_ms": self.prep_ms, "device_ms": self.device_ms,
+                "total_ms": self.total_ms, "entries": int(self.entries)}
+
+
 _lib = None
 vp = ctypes.c_void_p
 pp = ctypes.POINTER(ctypes.c_void_p)
@@ -117,6 +161,13 @@ def lib():
     L.bmi_matrix_to_host.argtypes = [vp, vp, i32]
     L.bmi_matrix_apply.argtypes = [vp, vp, vp, i32]
     L.bmi_matrix_compare.argtypes = [vp, vp, vp]
+    L.bmi_device_memory.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.bmi_server_create.argtypes = [i32, ctypes.POINTER(ServerOptions), pp]
+    L.bmi_server_destroy.argtypes = [vp]
+    L.bmi_server_submit.argtypes = [vp, ctypes.POINTER(Request), vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.bmi_server_wait.argtypes = [vp, ctypes.c_uint64, i32, ctypes.POINTER(Result)]
+    L.bmi_server_release.argtypes = [vp, ctypes.c_uint64]
+    L.bmi_server_memory.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("bmi_version", "bmi_last_error", "bmi_options_default"):

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -29,6 +30,102 @@ bmi_status fail(bmi_status s, const std::string &msg) {
         if (e_ != hipSuccess) return fail(BMI_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+}  // namespace
+
+bmi_status set_error(bmi_status s, const std::string &msg) { return fail(s, msg); }
+
+// Device memory comes from the device's stream-ordered pool, allocated and
+// freed on a library stream of its own per device (it carries nothing else):
+// a free does not wait for the whole device to go idle, as hipFree does, and
+// an allocation waits only for that stream, which matters when several
+// contexts work on one device at once.  Every entry point finishes its
+// device work before it returns, so a buffer is idle when it is freed.
+namespace {
+std::mutex g_alloc_mu;
+hipStream_t g_alloc_stream[64] = {};
+
+hipStream_t alloc_stream(int *dev) {
+    if (hipGetDevice(dev) != hipSuccess || *dev < 0 || *dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    if (!g_alloc_stream[*dev]) {
+        hipStream_t s = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        g_alloc_stream[*dev] = s;
+    }
+    return g_alloc_stream[*dev];
+}
+
+hipMemPool_t device_pool(int dev) {
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return nullptr;
+    return pool;
+}
+}  // namespace
+
+// Bytes the pool holds for reuse (reserved but not in use).
+size_t pool_cached_bytes(int dev) {
+    hipMemPool_t pool = device_pool(dev);
+    if (!pool) return 0;
+    uint64_t reserved = 0, used = 0;
+    if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) != hipSuccess ||
+        hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) != hipSuccess)
+        return 0;
+    return reserved > used ? (size_t)(reserved - used) : 0;
+}
+
+namespace {
+// stream of the entry point running on this thread (its context's), if any
+thread_local hipStream_t t_work_stream = nullptr;
+}  // namespace
+
+WorkStream::WorkStream(hipStream_t s) : prev_(t_work_stream) { t_work_stream = s; }
+WorkStream::~WorkStream() { t_work_stream = prev_; }
+
+hipError_t dev_alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    int dev = 0;
+    hipStream_t s = alloc_stream(&dev);
+    if (!s) return hipErrorInvalidDevice;
+    // allocate where the caller's work runs: the allocation stream is shared
+    // by every context of the device and may sit in a hardware queue behind
+    // another context's kernels
+    if (t_work_stream) s = t_work_stream;
+    hipError_t e = hipMallocAsync(p, bytes ? bytes : 1, s);
+    if (e == hipErrorOutOfMemory) {
+        // give the pool's cached blocks back and try once more
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(s);
+        if (hipMemPool_t pool = device_pool(dev)) (void)hipMemPoolTrimTo(pool, 0);
+        e = hipMallocAsync(p, bytes ? bytes : 1, s);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    return hipStreamSynchronize(s);
+}
+
+// Host-to-device copy on the caller's stream (a plain hipMemcpy goes through
+// the null stream and waits for other contexts' work).
+hipError_t dev_upload(void *dst, const void *src, size_t bytes) {
+    int dev = 0;
+    hipStream_t s = alloc_stream(&dev);
+    if (!s) return hipErrorInvalidDevice;
+    if (t_work_stream) s = t_work_stream;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+void dev_free(void *p) {
+    if (!p) return;
+    int dev = 0;
+    hipStream_t s = alloc_stream(&dev);
+    if (s) (void)hipFreeAsync(p, s);
+}
+
+namespace {
+
 template <class T>
 struct DevBuf {
     T *p = nullptr;
@@ -38,20 +135,20 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        dev_free(p);
         p = nullptr;
         n = 0;
     }
     hipError_t alloc(size_t count) {
         release();
         n = count;
-        return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1));
+        return dev_alloc((void **)&p, sizeof(T) * (count ? count : 1));
     }
     hipError_t upload(const std::vector<T> &h) {
         hipError_t e = alloc(h.size());
         if (e != hipSuccess) return e;
         if (h.empty()) return hipSuccess;
-        return hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+        return dev_upload(p, h.data(), sizeof(T) * h.size());
     }
 };
 
@@ -132,6 +229,9 @@ uint64_t spread3(uint64_t v) {
 struct bmi_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // grow-only device workspace of the apply kernels
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
     // pinned staging ring for device-to-host copies of matrices
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -181,6 +281,7 @@ struct bmi_matrix {
     int64_t rows = 0, cols = 0;
     int dtype = BMI_C64;
     void *d = nullptr;
+    bool owned = true;  // false: d belongs to the caller (matrix_wrap)
 };
 
 struct bmi_plan {
@@ -202,6 +303,7 @@ struct bmi_plan {
     SpecialArgs sargs{};
     ClassifyArgs cargs{};
     int64_t pairs[6] = {0, 0, 0, 0, 0, 0};
+    bool pairs_ready = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bmi_stats last{};
 };
@@ -678,6 +780,51 @@ std::vector<double> rule_points(const bmi_mesh *m, int n, const double l[BMI_MAX
 
 }  // namespace
 
+hipStream_t context_stream(const bmi_context *ctx) { return ctx->stream; }
+
+// y = Z x on run's stream, with run's workspace (grown when too small).
+bmi_status matrix_apply_on(bmi_context *run, const bmi_matrix *m, const double *x, double *y, int32_t nrhs) {
+    if (!run || !m || !x || !y) return fail(BMI_EINVAL, "NULL argument");
+    if (nrhs < 1 || nrhs > BMI_MAX_RHS) return fail(BMI_EINVAL, "nrhs must be 1..8");
+    if (m->rows > (1LL << 31) - 1 || m->cols > (1LL << 31) - 1) return fail(BMI_EUNSUPPORTED, "matrix too large");
+    if (run->device != m->ctx->device) return fail(BMI_EINVAL, "matrix and context are on different devices");
+    BMI_HIP(hipSetDevice(run->device));
+    hipStream_t s = run->stream;
+    WorkStream wsc(s);
+    const int rows = (int)m->rows, cols = (int)m->cols;
+    const int nchunk = (cols + BMI_APPLY_CHUNK - 1) / BMI_APPLY_CHUNK;
+    const size_t nx = (size_t)cols * nrhs, ny = (size_t)rows * nrhs, np = (size_t)nchunk * rows * nrhs;
+    const size_t need = sizeof(double2) * (nx + ny + np);
+    if (need > run->ws_bytes) {
+        dev_free(run->ws);
+        run->ws = nullptr;
+        run->ws_bytes = 0;
+        hipError_t e = dev_alloc(&run->ws, need);
+        if (e != hipSuccess) return fail(BMI_ENOMEM, std::string("apply workspace: ") + hipGetErrorString(e));
+        run->ws_bytes = need;
+    }
+    double2 *dx = (double2 *)run->ws, *dy = dx + nx, *part = dy + ny;
+    BMI_HIP(hipMemcpyAsync(dx, x, sizeof(double2) * nx, hipMemcpyHostToDevice, s));
+    BMI_HIP(launch_apply(m->d, m->dtype == BMI_C32, m->rows, rows, cols, dx, nrhs, part, dy, s));
+    BMI_HIP(hipMemcpyAsync(y, dy, sizeof(double2) * ny, hipMemcpyDeviceToHost, s));
+    BMI_HIP(hipStreamSynchronize(s));
+    return BMI_OK;
+}
+
+// A matrix over caller-owned device memory of at least rows*cols elements.
+bmi_status matrix_wrap(bmi_context *ctx, void *mem, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out) {
+    if (!ctx || !mem || !out) return fail(BMI_EINVAL, "NULL argument");
+    auto m = std::make_unique<bmi_matrix>();
+    m->ctx = ctx;
+    m->rows = rows;
+    m->cols = cols;
+    m->dtype = dtype;
+    m->d = mem;
+    m->owned = false;
+    *out = m.release();
+    return BMI_OK;
+}
+
 extern "C" {
 
 const char *bmi_version(void) { return "bmi 0.1 (gfx950)"; }
@@ -703,7 +850,24 @@ bmi_status bmi_context_create(int32_t device, bmi_context **out) {
     auto ctx = std::make_unique<bmi_context>();
     ctx->device = device;
     BMI_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // keep freed pool memory cached for the next allocation
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     *out = ctx.release();
+    return BMI_OK;
+}
+
+bmi_status bmi_device_memory(const bmi_context *ctx, int64_t *free_bytes, int64_t *total_bytes) {
+    if (!ctx || !free_bytes || !total_bytes) return fail(BMI_EINVAL, "NULL argument");
+    BMI_HIP(hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    BMI_HIP(hipMemGetInfo(&f, &t));
+    f += pool_cached_bytes(ctx->device);  // freed blocks the pool keeps for reuse
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
     return BMI_OK;
 }
 
@@ -714,6 +878,7 @@ bmi_status bmi_context_destroy(bmi_context *ctx) {
         if (ctx->stage[i]) (void)hipHostFree(ctx->stage[i]);
         if (ctx->stage_ev[i]) (void)hipEventDestroy(ctx->stage_ev[i]);
     }
+    dev_free(ctx->ws);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BMI_OK;
@@ -724,6 +889,7 @@ bmi_status bmi_mesh_create(bmi_context *ctx, const double *xyz, int64_t nv, cons
     if (!ctx || !xyz || !tri || !out) return fail(BMI_EINVAL, "NULL argument");
     if (nv < 3 || nt < 1 || nt > (1LL << 30)) return fail(BMI_EINVAL, "bad mesh size");
     BMI_HIP(hipSetDevice(ctx->device));
+    WorkStream ws(ctx->stream);
     auto m = std::make_unique<bmi_mesh>();
     m->ctx = ctx;
     m->nv = nv;
@@ -784,6 +950,7 @@ bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, c
         if (gidx[i] >= 0 && !std::isfinite(coeff[i])) return fail(BMI_EINVAL, "non-finite coefficient");
     }
     BMI_HIP(hipSetDevice(mesh->ctx->device));
+    WorkStream ws(mesh->ctx->stream);
     auto s = std::make_unique<bmi_space>();
     s->mesh = mesh;
     s->kind = kind;
@@ -810,18 +977,16 @@ bmi_status bmi_matrix_create(bmi_context *ctx, int64_t rows, int64_t cols, int32
     if (rows < 1 || cols < 1) return fail(BMI_EINVAL, "bad matrix size");
     if (dtype != BMI_C64 && dtype != BMI_C32) return fail(BMI_EINVAL, "bad dtype");
     BMI_HIP(hipSetDevice(ctx->device));
+    WorkStream ws(ctx->stream);
     const size_t es = dtype == BMI_C64 ? 16 : 8;
     const size_t bytes = (size_t)rows * (size_t)cols * es;
-    size_t freeb = 0, totalb = 0;
-    BMI_HIP(hipMemGetInfo(&freeb, &totalb));
-    if (bytes > freeb) return fail(BMI_ENOMEM, "matrix of " + std::to_string(bytes) + " bytes exceeds free device memory");
     auto m = std::make_unique<bmi_matrix>();
     m->ctx = ctx;
     m->rows = rows;
     m->cols = cols;
     m->dtype = dtype;
-    hipError_t e = hipMalloc(&m->d, bytes);
-    if (e != hipSuccess) return fail(BMI_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    hipError_t e = dev_alloc(&m->d, bytes);
+    if (e != hipSuccess) return fail(BMI_ENOMEM, std::string("matrix allocation: ") + hipGetErrorString(e));
     *out = m.release();
     return BMI_OK;
 }
@@ -829,7 +994,7 @@ bmi_status bmi_matrix_create(bmi_context *ctx, int64_t rows, int64_t cols, int32
 bmi_status bmi_matrix_free(bmi_matrix *m) {
     if (m) {
         (void)hipSetDevice(m->ctx->device);
-        if (m->d) (void)hipFree(m->d);
+        if (m->owned) dev_free(m->d);
         delete m;
     }
     return BMI_OK;
@@ -868,6 +1033,7 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
         if (!std::isfinite(v)) return fail(BMI_EINVAL, "non-finite operator coefficient");
 
     BMI_HIP(hipSetDevice(ctx->device));
+    WorkStream ws(ctx->stream);
     auto P = std::make_unique<bmi_plan>();
     P->ctx = ctx;
     P->X = test;
@@ -948,27 +1114,6 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     C.spec_loc = P->d_spec_loc.p;
     C.capacity = total;
     BMI_HIP(launch_classify(C, 1, s));
-    // pair statistics: a cell of several panels is classified once per panel,
-    // so count each (test cell, trial cell) pair once
-    std::vector<unsigned char> cls(cap, 0);
-    std::vector<int> sk(cap, 0), sl(cap, 0);
-    if (total > 0) {
-        BMI_HIP(hipMemcpyAsync(cls.data(), P->d_spec_cls.p, total, hipMemcpyDeviceToHost, s));
-        BMI_HIP(hipMemcpyAsync(sk.data(), P->d_spec_K.p, sizeof(int) * total, hipMemcpyDeviceToHost, s));
-        BMI_HIP(hipMemcpyAsync(sl.data(), P->d_spec_L.p, sizeof(int) * total, hipMemcpyDeviceToHost, s));
-    }
-    BMI_HIP(hipStreamSynchronize(s));
-    {
-        std::vector<std::pair<int64_t, unsigned char>> keys(total);
-        for (int i = 0; i < total; ++i) keys[i] = {(int64_t)sk[i] * trial->mesh->nt + sl[i], cls[i] & 0x7f};
-        std::sort(keys.begin(), keys.end());
-        for (int i = 0; i < total; ++i)
-            if (i == 0 || keys[i].first != keys[i - 1].first) P->pairs[keys[i].second < 6 ? keys[i].second : 0]++;
-    }
-    int64_t special = 0;
-    for (int c = 1; c < 6; ++c) special += P->pairs[c];
-    P->pairs[0] = test->mesh->nt * trial->mesh->nt - special;
-
     SpecialArgs &S = P->sargs;
     S.X = C.X;
     S.Y = C.Y;
@@ -1087,15 +1232,33 @@ bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
     P->last.special_ms = t12;
     P->last.far_ms = t23;
     P->last.total_ms = t03;
-    for (int i = 0; i < 6; ++i) P->last.pairs[i] = P->pairs[i];
     P->last.far_launches = launches;
     P->last.n_colours_test = 1;
     P->last.n_colours_trial = P->Y->ncolour;
     return BMI_OK;
 }
 
-bmi_status bmi_plan_stats(const bmi_plan *P, bmi_stats *st) {
-    if (!P || !st) return fail(BMI_EINVAL, "NULL argument");
+bmi_status bmi_plan_stats(const bmi_plan *Pc, bmi_stats *st) {
+    if (!Pc || !st) return fail(BMI_EINVAL, "NULL argument");
+    bmi_plan *P = const_cast<bmi_plan *>(Pc);
+    if (!P->pairs_ready) {
+        // pair counts, on first request: each (test cell, trial cell) pair
+        // has one owner entry (the other entries of a cell repeated in
+        // several panels carry SPEC_NOT_OWNER); the rest are far pairs
+        BMI_HIP(hipSetDevice(P->ctx->device));
+        std::vector<unsigned char> cls(std::max(P->capacity, 1), 0);
+        if (P->capacity > 0) {
+            BMI_HIP(hipMemcpyAsync(cls.data(), P->d_spec_cls.p, P->capacity, hipMemcpyDeviceToHost, P->ctx->stream));
+            BMI_HIP(hipStreamSynchronize(P->ctx->stream));
+        }
+        for (int c = 0; c < 6; ++c) P->pairs[c] = 0;
+        for (int i = 0; i < P->capacity; ++i)
+            if (!(cls[i] & SPEC_NOT_OWNER) && cls[i] >= 1 && cls[i] < 6) P->pairs[cls[i]]++;
+        int64_t special = 0;
+        for (int c = 1; c < 6; ++c) special += P->pairs[c];
+        P->pairs[0] = P->X->mesh->nt * P->Y->mesh->nt - special;
+        P->pairs_ready = true;
+    }
     *st = P->last;
     for (int i = 0; i < 6; ++i) st->pairs[i] = P->pairs[i];
     st->n_colours_test = 1;
@@ -1139,6 +1302,7 @@ bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype) {
     if (dtype != BMI_C64 && dtype != BMI_C32) return fail(BMI_EINVAL, "bad dtype");
     bmi_context *ctx = m->ctx;
     BMI_HIP(hipSetDevice(ctx->device));
+    WorkStream ws(ctx->stream);
     hipStream_t s = ctx->stream;
     const size_t kStage = 64u << 20;  // bytes per pinned staging buffer
     for (int i = 0; i < 2; ++i) {
@@ -1169,23 +1333,8 @@ bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype) {
 }
 
 bmi_status bmi_matrix_apply(const bmi_matrix *m, const double *x, double *y, int32_t nrhs) {
-    if (!m || !x || !y) return fail(BMI_EINVAL, "NULL argument");
-    if (nrhs < 1 || nrhs > BMI_MAX_RHS) return fail(BMI_EINVAL, "nrhs must be 1..8");
-    if (m->rows > (1LL << 31) - 1 || m->cols > (1LL << 31) - 1) return fail(BMI_EUNSUPPORTED, "matrix too large");
-    BMI_HIP(hipSetDevice(m->ctx->device));
-    hipStream_t s = m->ctx->stream;
-    const int rows = (int)m->rows, cols = (int)m->cols;
-    const int nchunk = (cols + BMI_APPLY_CHUNK - 1) / BMI_APPLY_CHUNK;
-    DevBuf<double2> dx, dy, part;
-    hipError_t e;
-    if ((e = dx.alloc((size_t)cols * nrhs)) != hipSuccess || (e = dy.alloc((size_t)rows * nrhs)) != hipSuccess ||
-        (e = part.alloc((size_t)nchunk * rows * nrhs)) != hipSuccess)
-        return fail(BMI_ENOMEM, std::string("apply buffers: ") + hipGetErrorString(e));
-    BMI_HIP(hipMemcpyAsync(dx.p, x, sizeof(double2) * cols * nrhs, hipMemcpyHostToDevice, s));
-    BMI_HIP(launch_apply(m->d, m->dtype == BMI_C32, m->rows, rows, cols, dx.p, nrhs, part.p, dy.p, s));
-    BMI_HIP(hipMemcpyAsync(y, dy.p, sizeof(double2) * rows * nrhs, hipMemcpyDeviceToHost, s));
-    BMI_HIP(hipStreamSynchronize(s));
-    return BMI_OK;
+    if (!m) return fail(BMI_EINVAL, "NULL argument");
+    return matrix_apply_on(m->ctx, m, x, y, nrhs);
 }
 
 bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double out[3]) {
@@ -1193,6 +1342,7 @@ bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double
     if (a->rows != ref->rows || a->cols != ref->cols) return fail(BMI_EINVAL, "matrix shapes differ");
     if (a->ctx != ref->ctx) return fail(BMI_EINVAL, "matrices belong to different contexts");
     BMI_HIP(hipSetDevice(a->ctx->device));
+    WorkStream ws(a->ctx->stream);
     hipStream_t s = a->ctx->stream;
     const int groups = 4096;
     DevBuf<double> part;

@@ -156,3 +156,26 @@ hipError_t launch_apply(const void *Z, int c32, long long ld, int rows, int cols
                         double2 *partial, double2 *y, hipStream_t s);
 hipError_t launch_compare(const void *a, int a32, const void *b, int b32, long long n, int groups, double *out,
                           hipStream_t s);
+
+// host helpers shared by the C ABI files (bmi_host.cpp)
+#include <string>
+struct bmi_matrix;
+struct bmi_context;
+hipStream_t context_stream(const bmi_context *ctx);
+// Allocations and uploads of an entry point go to its context's stream
+// while one of these is alive on the thread.
+class WorkStream {
+public:
+    explicit WorkStream(hipStream_t s);
+    ~WorkStream();
+    WorkStream(const WorkStream &) = delete;
+    WorkStream &operator=(const WorkStream &) = delete;
+private:
+    hipStream_t prev_;
+};
+hipError_t dev_alloc(void **p, size_t bytes);
+void dev_free(void *p);
+hipError_t dev_upload(void *dst, const void *src, size_t bytes);
+int32_t set_error(int32_t status, const std::string &msg);
+int32_t matrix_apply_on(bmi_context *run, const bmi_matrix *m, const double *x, double *y, int32_t nrhs);
+int32_t matrix_wrap(bmi_context *ctx, void *mem, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out);

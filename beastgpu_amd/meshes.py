@@ -119,10 +119,13 @@ def real_sph_harm(l: int, m: int, polar: np.ndarray, azimuth: np.ndarray) -> np.
     return np.sqrt(2.0) * (np.real(y) if m > 0 else np.imag(y))
 
 
-def perturb_radial(mesh: Mesh, seed: int, amplitude: float, n_terms: int = 8, lmax: int = 6) -> Mesh:
+def perturb_radial(mesh: Mesh, seed: int, amplitude: float, n_terms: int = 8, lmax: int = 6,
+                   cache: dict | None = None) -> Mesh:
     """r -> r * (1 + delta(theta, phi)), delta = sum of ``n_terms`` real spherical
     harmonics with random degree 1..lmax, random order and N(0,1) weights,
-    rescaled so that max |delta| over the vertices equals ``amplitude``."""
+    rescaled so that max |delta| over the vertices equals ``amplitude``.
+    ``cache`` (a dict kept by the caller for one base mesh) holds the
+    harmonics already evaluated on its vertices."""
     v = mesh.vertices
     r = np.linalg.norm(v, axis=1)
     if amplitude == 0.0:
@@ -135,7 +138,13 @@ def perturb_radial(mesh: Mesh, seed: int, amplitude: float, n_terms: int = 8, lm
         l = int(rng.integers(1, lmax + 1))
         m = int(rng.integers(-l, l + 1))
         a = rng.standard_normal()
-        delta += a * real_sph_harm(l, m, polar, azimuth)
+        if cache is None:
+            y = real_sph_harm(l, m, polar, azimuth)
+        else:
+            y = cache.get((l, m))
+            if y is None:
+                y = cache[(l, m)] = real_sph_harm(l, m, polar, azimuth)
+        delta += a * y
     delta *= amplitude / np.abs(delta).max()
     return Mesh(v * (1.0 + delta)[:, None], mesh.faces.copy())
 

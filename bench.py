@@ -62,6 +62,15 @@ def workload(name: str):
         m = meshes.icosphere(3)
         return dict(op=Maxwell3D.singlelayer(wavenumber=1.0), X=spaces.raviartthomas(m), kind="efie",
                     desc="EFIE on RWG, icosphere L3, k=1.0, 1,280 triangles, 1,920 RWG")
+    if name == "efie-L3-k4":
+        m = meshes.icosphere(3)
+        return dict(op=Maxwell3D.singlelayer(wavenumber=4.0), X=spaces.raviartthomas(m), kind="efie",
+                    desc="EFIE on RWG, icosphere L3, k=4.0, 1,280 triangles, 1,920 RWG")
+    if name == "ellipsoid-60k":
+        m = meshes.ellipsoid()
+        return dict(op=Maxwell3D.singlelayer(wavenumber=4.0), X=spaces.raviartthomas(m), kind="efie",
+                    desc="EFIE on RWG, triaxial ellipsoid 1.0/0.7/0.5 (cube-sphere grid 95x60x60, seed 3 jitter), "
+                         "k=4.0, 60,000 triangles, 90,000 RWG")
     if name.startswith("hh3d-cube"):
         d = int(name[len("hh3d-cube"):])
         m = meshes.meshcuboid(1.0, 1.0, 1.0, 0.5 / d)
@@ -136,6 +145,107 @@ def calibrate_copy(mib: int, reps: int = 3):
     print(json.dumps({"calibrate_copy_bytes": n, "reps": reps}), flush=True)
 
 
+def serving_run(n_req: int, rate: float = 20.0, seed: int = 5, slots: int = 4, precision: str = "fp64",
+                output: str = "c64") -> dict:
+    """Poisson stream of assemble-and-apply requests through the request loop:
+    EFIE on RWG of perturbed icospheres, level 2..5 uniform (log-uniform
+    triangle count 320..20,480), amplitude U[0, 5%], k ~ U[0.5, 6], 4 random
+    right-hand sides with entries uniform in the unit disk.  Latency is
+    arrival -> result, measured against the scheduled arrival time."""
+    import threading
+
+    from beastgpu_amd.serving import Server
+    from beastgpu_amd.spaces import Space
+
+    rng = np.random.default_rng(seed)
+    levels = rng.integers(2, 6, size=n_req)
+    ks = rng.uniform(0.5, 6.0, size=n_req)
+    amps = rng.uniform(0.0, 0.05, size=n_req)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    base = {L: meshes.icosphere(L) for L in range(2, 6)}
+    bsp = {L: spaces.raviartthomas(base[L]) for L in base}
+    cache = {L: {} for L in base}
+    reqs = []
+    for i in range(n_req):
+        L = int(levels[i])
+        m = meshes.perturb_radial(base[L], seed=10000 + i, amplitude=float(amps[i]), cache=cache[L])
+        X = Space(m, bsp[L].kind, bsp[L].idx, bsp[L].coeff, bsp[L].nfun)
+        reqs.append((Maxwell3D.singlelayer(wavenumber=float(ks[i])), X))
+    cfg = GPUConfiguration(precision=precision, output=output)
+
+    def rhs(i, n):
+        r = np.random.default_rng([55, i])
+        return np.sqrt(r.uniform(size=(n, 4))) * np.exp(2j * np.pi * r.uniform(size=(n, 4)))
+
+    srv = Server(slots=slots)
+    solo = {}
+    for rep in range(2):  # warm-up, then one request per level alone (untimed stream)
+        for L in base:
+            res, _ = srv.wait(srv.submit_assemble_apply(Maxwell3D.singlelayer(wavenumber=1.0), bsp[L],
+                                                        rhs(0, bsp[L].nfun), cfg))
+            if rep == 1:
+                solo[f"L{L}"] = {k: res[k] for k in ("total_ms", "prep_ms", "device_ms")}
+    ids = [None] * n_req
+    lag = np.zeros(n_req)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, (op, X) in enumerate(reqs):
+            x = rhs(i, X.nfun)
+            dt = t0 + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            lag[i] = time.perf_counter() - (t0 + arrivals[i])
+            ids[i] = srv.submit_assemble_apply(op, X, x, cfg)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = np.zeros(n_req)
+    parts = np.zeros((n_req, 3))
+    entries = 0
+    for i in range(n_req):
+        while ids[i] is None:
+            time.sleep(0.001)
+        res, y = srv.wait(ids[i])
+        lat[i] = lag[i] * 1e3 + res["total_ms"]
+        parts[i] = (res["queue_ms"], res["prep_ms"], res["device_ms"])
+        entries += res["entries"]
+        if (i + 1) % 200 == 0:
+            print(f"[bench] serving {i + 1}/{n_req} requests, p50 so far {np.percentile(lat[:i + 1], 50):.1f} ms",
+                  file=sys.stderr, flush=True)
+    t_end = time.perf_counter()
+    th.join()
+    _, peak = srv.memory()
+    srv.close()
+    makespan = t_end - t0
+    per_level = {}
+    for L in base:
+        sel = levels == L
+        if sel.any():
+            per_level[f"L{L}"] = {"n": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
+                                  "p99_ms": float(np.percentile(lat[sel], 99))}
+    return {
+        "requests": n_req,
+        "arrival_rate_per_s": rate,
+        "slots": slots,
+        "precision": precision,
+        "output": output,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()),
+        "max_ms": float(lat.max()),
+        "entries_per_s": entries / makespan,
+        "makespan_s": makespan,
+        "mean_queue_ms": float(parts[:, 0].mean()),
+        "mean_prep_ms": float(parts[:, 1].mean()),
+        "mean_device_ms": float(parts[:, 2].mean()),
+        "max_submit_lag_ms": float(lag.max() * 1e3),
+        "peak_resident_gb": peak / 1e9,
+        "per_level": per_level,
+        "alone_ms": solo,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +255,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--serving-requests", type=int, default=400,
+                    help="requests of the serving stream in the default run (0: skip); --workload serving runs 2,000")
+    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-total", type=int, default=2000, help="requests of --workload serving")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -156,6 +270,15 @@ def main():
               "(assembly shards by test rows with no exchange)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serving":
+        sv = serving_run(args.serving_total, slots=args.serving_slots)
+        print(json.dumps({"metric": "serving p50/p99 latency (ms), assemble-and-apply requests",
+                          "value": sv["p50_ms"], "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+                          "data": "synthetic", "dtype": "f64",
+                          "config": {"workload": f"Poisson arrivals 20 req/s (seed 5), {args.serving_total} EFIE assemble-and-apply "
+                                                 "requests on perturbed icospheres L2-L5, k in [0.5, 6], 4 RHS"},
+                          "serving": sv}), flush=True)
+        return
     w = workload(args.workload)
     X, op, kind = w["X"], w["op"], w["kind"]
     n = X.nfun
@@ -235,6 +358,10 @@ def main():
         result["mixed"] = mixed
 
     asm.close()
+    if args.serving_requests > 0 and args.workload == "efie-L5":
+        sv = serving_run(args.serving_requests, slots=args.serving_slots)
+        print(f"[bench] serving {sv}", file=sys.stderr, flush=True)
+        result["serving"] = sv
     if not args.no_cpu:
         cb = cpu_baseline(w, args.cpu_seconds)
         print(f"[bench] cpu baseline {cb}", file=sys.stderr, flush=True)

@@ -168,6 +168,77 @@ bmi_status bmi_matrix_apply(const bmi_matrix *m, const double *x, double *y, int
  * Replaces test(M, M_ref) of test/difference.jl:1-6, which compared one column. */
 bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double out[3]);
 
+/* Free and total device memory of the context's device (bytes).
+ * No counterpart in the reference; callers that size work by what fits in
+ * HBM (admission control, the full-size tests) use it. */
+bmi_status bmi_device_memory(const bmi_context *ctx, int64_t *free_bytes, int64_t *total_bytes);
+
+/* ---------------------------------------------------------------- serving
+ * A request loop over one device: assembly, assemble-and-apply and apply
+ * requests arrive at any time from any thread; up to `slots` of them run at
+ * once, each on its own stream, so the device work of different requests
+ * overlaps; a request is admitted when a slot is free and the device memory
+ * it needs (its N x N matrix plus plan buffers) fits beside the matrices
+ * already resident.  Assembled matrices of assembly requests stay resident
+ * in HBM until released; apply requests use them.  Results do not depend on
+ * what else runs: every kernel sums in a fixed order.
+ * No counterpart in the reference (its only scheduling precedent is the
+ * producer / Channel tiling of GPU_scheduler.jl:160-206). */
+typedef struct bmi_server bmi_server;
+
+#define BMI_REQ_ASSEMBLE 0       /* assemble and keep the matrix resident */
+#define BMI_REQ_ASSEMBLE_APPLY 1 /* assemble, y = Z x, free the matrix */
+#define BMI_REQ_APPLY 2          /* y = Z x on a resident matrix */
+
+typedef struct {
+    int32_t slots;          /* requests on the device at once, 1..16 */
+    int64_t memory_budget;  /* bytes requests may hold together; 0 = free memory at creation less 4 GiB */
+} bmi_server_options;
+
+typedef struct {
+    int32_t kind;             /* BMI_REQ_* */
+    /* geometry and space (assembly kinds); host arrays are copied at submit */
+    const double *xyz;        /* 3 x nv */
+    int64_t nv;
+    const int32_t *tri;       /* 3 x nt, 0-based */
+    int64_t nt;
+    int32_t space_kind;       /* BMI_SPACE_*; test space = trial space */
+    const int64_t *gidx;      /* 3 x nt */
+    const double *coeff;      /* 3 x nt */
+    int64_t nfun;
+    bmi_operator op;
+    bmi_options opts;
+    int32_t dtype;            /* BMI_C64 | BMI_C32 */
+    uint64_t matrix;          /* BMI_REQ_APPLY: id of the assembly request that holds the matrix */
+    int32_t nrhs;             /* apply kinds: 1..8 */
+    const double *x;          /* apply kinds: nfun x nrhs complex, copied at submit */
+} bmi_request;
+
+typedef struct {
+    int32_t status;       /* BMI_OK or the failure's status */
+    char message[160];    /* failure message */
+    double queue_ms;      /* submit -> admitted */
+    double prep_ms;       /* host-side preparation: mesh, space, plan */
+    double device_ms;     /* assembly and apply */
+    double total_ms;      /* submit -> done */
+    int64_t entries;      /* matrix entries assembled */
+} bmi_result;
+
+bmi_status bmi_server_create(int32_t device, const bmi_server_options *opts, bmi_server **out);
+/* Waits for every submitted request, frees resident matrices. */
+bmi_status bmi_server_destroy(bmi_server *server);
+/* Queue a request; y (nfun x nrhs complex, apply kinds) is written when the
+ * request completes and must stay valid until then.  Returns at once. */
+bmi_status bmi_server_submit(bmi_server *server, const bmi_request *req, double *y, uint64_t *id);
+/* Wait up to timeout_ms (< 0: no limit) for request id; BMI_EBUSY if it has
+ * not completed yet.  The result is returned once; then the id is forgotten
+ * (an assembly request's id still names its resident matrix). */
+bmi_status bmi_server_wait(bmi_server *server, uint64_t id, int32_t timeout_ms, bmi_result *res);
+/* Free the resident matrix of assembly request id. */
+bmi_status bmi_server_release(bmi_server *server, uint64_t id);
+/* Device memory held by requests now and at most so far (bytes). */
+bmi_status bmi_server_memory(const bmi_server *server, int64_t *resident, int64_t *peak);
+
 #ifdef __cplusplus
 }
 #endif

@@ -151,3 +151,46 @@ def test_lossy_operator_matches_oracle(gpu_ctx, kind, precision, tol):
     err = relfro(Z, Zref)
     print("rel err", err)
     assert err <= tol
+
+
+def test_config3_ellipsoid_full_size(gpu_ctx):
+    """The 60,000-triangle ellipsoid (90,000 RWG, k = 4): FP64 matrix (129.6 GB)
+    and mixed ComplexF32 matrix (64.8 GB) resident together.  Checks: class
+    counts against the Euler closed forms; (Z x) on 24 sampled rows against
+    the oracle's rows times x at 1e-12; the mixed matrix against the FP64 one
+    (relative Frobenius and Z x) at 1e-6."""
+    import ctypes
+    from beastgpu_amd import _bmi
+
+    free, total = ctypes.c_int64(), ctypes.c_int64()
+    _bmi.check(_bmi.lib().bmi_device_memory(gpu_ctx.h, ctypes.byref(free), ctypes.byref(total)))
+    if free.value < 210e9:
+        pytest.skip("needs about 200 GB of free device memory")
+    m = meshes.ellipsoid()
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.singlelayer(wavenumber=4.0)
+    print("case config3 ellipsoid-60k", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(), ctx=gpu_ctx)
+    M = a.assemble()
+    st = a.stats()
+    deg = m.vertex_degrees().astype(np.int64)
+    assert st["pairs"]["common_face"] == m.nt and st["pairs"]["common_edge"] == 3 * m.nt
+    assert st["pairs"]["common_vertex"] == int((deg ** 2).sum() - 9 * m.nt)
+    rng = np.random.default_rng(33)
+    x = np.sqrt(rng.uniform(size=X.nfun)) * np.exp(2j * np.pi * rng.uniform(size=X.nfun))
+    y64 = M.apply(x)
+    rows = np.sort(rng.choice(X.nfun, size=24, replace=False))
+    Zr, _ = O.assemble(op, X, X, rows=rows)
+    yr = Zr @ x
+    err_rows = np.linalg.norm(y64[rows] - yr) / np.linalg.norm(yr)
+    print("config3 sampled-row Zx err", err_rows, st, flush=True)
+    assert err_rows <= FP64_TOL
+    b = Assembler(op, X, X, GPUConfiguration(precision="mixed", output="c32"), ctx=gpu_ctx)
+    cmp = b.assemble().compare(M)
+    ym = b.matrix.apply(x)
+    zx = np.linalg.norm(ym - y64) / np.linalg.norm(y64)
+    print("config3 mixed c32", cmp, "zx", zx, b.stats(), flush=True)
+    b.close()
+    a.close()
+    assert cmp["rel_fro"] <= MIXED_TOL
+    assert zx <= MIXED_TOL

@@ -138,3 +138,43 @@ def test_null_arguments_are_rejected():
     assert L.bmi_assemble(None, None) == _bmi.EINVAL
     assert b"NULL" in L.bmi_last_error()
     assert L.bmi_matrix_free(None) == _bmi.OK
+
+
+def _c_layout(struct: str, fields: list) -> list:
+    """sizeof and offsetof of a struct from include/bmi.h, by compiling a probe with gcc."""
+    import subprocess
+    import tempfile
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bmi.h"', "int main(void) {",
+             f'  printf("%zu\\n", sizeof({struct}));']
+    lines += [f'  printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        with open(src, "w") as fh:
+            fh.write("\n".join(lines))
+        subprocess.run(["gcc", "-I", os.path.join(root, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    return [int(v) for v in out]
+
+
+@pytest.mark.parametrize("cname,pyname", [
+    ("bmi_operator", "Operator"), ("bmi_options", "Options"), ("bmi_stats", "Stats"),
+    ("bmi_server_options", "ServerOptions"), ("bmi_request", "Request"), ("bmi_result", "Result"),
+])
+def test_ctypes_layouts_match_c_compiler(cname, pyname):
+    S = getattr(_bmi, pyname)
+    names = [f[0] for f in S._fields_]
+    want = _c_layout(cname, names)
+    got = [ctypes.sizeof(S)] + [getattr(S, n).offset for n in names]
+    assert got == want
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_server_without_device_fails_loudly():
+    from beastgpu_amd.serving import Server
+
+    with pytest.raises(_bmi.BMIError):
+        Server(slots=2)
