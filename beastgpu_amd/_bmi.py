@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libbmi.so")
 
 OK, EINVAL, ENOMEM, EDEVICE, EBUSY, ECANCELED, EUNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
 SPACE_LAGRANGE_C0D1, SPACE_RWG = 1, 2
-OP_HH3D_SINGLELAYER, OP_MW_SINGLELAYER = 1, 2
+OP_HH3D_SINGLELAYER, OP_MW_SINGLELAYER, OP_MW_DOUBLELAYER = 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 C64, C32 = 0, 1
 CLASS_NAMES = ("far", "common_vertex", "common_edge", "common_face", "near", "far_fp64")

@@ -17,7 +17,7 @@ import dataclasses
 import numpy as np
 
 from . import _bmi
-from .operators import HH3D_SINGLELAYER, MW_SINGLELAYER, Operator
+from .operators import HH3D_SINGLELAYER, MW_DOUBLELAYER, MW_SINGLELAYER, Operator
 from .spaces import P1, RWG, Space
 
 
@@ -119,7 +119,8 @@ class DeviceSpace(_Handle):
 
 def _c_operator(op: Operator) -> _bmi.Operator:
     o = _bmi.Operator()
-    o.kind = {HH3D_SINGLELAYER: _bmi.OP_HH3D_SINGLELAYER, MW_SINGLELAYER: _bmi.OP_MW_SINGLELAYER}[op.kind]
+    o.kind = {HH3D_SINGLELAYER: _bmi.OP_HH3D_SINGLELAYER, MW_SINGLELAYER: _bmi.OP_MW_SINGLELAYER,
+              MW_DOUBLELAYER: _bmi.OP_MW_DOUBLELAYER}[op.kind]
     for name in ("alpha", "beta", "gamma"):
         z = complex(getattr(op, name))
         getattr(o, name)[0] = z.real

@@ -1017,9 +1017,9 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     if (op->kind == BMI_OP_HH3D_SINGLELAYER) {
         if (test->kind != BMI_SPACE_LAGRANGE_C0D1 || trial->kind != BMI_SPACE_LAGRANGE_C0D1)
             return fail(BMI_EUNSUPPORTED, "the Helmholtz single layer is assembled on lagrangec0d1 spaces");
-    } else if (op->kind == BMI_OP_MW_SINGLELAYER) {
+    } else if (op->kind == BMI_OP_MW_SINGLELAYER || op->kind == BMI_OP_MW_DOUBLELAYER) {
         if (test->kind != BMI_SPACE_RWG || trial->kind != BMI_SPACE_RWG)
-            return fail(BMI_EUNSUPPORTED, "the Maxwell single layer is assembled on RWG spaces");
+            return fail(BMI_EUNSUPPORTED, "the Maxwell operators are assembled on RWG spaces");
     } else {
         return fail(BMI_EUNSUPPORTED, "unknown operator kind");
     }
@@ -1040,7 +1040,7 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     P->Y = trial;
     P->op = *op;
     P->opts = o;
-    P->op_kind = op->kind == BMI_OP_MW_SINGLELAYER ? OPK_MW : OPK_HH3D;
+    P->op_kind = op->kind == BMI_OP_MW_SINGLELAYER ? OPK_MW : (op->kind == BMI_OP_MW_DOUBLELAYER ? OPK_DL : OPK_HH3D);
     P->opc.alpha_re = op->alpha[0];
     P->opc.alpha_im = op->alpha[1];
     P->opc.beta_re = op->beta[0];

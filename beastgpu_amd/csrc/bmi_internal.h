@@ -25,7 +25,7 @@ static_assert(BMI_PANEL_ROWS <= 64, "a panel's rows are tracked in a 64-bit mask
 #define BMI_MAX_RHS 8       // right-hand sides per bmi_matrix_apply call
 #define BMI_APPLY_CHUNK 512 // columns per work-group of the apply kernel
 
-enum { OPK_HH3D = 1, OPK_MW = 2 };
+enum { OPK_HH3D = 1, OPK_MW = 2, OPK_DL = 3 };
 enum { CLS_FAR = 0, CLS_CV = 1, CLS_CE = 2, CLS_CF = 3, CLS_NEAR = 4, CLS_FAR64 = 5 };
 // flag on a special-pair entry's class: the pair is integrated in the entry
 // of the test cell's home group and copied here

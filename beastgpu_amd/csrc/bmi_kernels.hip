@@ -133,6 +133,49 @@ __device__ __forceinline__ void green64(const OpConst &op, bool lossy, double w4
     hi = -h * s;
 }
 
+// grad_x G factor for one point pair in FP64: grad_x G(r) = q(R) r with
+// q = -(1 + gamma R) exp(-gamma R) / (4 pi R^3); returns h = 4 pi w4pi q as (re, im).
+template <bool TAB>
+__device__ __forceinline__ void green_grad64(const OpConst &op, bool lossy, double w4pi, double dx, double dy,
+                                             double dz, double &hr, double &hi, const double2 *tab) {
+    const double R2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double rinv = rsqrt64(R2);
+    const double R = R2 * rinv;
+    const double tt = R * op.k2pi;
+    double s, c;
+    if (TAB) sincos_turns64_tab(tt, tab, s, c);
+    else sincos_turns64(tt, s, c);
+    const double kr = tt * 6.283185307179586476925;  // kappa R
+    double a = 1.0, h = -w4pi * rinv * rinv * rinv;
+    if (lossy) {
+        a = fma(op.gamma_re, R, 1.0);
+        h *= exp(-op.gamma_re * R);
+    }
+    // (a + i kr)(c - i s)
+    hr = h * fma(a, c, kr * s);
+    hi = h * fma(kr, c, -(a * s));
+}
+
+// FP32 variant: phase reduced modulo one turn in FP64 first (as green32).
+__device__ __forceinline__ void green_grad32(const OpConst &op, bool lossy, float w4pi, float dx, float dy, float dz,
+                                             float &hr, float &hi) {
+    const float R2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float rinv = __builtin_amdgcn_rsqf(R2);
+    const float R = R2 * rinv;
+    const double tt = (double)R * op.k2pi;
+    const float f = (float)(tt - rint(tt));
+    const float s = __builtin_amdgcn_sinf(f);
+    const float c = __builtin_amdgcn_cosf(f);
+    const float kr = (float)(tt * 6.283185307179586476925);
+    float a = 1.0f, h = -w4pi * rinv * rinv * rinv;
+    if (lossy) {
+        a = fmaf((float)op.gamma_re, R, 1.0f);
+        h *= __expf(-(float)op.gamma_re * R);
+    }
+    hr = h * fmaf(a, c, kr * s);
+    hi = h * fmaf(kr, c, -(a * s));
+}
+
 // The far triangle rules of order 1..3 as compile-time values, equal to the
 // ones the host writes into RuleConst (weights on the reference simplex sum to
 // 1/2; w4pi = w_I w_J / (4 pi) in the same operation order).
@@ -213,6 +256,115 @@ __device__ __forceinline__ void hh_epilogue(const OpConst &op, const double *pr,
     }
 }
 
+// ------------------------------------- far pair, double layer (n x K)
+
+// Double layer tested with rotated RWG functions, one far pair.  With
+// r = d + x' - y' (d = c_K - c_L, primed points relative to the centroids),
+// a_i = x' - alpha_i, b_j = y' - beta_j, t = n.y', p = n.r = n.d - t:
+//   (n x a_i).(grad G x b_j) = q [p (a_i.b_j) - (n.b_j)(a_i.r)]
+// = q [p x'.y' - beta_j.(p x') - alpha_i.(p y') + (alpha_i.beta_j) p
+//      - t x'.r + alpha_i.(t r) + (n.beta_j) x'.r - (n.beta_j) alpha_i.r],
+// so eight moments of h = w w' q carry the pair; the 1/(2A) of each RWG
+// shape cancels the Jacobian 2A of the point weights.  S is the arithmetic
+// type (double, or float for the mixed path).
+template <typename S, int FAST, bool TAB>
+__device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule_, const double *xp, const double *xa,
+                                        const double *xc, const double *yp, const double *ya, const double *yc,
+                                        double2 loc[9], const double2 *tab) {
+    const FarRule<FAST> rule{rule_, op};
+    const bool lossy = rule.lossy();
+    const S dcx = (S)(xc[0] - yc[0]), dcy = (S)(xc[1] - yc[1]), dcz = (S)(xc[2] - yc[2]);
+    // unit normal of the test cell, (v1 - v3) x (v2 - v3) direction
+    double nx, ny, nz;
+    {
+        const double e1x = xa[0] - xa[6], e1y = xa[1] - xa[7], e1z = xa[2] - xa[8];
+        const double e2x = xa[3] - xa[6], e2y = xa[4] - xa[7], e2z = xa[5] - xa[8];
+        nx = e1y * e2z - e1z * e2y;
+        ny = e1z * e2x - e1x * e2z;
+        nz = e1x * e2y - e1y * e2x;
+        const double ninv = rsqrt64(nx * nx + ny * ny + nz * nz);
+        nx *= ninv;
+        ny *= ninv;
+        nz *= ninv;
+    }
+    const S n0 = (S)nx, n1 = (S)ny, n2 = (S)nz;
+    const S nd = n0 * dcx + n1 * dcy + n2 * dcz;
+    S Ar = 0, Ai = 0, Dr = 0, Di = 0, Er = 0, Ei = 0, Gr = 0, Gi = 0;
+    S Br[3] = {}, Bi[3] = {}, Cr[3] = {}, Ci[3] = {}, Fr[3] = {}, Fi[3] = {}, Hr[3] = {}, Hi[3] = {};
+#pragma unroll
+    for (int I = 0; I < BMI_MAX_RULE; ++I) {
+        if (I >= rule.nt()) break;
+        const S x0 = (S)xp[3 * I], x1 = (S)xp[3 * I + 1], x2 = (S)xp[3 * I + 2];
+        const S u0 = dcx + x0, u1 = dcy + x1, u2 = dcz + x2;
+        S shr = 0, shi = 0, str = 0, sti = 0;
+        S syr[3] = {}, syi[3] = {}, styr[3] = {}, styi[3] = {};
+#pragma unroll
+        for (int J = 0; J < BMI_MAX_RULE; ++J) {
+            if (J >= rule.nb()) break;
+            const S y0 = (S)yp[3 * J], y1 = (S)yp[3 * J + 1], y2 = (S)yp[3 * J + 2];
+            S hr, hi;
+            if constexpr (sizeof(S) == 8) {
+                green_grad64<TAB>(op, lossy, rule.w4pi(I, J), u0 - y0, u1 - y1, u2 - y2, hr, hi, tab);
+            } else {
+                green_grad32(op, lossy, (float)rule.w4pi(I, J), u0 - y0, u1 - y1, u2 - y2, hr, hi);
+            }
+            const S t = n0 * y0 + n1 * y1 + n2 * y2;
+            const S htr = hr * t, hti = hi * t;
+            shr += hr;
+            shi += hi;
+            str += htr;
+            sti += hti;
+            syr[0] = fma(hr, y0, syr[0]); syr[1] = fma(hr, y1, syr[1]); syr[2] = fma(hr, y2, syr[2]);
+            syi[0] = fma(hi, y0, syi[0]); syi[1] = fma(hi, y1, syi[1]); syi[2] = fma(hi, y2, syi[2]);
+            styr[0] = fma(htr, y0, styr[0]); styr[1] = fma(htr, y1, styr[1]); styr[2] = fma(htr, y2, styr[2]);
+            styi[0] = fma(hti, y0, styi[0]); styi[1] = fma(hti, y1, styi[1]); styi[2] = fma(hti, y2, styi[2]);
+        }
+        // sums over J of h p and h p y' (p = n.d - t)
+        const S pr = nd * shr - str, pi = nd * shi - sti;
+        const S pyr[3] = {nd * syr[0] - styr[0], nd * syr[1] - styr[1], nd * syr[2] - styr[2]};
+        const S pyi[3] = {nd * syi[0] - styi[0], nd * syi[1] - styi[1], nd * syi[2] - styi[2]};
+        const S xu = x0 * u0 + x1 * u1 + x2 * u2;  // x'.(d + x')
+        const S x[3] = {x0, x1, x2}, u[3] = {u0, u1, u2};
+        Dr += pr;
+        Di += pi;
+        Ar += x0 * pyr[0] + x1 * pyr[1] + x2 * pyr[2];
+        Ai += x0 * pyi[0] + x1 * pyi[1] + x2 * pyi[2];
+        Er += xu * str - (x0 * styr[0] + x1 * styr[1] + x2 * styr[2]);
+        Ei += xu * sti - (x0 * styi[0] + x1 * styi[1] + x2 * styi[2]);
+        Gr += xu * shr - (x0 * syr[0] + x1 * syr[1] + x2 * syr[2]);
+        Gi += xu * shi - (x0 * syi[0] + x1 * syi[1] + x2 * syi[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Br[k] = fma(x[k], pr, Br[k]);
+            Bi[k] = fma(x[k], pi, Bi[k]);
+            Cr[k] += pyr[k];
+            Ci[k] += pyi[k];
+            Fr[k] += u[k] * str - styr[k];
+            Fi[k] += u[k] * sti - styi[k];
+            Hr[k] += u[k] * shr - syr[k];
+            Hi[k] += u[k] * shi - syi[k];
+        }
+    }
+    const S alr = (S)op.alpha_re, ali = (S)op.alpha_im;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const S a0 = (S)xa[3 * i], a1 = (S)xa[3 * i + 1], a2 = (S)xa[3 * i + 2];
+        // terms that depend on i only
+        const S ir = Ar - (a0 * Cr[0] + a1 * Cr[1] + a2 * Cr[2]) - Er + (a0 * Fr[0] + a1 * Fr[1] + a2 * Fr[2]);
+        const S ii = Ai - (a0 * Ci[0] + a1 * Ci[1] + a2 * Ci[2]) - Ei + (a0 * Fi[0] + a1 * Fi[1] + a2 * Fi[2]);
+        const S aHr = a0 * Hr[0] + a1 * Hr[1] + a2 * Hr[2], aHi = a0 * Hi[0] + a1 * Hi[1] + a2 * Hi[2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const S b0 = (S)ya[3 * j], b1 = (S)ya[3 * j + 1], b2 = (S)ya[3 * j + 2];
+            const S ab = a0 * b0 + a1 * b1 + a2 * b2, nb = n0 * b0 + n1 * b1 + n2 * b2;
+            const S vr = ir - (b0 * Br[0] + b1 * Br[1] + b2 * Br[2]) + ab * Dr + nb * (Gr - aHr);
+            const S vi = ii - (b0 * Bi[0] + b1 * Bi[1] + b2 * Bi[2]) + ab * Di + nb * (Gi - aHi);
+            loc[3 * i + j].x = (double)(alr * vr - ali * vi);
+            loc[3 * i + j].y = (double)(alr * vi + ali * vr);
+        }
+    }
+}
+
 // -------------------------------------------------- far pair, FP64 rule
 
 // Local 3x3 matrix of one far pair with the product rule
@@ -224,6 +376,10 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
                                           const double *xa, const double *xc, double xs, const double *yp,
                                           const double *ya, const double *yc, double ys, double2 loc[9],
                                           const double2 *tab) {
+    if constexpr (OP == OPK_DL) {
+        pair_dl<double, FAST, TAB>(op, rule_, xp, xa, xc, yp, ya, yc, loc, tab);
+        return;
+    }
     const FarRule<FAST> rule{rule_, op};
     const bool lossy = rule.lossy();
     const double dcx = xc[0] - yc[0], dcy = xc[1] - yc[1], dcz = xc[2] - yc[2];
@@ -309,6 +465,10 @@ template <int OP, int FAST>
 __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &rule_, const double *xp,
                                            const double *xa, const double *xc, double xs, const double *yp,
                                            const double *ya, const double *yc, double ys, double2 loc[9]) {
+    if constexpr (OP == OPK_DL) {
+        pair_dl<float, FAST, false>(op, rule_, xp, xa, xc, yp, ya, yc, loc, nullptr);
+        return;
+    }
     const FarRule<FAST> rule{rule_, op};
     const bool lossy = rule.lossy();
     const float dcx = (float)(xc[0] - yc[0]), dcy = (float)(xc[1] - yc[1]), dcz = (float)(xc[2] - yc[2]);
@@ -644,12 +804,43 @@ struct CellG {
 template <int OP>
 __device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const double2 *tab, SpecAcc &acc, const CellG &X, const CellG &Y,
                                            const double *dc, double bx1, double bx2, double by1, double by2,
-                                           double w) {
+                                           double w, const double *nrm) {
     double x[3], y[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         x[d] = X.o[d] + X.t1[d] * bx1 + X.t2[d] * bx2;
         y[d] = Y.o[d] + Y.t1[d] * by1 + Y.t2[d] * by2;
+    }
+    if constexpr (OP == OPK_DL) {
+        // (n x a_c).(grad G x b_c') = q [(n.r)(a_c.b_c') - (n.b_c')(a_c.r)] for the
+        // chart vertices c, c' (a_c = x - chart vertex c, relative to the centroid)
+        const double r0 = dc[0] + x[0] - y[0], r1 = dc[1] + x[1] - y[1], r2 = dc[2] + x[2] - y[2];
+        double hr, hi;
+        green_grad64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), r0, r1, r2, hr, hi, tab);
+        const double nr = nrm[0] * r0 + nrm[1] * r1 + nrm[2] * r2;
+        double a[3][3], b[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a[0][d] = x[d] - (X.o[d] + X.t1[d]);
+            a[1][d] = x[d] - (X.o[d] + X.t2[d]);
+            a[2][d] = x[d] - X.o[d];
+            b[0][d] = y[d] - (Y.o[d] + Y.t1[d]);
+            b[1][d] = y[d] - (Y.o[d] + Y.t2[d]);
+            b[2][d] = y[d] - Y.o[d];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double ar = a[c][0] * r0 + a[c][1] * r1 + a[c][2] * r2;
+#pragma unroll
+            for (int c2 = 0; c2 < 3; ++c2) {
+                const double ab = a[c][0] * b[c2][0] + a[c][1] * b[c2][1] + a[c][2] * b[c2][2];
+                const double nb = nrm[0] * b[c2][0] + nrm[1] * b[c2][1] + nrm[2] * b[c2][2];
+                const double v = nr * ab - nb * ar;
+                acc.v[2 * (3 * c + c2)] = fma(hr, v, acc.v[2 * (3 * c + c2)]);
+                acc.v[2 * (3 * c + c2) + 1] = fma(hi, v, acc.v[2 * (3 * c + c2) + 1]);
+            }
+        }
+        return;
     }
     double hr, hi;
     green64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1], dc[2] + x[2] - y[2], hr, hi,
@@ -824,6 +1015,24 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
             Y.t2[d] = y1 - y2;
             dc[d] = cx - cy;
         }
+        // unit normal of the test cell ((v1 - v3) x (v2 - v3) direction; the
+        // chart permutation may reverse t1 x t2)
+        double nrm[3] = {0.0, 0.0, 0.0};
+        if (OP == OPK_DL) {
+            double e1[3], e2[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                e1[d] = a.X.vtx[9 * K + d] - a.X.vtx[9 * K + 6 + d];
+                e2[d] = a.X.vtx[9 * K + 3 + d] - a.X.vtx[9 * K + 6 + d];
+            }
+            nrm[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            nrm[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            nrm[2] = e1[0] * e2[1] - e1[1] * e2[0];
+            const double ninv = rsqrt64(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            nrm[0] *= ninv;
+            nrm[1] *= ninv;
+            nrm[2] *= ninv;
+        }
         SpecAcc acc;
 #pragma unroll
         for (int i = 0; i < 18; ++i) acc.v[i] = 0.0;
@@ -834,7 +1043,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
             for (int idx = lane; idx < tot; idx += 64) {
                 const int I = idx / np, J = idx - np * (idx / np);
                 spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, s_nb1[I], s_nb2[I], s_nb1[J], s_nb2[J],
-                               s_nw[I] * s_nw[J]);
+                               s_nw[I] * s_nw[J], nrm);
             }
         } else {
             const int g = cls - 1;
@@ -851,7 +1060,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
                     const double wgt = s_glw[g][i1] * s_glw[g][i2] * s_glw[g][i3] * s_glw[g][i4];
                     double u1, u2, v1, v2, m;
                     ss_map(cls, r, s_glx[g][i1], s_glx[g][i2], s_glx[g][i3], s_glx[g][i4], wgt, u1, u2, v1, v2, m);
-                    spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m);
+                    spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
                 }
             }
         }
@@ -897,7 +1106,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
                         pr[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2)];
                         pi[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2) + 1];
                     }
-                hh_epilogue(a.op, pr, pi, 4.0 * a.X.area[K] * a.Y.area[L], loc);
+                // Helmholtz: Jacobian 4 A_K A_L; double layer: it cancels the
+                // 1/(2A) of the two RWG shapes
+                hh_epilogue(a.op, pr, pi, OP == OPK_DL ? 1.0 : 4.0 * a.X.area[K] * a.Y.area[L], loc);
             }
 #pragma unroll
             for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
@@ -966,10 +1177,12 @@ hipError_t launch_special(const SpecialArgs &a, int capacity, int mixed, int op_
     if (capacity <= 0) return hipSuccess;
     const int grid = std::min((capacity + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, 4096);
     if (op_kind == OPK_MW) hipLaunchKernelGGL(k_special<OPK_MW>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
+    else if (op_kind == OPK_DL) hipLaunchKernelGGL(k_special<OPK_DL>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
     else hipLaunchKernelGGL(k_special<OPK_HH3D>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
     const int g2 = std::min((capacity + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, 2048);
     if (mixed) {
         if (op_kind == OPK_MW) hipLaunchKernelGGL(k_far64<OPK_MW>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL(k_far64<OPK_DL>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
         else hipLaunchKernelGGL(k_far64<OPK_HH3D>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
     }
     hipLaunchKernelGGL(k_link, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
@@ -1182,7 +1395,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
 // the summation order is fixed.  RWG functions have at most two cells, so
 // there are at most 2 x 2 rounds.
 template <int OP, int MIXED, int FAST, typename OutT>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS, 3) void k_panels_rr(PanelArgs a) {
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_rr(PanelArgs a) {
     extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x ncm
     __shared__ double sg[2][BMI_GEO];             // test sub-batch geometry, double-buffered
     __shared__ double bg[BMI_GEO];                // trial patch geometry
@@ -1356,6 +1569,9 @@ hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hi
     if (op_kind == OPK_MW) {
         if (mixed) launch_panels_m<OPK_MW, 1>(a, c32, s);
         else launch_panels_m<OPK_MW, 0>(a, c32, s);
+    } else if (op_kind == OPK_DL) {
+        if (mixed) launch_panels_m<OPK_DL, 1>(a, c32, s);
+        else launch_panels_m<OPK_DL, 0>(a, c32, s);
     } else {
         if (mixed) launch_panels_m<OPK_HH3D, 1>(a, c32, s);
         else launch_panels_m<OPK_HH3D, 0>(a, c32, s);

@@ -10,6 +10,11 @@ doubleQuadRuleMainCase.jl:143-146).
 * ``Maxwell3D.singlelayer(wavenumber=k)``: EFIE single layer,
   Z_mn = alpha <f_m, G f_n> + beta <div f_m, G div f_n> with BEAST's defaults
   alpha = -gamma = -ik and beta = -1/gamma = 1/(-ik).
+* ``Maxwell3D.doublelayer(wavenumber=k)``: the double layer K tested with
+  rotated RWG functions, the n x K block of the MFIE (BEAST's
+  ``assemble(K, n x X, X)``; the identity part 1/2 <n x f_m, f_n> is a sparse
+  local Gram matrix outside this path):
+  Z_mn = alpha <n x f_m, grad_x G x f_n>, alpha = 1 by default.
 """
 from __future__ import annotations
 
@@ -17,6 +22,7 @@ import dataclasses
 
 HH3D_SINGLELAYER = 1
 MW_SINGLELAYER = 2
+MW_DOUBLELAYER = 3
 
 
 @dataclasses.dataclass(frozen=True)
@@ -54,3 +60,8 @@ class Maxwell3D:
         a = complex(-g if alpha is None else alpha)
         b = complex(-1.0 / g if beta is None else beta)
         return Operator(MW_SINGLELAYER, a, b, g)
+
+    @staticmethod
+    def doublelayer(*, wavenumber=None, gamma=None, alpha=None) -> Operator:
+        g = _gamma(gamma, wavenumber)
+        return Operator(MW_DOUBLELAYER, complex(1.0 if alpha is None else alpha), 0j, g)

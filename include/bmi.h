@@ -38,9 +38,13 @@ typedef int32_t bmi_status;
 #define BMI_SPACE_LAGRANGE_C0D1 1
 #define BMI_SPACE_RWG 2
 
-/* operators: Helmholtz3D.singlelayer, Maxwell3D.singlelayer */
+/* operators: Helmholtz3D.singlelayer, Maxwell3D.singlelayer, and
+ * Maxwell3D.doublelayer tested with rotated RWG functions n x f_m (the n x K
+ * block of the MFIE, BEAST's assemble(K, n x X, X)):
+ *   Z_mn = alpha < n x f_m, grad_x G x f_n >  (beta unused) */
 #define BMI_OP_HH3D_SINGLELAYER 1
 #define BMI_OP_MW_SINGLELAYER 2
+#define BMI_OP_MW_DOUBLELAYER 3
 
 /* arithmetic of the far-pair double quadrature */
 #define BMI_PREC_FP64 0

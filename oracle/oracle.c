@@ -18,7 +18,12 @@
  * and extends them, from BEAST's operator semantics, to the Maxwell
  * single-layer (EFIE) operator on RWG (Raviart-Thomas, unit-flux) functions,
  * which the reference does not implement:
- *   Z_mn = alpha <f_m, G f_n> + beta <div f_m, G div f_n>,  G = exp(-gamma R)/(4 pi R).
+ *   Z_mn = alpha <f_m, G f_n> + beta <div f_m, G div f_n>,  G = exp(-gamma R)/(4 pi R),
+ * and to the Maxwell double layer tested with rotated RWG functions (the
+ * "n x K" block of the MFIE):
+ *   Z_mn = alpha < n x f_m, grad_x G x f_n >,
+ *   grad_x G = -(x - y) (1 + gamma R) exp(-gamma R) / (4 pi R^3),
+ * n the unit normal of the test cell ((v1 - v3) x (v2 - v3) direction).
  *
  * Pair classes whose rule the reference leaves open: class-4 ("near") pairs are
  * dropped by the reference (doubleQuadRuleMainCase.jl:154); here they are
@@ -39,16 +44,18 @@
 
 typedef double complex cplx;
 
-enum { ORC_HH3D_SL = 1, ORC_MW_SL = 2 };
+enum { ORC_HH3D_SL = 1, ORC_MW_SL = 2, ORC_MW_DL = 3 };
 enum { ORC_P1 = 1, ORC_RWG = 2 };
 enum { ORC_KERNEL_GREEN = 0, ORC_KERNEL_ONE = 1, ORC_KERNEL_LINEAR = 2, ORC_KERNEL_QUAD = 3 };
 enum { ORC_FAR = 0, ORC_CV = 1, ORC_CE = 2, ORC_CF = 3, ORC_NEAR = 4 };
 
 typedef struct {
-    int32_t op;      /* ORC_HH3D_SL | ORC_MW_SL */
+    int32_t op;      /* ORC_HH3D_SL | ORC_MW_SL | ORC_MW_DL */
     int32_t kernel;  /* ORC_KERNEL_*; anything but GREEN is a polynomial test kernel */
     double alpha[2], beta[2], gamma[2];
-    double lin[3];   /* ORC_KERNEL_LINEAR: G = 1 + lin . (x - y) */
+    double lin[3];   /* ORC_KERNEL_LINEAR: G = 1 + lin . (x - y).  Double layer:
+                        grad G replaced by lin (ONE), lin + (x - y) (LINEAR),
+                        |x - y|^2 lin (QUAD) */
 } orc_operator;
 
 typedef struct {
@@ -152,6 +159,7 @@ typedef struct {
     double v[3][3];  /* vertices */
     double t1[3], t2[3];
     double area;
+    double n[3];     /* unit normal, direction of t1 x t2 */
 } cell;
 
 static void load_cell(const orc_space *s, int64_t k, cell *c) {
@@ -166,7 +174,11 @@ static void load_cell(const orc_space *s, int64_t k, cell *c) {
     double n0 = c->t1[1] * c->t2[2] - c->t1[2] * c->t2[1];
     double n1 = c->t1[2] * c->t2[0] - c->t1[0] * c->t2[2];
     double n2 = c->t1[0] * c->t2[1] - c->t1[1] * c->t2[0];
-    c->area = 0.5 * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    c->area = 0.5 * nn;
+    c->n[0] = n0 / nn;
+    c->n[1] = n1 / nn;
+    c->n[2] = n2 / nn;
 }
 
 static inline void chart_point(const cell *c, double b1, double b2, double *x) {
@@ -202,6 +214,49 @@ static inline cplx kernel_value(const orc_operator *op, const double *x, const d
         cplx g = op->gamma[0] + I * op->gamma[1];
         return cexp(-g * R) / (4.0 * M_PI * R);
     }
+    }
+}
+
+/* grad_x G(x - y) of the double layer, or its polynomial test stand-in */
+static void kernel_gradient(const orc_operator *op, const double *x, const double *y, cplx gr[3]) {
+    double r[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+    double R2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    switch (op->kernel) {
+    case ORC_KERNEL_ONE:
+        for (int d = 0; d < 3; d++) gr[d] = op->lin[d];
+        return;
+    case ORC_KERNEL_LINEAR:
+        for (int d = 0; d < 3; d++) gr[d] = op->lin[d] + r[d];
+        return;
+    case ORC_KERNEL_QUAD:
+        for (int d = 0; d < 3; d++) gr[d] = R2 * op->lin[d];
+        return;
+    default: {
+        double R = sqrt(R2);
+        cplx g = op->gamma[0] + I * op->gamma[1];
+        cplx q = -(1.0 + g * R) * cexp(-g * R) / (4.0 * M_PI * R * R2);
+        for (int d = 0; d < 3; d++) gr[d] = q * r[d];
+    }
+    }
+}
+
+/* double layer with rotated test functions: w alpha (n x f_i) . (grad G x g_j) */
+static void accumulate_dl(const orc_operator *op, const cell *t, const shapes *f, const shapes *g,
+                          const double *x, const double *y, double w, cplx Z[3][3]) {
+    cplx a = op->alpha[0] + I * op->alpha[1];
+    cplx gr[3];
+    kernel_gradient(op, x, y, gr);
+    for (int i = 0; i < 3; i++) {
+        const double *fi = f->val[i];
+        double u[3] = {t->n[1] * fi[2] - t->n[2] * fi[1], t->n[2] * fi[0] - t->n[0] * fi[2],
+                       t->n[0] * fi[1] - t->n[1] * fi[0]};
+        for (int j = 0; j < 3; j++) {
+            const double *gj = g->val[j];
+            cplx c0 = gr[1] * gj[2] - gr[2] * gj[1];
+            cplx c1 = gr[2] * gj[0] - gr[0] * gj[2];
+            cplx c2 = gr[0] * gj[1] - gr[1] * gj[0];
+            Z[i][j] += w * a * (u[0] * c0 + u[1] * c1 + u[2] * c2);
+        }
     }
 }
 
@@ -265,7 +320,8 @@ static void double_rule(const orc_operator *op, const orc_space *X, const orc_sp
             shapes g;
             eval_shapes(Y->kind, s, rs->b1[jb], rs->b2[jb], y, &g);
             double wy = rs->w[jb] * 2.0 * s->area;
-            accumulate(op, X->kind, Y->kind, &f, &g, kernel_value(op, x, y), wx * wy, Z);
+            if (op->op == ORC_MW_DL) accumulate_dl(op, t, &f, &g, x, y, wx * wy, Z);
+            else accumulate(op, X->kind, Y->kind, &f, &g, kernel_value(op, x, y), wx * wy, Z);
         }
     }
 }
@@ -327,7 +383,8 @@ static void ss_eval(const orc_operator *op, const orc_space *X, const orc_space 
     shapes f, g;
     eval_shapes(X->kind, t, bx1, bx2, x, &f);
     eval_shapes(Y->kind, s, by1, by2, y, &g);
-    accumulate(op, X->kind, Y->kind, &f, &g, kernel_value(op, x, y), 4.0 * t->area * s->area * mul, Z);
+    if (op->op == ORC_MW_DL) accumulate_dl(op, t, &f, &g, x, y, 4.0 * t->area * s->area * mul, Z);
+    else accumulate(op, X->kind, Y->kind, &f, &g, kernel_value(op, x, y), 4.0 * t->area * s->area * mul, Z);
 }
 
 static void sauter_schwab(const orc_operator *op, const orc_space *X, const orc_space *Y,
@@ -380,6 +437,7 @@ static void sauter_schwab(const orc_operator *op, const orc_space *X, const orc_
 /* Local element matrix of one (test cell p, trial cell q) pair; returns its class. */
 int orc_pair_block(const orc_operator *op, const orc_space *X, const orc_space *Y,
                    const orc_quadrature *q, int64_t p, int64_t qq, double *out /* 18 doubles, (i,j) row-major, re/im */) {
+    if (op->op == ORC_MW_DL && (X->kind != ORC_RWG || Y->kind != ORC_RWG)) return -1;
     cell t, s;
     load_cell(X, p, &t);
     load_cell(Y, qq, &s);

@@ -22,6 +22,16 @@ Parity status of each oracle function, and what pins it (tests named):
   a brute-force near test (test_classifier_*).
 * scatter (store_with_kernel.jl:88-97) -- pinned by renumbering invariance and
   by the G = 1 closed form for whole matrices (test_scatter_*).
+* double layer with rotated test functions (the n x K block of the MFIE; the
+  reference has no such operator) -- pinned by polynomial stand-ins of grad G
+  for every pair class against numpy cubature (test_dl_ss_polynomial_kernels),
+  the constant-gradient whole-matrix closed form
+  (test_dl_closed_form_constant_gradient), the gradient factor against central
+  differences of G (test_dl_gradient_factor_by_finite_differences), the
+  vanishing of the operator on a flat sheet (test_dl_vanishes_on_a_flat_sheet)
+  and convergence in the Sauter-Schwab order (test_dl_ss_convergence).  BEAST's
+  own MWDoubleLayer3D integrand is not importable here: its sign and
+  normalisation conventions are parity unpinned.
 * BEAST conventions that only an import of BEAST 2.6.0 / CompScienceMeshes
   0.9.3 / SauterSchwabQuadrature 2.4.1 could fix (triangle-rule tables and
   point order, RT orientation signs and edge numbering, near-pair WiltonSE

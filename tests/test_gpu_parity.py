@@ -194,3 +194,44 @@ def test_config3_ellipsoid_full_size(gpu_ctx):
     a.close()
     assert cmp["rel_fro"] <= MIXED_TOL
     assert zx <= MIXED_TOL
+
+
+@pytest.mark.parametrize("level,k,amp", [(1, 1.0, 0.0), (2, 2.0, 0.03), (3, 1.5, 0.02)])
+def test_double_layer_fp64_matches_oracle(gpu_ctx, level, k, amp):
+    """MFIE block n x K on RWG, FP64 path."""
+    m = meshes.icosphere(level)
+    if amp:
+        m = meshes.perturb_radial(m, 3, amp)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.doublelayer(wavenumber=k)
+    print(f"case dl L{level} k={k}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(), ctx=gpu_ctx)
+    Z = a.assemble().to_host()
+    st = a.stats()
+    a.close()
+    Zref, counts = O.assemble(op, X, X)
+    err = relfro(Z, Zref)
+    print("dl rel err", err)
+    assert [st["pairs"][n] for n in ("far", "common_vertex", "common_edge", "common_face", "near")] == list(counts)
+    assert err <= FP64_TOL
+
+
+@pytest.mark.parametrize("variant", ["mixed", "lossy", "rules"])
+def test_double_layer_variants(gpu_ctx, variant):
+    m = meshes.perturb_radial(meshes.icosphere(2), 8, 0.03)
+    X = spaces.raviartthomas(m)
+    if variant == "lossy":
+        op = Maxwell3D.doublelayer(gamma=complex(0.5, 2.0), alpha=0.7 - 0.4j)
+    else:
+        op = Maxwell3D.doublelayer(wavenumber=2.5)
+    cfg = GPUConfiguration(precision="mixed" if variant == "mixed" else "fp64",
+                           far_test_order=3 if variant == "rules" else 2,
+                           far_trial_order=2 if variant == "rules" else 3)
+    print(f"case dl {variant}", flush=True)
+    a = Assembler(op, X, X, cfg, ctx=gpu_ctx)
+    Z = a.assemble().to_host()
+    a.close()
+    Zref, _ = O.assemble(op, X, X, far_test=cfg.far_test_order, far_trial=cfg.far_trial_order)
+    err = relfro(Z, Zref)
+    print("dl rel err", err)
+    assert err <= (MIXED_TOL if variant == "mixed" else FP64_TOL)

@@ -317,3 +317,144 @@ def test_rows_subset_matches_full():
     Zr, _ = O.assemble(op, X, X, rows=rows)
     # same terms; the colouring (hence the summation order) differs
     assert np.abs(Zr - Z[rows]).max() <= 1e-15 * np.abs(Z).max()
+
+
+# ------------------------------------------------- double layer (n x K)
+
+def _dl_poly_reference(op, X, p, q, kernel, lin, n=10):
+    """alpha int_K int_L (n_K x f_i) . (P(x - y) x g_j) with the high-order
+    collapsed rule, P the polynomial stand-in of grad G (exact here)."""
+    tv = X.mesh.vertices[X.mesh.faces[p]]
+    sv = X.mesh.vertices[X.mesh.faces[q]]
+    b1, b2, w = ind.collapsed(n)
+    x, y = ind.cell_points(tv, b1, b2), ind.cell_points(sv, b1, b2)
+    At, As = ind.area(tv), ind.area(sv)
+    nrm = np.cross(tv[0] - tv[2], tv[1] - tv[2])
+    nrm /= np.linalg.norm(nrm)
+    d = x[:, None, :] - y[None, :, :]
+    lin = np.asarray(lin)
+    if kernel == O.KERNEL_ONE:
+        P = np.broadcast_to(lin, d.shape)
+    elif kernel == O.KERNEL_LINEAR:
+        P = lin + d
+    else:
+        P = np.einsum("abk,abk->ab", d, d)[..., None] * lin
+    W = (w * 2 * At)[:, None] * (w * 2 * As)[None, :]
+    Z = np.zeros((3, 3), dtype=complex)
+    for i in range(3):
+        u = np.cross(nrm, (x - tv[i]) / (2 * At))           # (na, 3)
+        for j in range(3):
+            g = (y - sv[j]) / (2 * As)                       # (nb, 3)
+            c = np.cross(P, np.broadcast_to(g[None, :, :], P.shape))
+            Z[i, j] = op.alpha * np.sum(W * np.einsum("ak,abk->ab", u, c))
+    return Z
+
+
+@pytest.mark.parametrize("kernel", [O.KERNEL_ONE, O.KERNEL_LINEAR, O.KERNEL_QUAD])
+def test_dl_ss_polynomial_kernels(kernel):
+    m = meshes.perturb_radial(meshes.icosphere(1), 5, 0.1)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.doublelayer(wavenumber=1.3, alpha=0.8 + 0.3j)
+    lin = (0.3, -1.1, 0.7)
+    pairs = _pairs_by_class(op, X)
+    for c in (O.CV, O.CE, O.CF, O.FAR):
+        p, q = pairs[c]
+        Zo, cls = O.pair_block(op, X, X, p, q, kernel=kernel, lin=lin, ss_order=(8, 8, 8))
+        assert cls == c
+        if c == O.FAR and kernel == O.KERNEL_QUAD:
+            continue  # degree 4 exceeds the 3-point rule
+        Zr = _dl_poly_reference(op, X, p, q, kernel, lin)
+        assert np.abs(Zo - Zr).max() <= 1e-13 * np.abs(Zr).max(), (c, Zo, Zr)
+
+
+def test_dl_closed_form_constant_gradient():
+    """grad G -> c: Z_mn = alpha (int n x f_m) . (c x int f_n), int_K f_i = (c_K - v_i)/2."""
+    m = meshes.perturb_radial(meshes.icosphere(2), 9, 0.05)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.doublelayer(wavenumber=1.0)
+    cvec = np.array([0.4, -0.9, 1.3])
+    Z, _ = O.assemble(op, X, X, kernel=O.KERNEL_ONE, lin=tuple(cvec))
+    v = m.vertices[m.faces]
+    cen = v.mean(axis=1)
+    nrm = np.cross(v[:, 0] - v[:, 2], v[:, 1] - v[:, 2])
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    a = np.zeros((X.nfun, 3))
+    b = np.zeros((X.nfun, 3))
+    for K in range(m.nt):
+        for i in range(3):
+            f = X.idx[K, i]
+            if f >= 0:
+                fi = X.coeff[K, i] * (cen[K] - v[K, i]) / 2
+                a[f] += np.cross(nrm[K], fi)
+                b[f] += np.cross(cvec, fi)
+    Zr = op.alpha * (a @ b.T)
+    assert np.abs(Z - Zr).max() <= 1e-13 * np.abs(Zr).max()
+
+
+def test_dl_gradient_factor_by_finite_differences():
+    """Centroid rules on a far pair: Z_ij = A_K A_L (n x f_i(c_K)) . (grad G(c_K - c_L) x g_j(c_L)),
+    grad G by central differences of exp(-gamma R)/(4 pi R)."""
+    m = meshes.perturb_radial(meshes.icosphere(2), 4, 0.05)
+    X = spaces.raviartthomas(m)
+    g = complex(0.3, 2.1)
+    op = Maxwell3D.doublelayer(gamma=g)
+    p, q = _pairs_by_class(op, X)[O.FAR]
+    Zo, cls = O.pair_block(op, X, X, p, q, far_test=1, far_trial=1)
+    assert cls == O.FAR
+    tv = m.vertices[m.faces[p]]
+    sv = m.vertices[m.faces[q]]
+    cK, cL = tv.mean(axis=0), sv.mean(axis=0)
+    AK, AL = ind.area(tv), ind.area(sv)
+    nrm = np.cross(tv[0] - tv[2], tv[1] - tv[2])
+    nrm /= np.linalg.norm(nrm)
+
+    def G(r):
+        R = np.linalg.norm(r)
+        return np.exp(-g * R) / (4 * np.pi * R)
+
+    h = 1e-5
+    r0 = cK - cL
+    grad = np.array([(G(r0 + h * e) - G(r0 - h * e)) / (2 * h) for e in np.eye(3)])
+    Zr = np.zeros((3, 3), dtype=complex)
+    for i in range(3):
+        u = np.cross(nrm, (cK - tv[i]) / (2 * AK))
+        for j in range(3):
+            w_ = (cL - sv[j]) / (2 * AL)
+            Zr[i, j] = AK * AL * np.dot(u, np.cross(grad, w_))
+    assert np.abs(Zo - Zr).max() <= 1e-8 * np.abs(Zr).max()
+
+
+def test_dl_vanishes_on_a_flat_sheet():
+    """Coplanar cells: grad G x f_n is normal to the plane, n x f_m lies in it."""
+    nx = 6
+    xs = np.linspace(0.0, 1.0, nx + 1)
+    V = np.array([(x, y, 0.0) for y in xs for x in xs])
+    F = []
+    for j in range(nx):
+        for i in range(nx):
+            a, b, c, d = j * (nx + 1) + i, j * (nx + 1) + i + 1, (j + 1) * (nx + 1) + i, (j + 1) * (nx + 1) + i + 1
+            F += [(a, b, d), (a, d, c)]
+    m = meshes.Mesh(V, np.array(F, dtype=np.int32))
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.doublelayer(wavenumber=3.0)
+    Z, counts = O.assemble(op, X, X)
+    assert counts[1] > 0 and counts[2] > 0 and counts[3] > 0
+    assert np.abs(Z).max() <= 1e-14
+
+
+def test_dl_ss_convergence():
+    m = meshes.perturb_radial(meshes.icosphere(1), 3, 0.1)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.doublelayer(wavenumber=2.0)
+    pairs = _pairs_by_class(op, X)
+    far, _ = O.pair_block(op, X, X, *pairs[O.FAR])
+    for c in (O.CV, O.CE):
+        p, q = pairs[c]
+        ref, _ = O.pair_block(op, X, X, p, q, ss_order=(14, 14, 14))
+        e5 = np.abs(O.pair_block(op, X, X, p, q, ss_order=(5, 5, 5))[0] - ref).max()
+        e8 = np.abs(O.pair_block(op, X, X, p, q, ss_order=(8, 8, 8))[0] - ref).max()
+        scale = np.abs(ref).max()
+        assert e8 <= 1e-6 * scale and e8 < e5, (c, e5, e8, scale)
+    # a cell against itself is coplanar: its block vanishes
+    cf, _ = O.pair_block(op, X, X, *pairs[O.CF])
+    assert np.abs(cf).max() <= 1e-14 * np.abs(far).max() * 1e3
