@@ -31,7 +31,9 @@ EXPORTS = (
     "bmi_matrix_apply", "bmi_matrix_compare", "bmi_device_memory",
     "bmi_server_create", "bmi_server_destroy", "bmi_server_submit", "bmi_server_wait",
     "bmi_server_release", "bmi_server_memory",
+    "bmi_batch_create", "bmi_batch_assemble", "bmi_batch_stats_get", "bmi_batch_free",
 )
+BATCH_MAX_BUCKETS = 8
 REQ_ASSEMBLE, REQ_ASSEMBLE_APPLY, REQ_APPLY = 0, 1, 2
 
 
@@ -127,6 +129,27 @@ class Result(ctypes.Structure):
                 "total_ms": self.total_ms, "entries": int(self.entries)}
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("last_ms", ctypes.c_double),
+        ("problems", ctypes.c_int32),
+        ("buckets", ctypes.c_int32),
+        ("singles", ctypes.c_int32),
+        ("bucket_problems", ctypes.c_int32 * 8),
+        ("bucket_mode", ctypes.c_int32 * 8),
+        ("bucket_wgs_per_cu", ctypes.c_int32 * 8),
+        ("bucket_tuned_ms", ctypes.c_double * 8),
+    ]
+
+    def as_dict(self) -> dict:
+        nb = self.buckets
+        return {"last_ms": self.last_ms, "problems": self.problems, "buckets": nb, "singles": self.singles,
+                "bucket_problems": list(self.bucket_problems[:nb]),
+                "bucket_mode": ["queue" if m == 1 else "per-colour" for m in self.bucket_mode[:nb]],
+                "bucket_wgs_per_cu": list(self.bucket_wgs_per_cu[:nb]),
+                "bucket_tuned_ms": list(self.bucket_tuned_ms[:nb])}
+
+
 _lib = None
 vp = ctypes.c_void_p
 pp = ctypes.POINTER(ctypes.c_void_p)
@@ -168,6 +191,10 @@ def lib():
     L.bmi_server_wait.argtypes = [vp, ctypes.c_uint64, i32, ctypes.POINTER(Result)]
     L.bmi_server_release.argtypes = [vp, ctypes.c_uint64]
     L.bmi_server_memory.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.bmi_batch_create.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i32, pp]
+    L.bmi_batch_assemble.argtypes = [vp]
+    L.bmi_batch_stats_get.argtypes = [vp, ctypes.POINTER(BatchStats)]
+    L.bmi_batch_free.argtypes = [vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if name not in ("bmi_version", "bmi_last_error", "bmi_options_default"):

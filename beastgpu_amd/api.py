@@ -212,6 +212,33 @@ class Assembler:
             o.close()
 
 
+class Batch(_Handle):
+    """Many resident assemblies run as one replayed graph (bmi_batch_*): the
+    assemblers' plans and matrices, on one context."""
+    _free = "bmi_batch_free"
+
+    def __init__(self, assemblers: list):
+        super().__init__()
+        if not assemblers:
+            raise ValueError("empty batch")
+        ctx = assemblers[0].ctx
+        if any(a.ctx is not ctx for a in assemblers):
+            raise ValueError("the assemblers of a batch share one context")
+        self.assemblers = list(assemblers)
+        n = len(assemblers)
+        plans = (ctypes.c_void_p * n)(*[a.plan.h.value for a in assemblers])
+        mats = (ctypes.c_void_p * n)(*[a.matrix.h.value for a in assemblers])
+        _bmi.check(_bmi.lib().bmi_batch_create(ctx.h, plans, mats, n, ctypes.byref(self.h)))
+
+    def assemble(self) -> None:
+        _bmi.check(_bmi.lib().bmi_batch_assemble(self.h))
+
+    def stats(self) -> dict:
+        s = _bmi.BatchStats()
+        _bmi.check(_bmi.lib().bmi_batch_stats_get(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+
 def assemble_gpu(operator: Operator, test_functions: Space, trial_functions: Space,
                  config: GPUConfiguration | None = None, writeBackStrategy: str | None = None) -> np.ndarray:
     """Dense Galerkin matrix of ``operator`` on (test, trial) as a host

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -825,6 +826,86 @@ bmi_status matrix_wrap(bmi_context *ctx, void *mem, int64_t rows, int64_t cols, 
     return BMI_OK;
 }
 
+// The matrix must have the plan's shape and context.
+bmi_status check_target(const bmi_plan *P, const bmi_matrix *out) {
+    if (out->ctx != P->ctx) return fail(BMI_EINVAL, "matrix belongs to another context");
+    if (out->rows != P->X->nfun || out->cols != P->Y->nfun)
+        return fail(BMI_EINVAL, "matrix is " + std::to_string(out->rows) + " x " + std::to_string(out->cols) +
+                                    ", the plan assembles " + std::to_string(P->X->nfun) + " x " +
+                                    std::to_string(P->Y->nfun));
+    return BMI_OK;
+}
+
+// Classification, compaction and special pairs of one assembly on stream s,
+// no host synchronisation (graph-capturable).  ev_mid (optional) is recorded
+// after the classification.
+bmi_status enqueue_prologue(bmi_plan *P, bmi_matrix *out, hipStream_t s, hipEvent_t ev_mid) {
+    const int nbk = (int)P->nblocks;
+    BMI_HIP(launch_classify(P->cargs, 0, s));
+    BMI_HIP(launch_scan(P->d_blk_count.p, P->d_blk_off.p, nbk, P->d_partials.p, P->d_total.p, s));
+    BMI_HIP(launch_classify(P->cargs, 1, s));
+    if (ev_mid) BMI_HIP(hipEventRecord(ev_mid, s));
+    const int mixed = P->opts.precision == BMI_PREC_MIXED;
+    BMI_HIP(launch_special(P->sargs, P->capacity, mixed, P->op_kind, s));
+    // functions without local shapes keep zero rows/columns (a fill kernel:
+    // memset nodes misbehave in replayed graphs)
+    if (P->X->uncovered || P->Y->uncovered)
+        BMI_HIP(launch_fill_zero(out->d, (size_t)out->rows * out->cols * (out->dtype == BMI_C64 ? 16 : 8), s));
+    return BMI_OK;
+}
+
+// Arguments of the far-pair kernels of plan P writing into out.
+PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) {
+    PanelArgs B{};
+    B.npanel = P->X->npanel;
+    B.nrows = (int)P->X->nfun;
+    B.sb_off = P->X->d_sb_off.p;
+    B.tgeo = P->d_tgeo.p;
+    B.trs = P->X->d_trs.p;
+    B.tkl = P->X->d_tkl.p;
+    B.tcf = P->X->d_tcf.p;
+    B.bgeo = P->d_bgeo.p;
+    B.bnc = P->Y->d_bnc.p;
+    B.bfn = P->Y->d_bfn.p;
+    B.bfirst = P->Y->d_bfirst.p;
+    B.bcs = P->Y->d_bcs.p;
+    B.bkl = P->Y->d_bkl.p;
+    B.bcf = P->Y->d_bcf.p;
+    B.tslot = P->X->d_tslot.p;
+    B.tscf = P->X->d_tscf.p;
+    B.bslot = P->Y->d_bslot.p;
+    B.bscf = P->Y->d_bscf.p;
+    B.rounds_t = P->X->rounds_t;
+    B.rounds_b = P->Y->rounds_b;
+    B.ncm = P->Y->ncmax;
+    B.rr = P->X->rounds_t * P->Y->rounds_b <= 6;
+    B.fast = P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
+    B.npb = P->npb;
+    B.blk_off = P->d_blk_off.p;
+    B.spec_loc = P->d_spec_loc.p;
+    B.spec_val = P->d_spec_val.p;
+    B.capacity = P->capacity;
+    B.Z = out->d;
+    B.ld = out->rows;
+    B.op = P->opc;
+    B.rule = P->rule;
+    return B;
+}
+
+// The far-pair launches of one assembly, one per trial colour, on stream s.
+bmi_status enqueue_far(bmi_plan *P, bmi_matrix *out, hipStream_t s, int *launches) {
+    PanelArgs B = panel_args(P, out);
+    const int mixed = P->opts.precision == BMI_PREC_MIXED;
+    *launches = 0;
+    for (int cb = 0; cb < P->Y->ncolour; ++cb) {
+        B.bq_list = P->Y->d_by_colour.p + P->Y->colour_off[cb];
+        B.nbq = P->Y->colour_off[cb + 1] - P->Y->colour_off[cb];
+        BMI_HIP(launch_panels(B, P->op_kind, mixed, out->dtype == BMI_C32, s));
+        ++*launches;
+    }
+    return BMI_OK;
+}
+
 extern "C" {
 
 const char *bmi_version(void) { return "bmi 0.1 (gfx950)"; }
@@ -1162,65 +1243,15 @@ bmi_status bmi_plan_free(bmi_plan *P) {
 
 bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
     if (!P || !out) return fail(BMI_EINVAL, "NULL argument");
-    if (out->ctx != P->ctx) return fail(BMI_EINVAL, "matrix belongs to another context");
-    if (out->rows != P->X->nfun || out->cols != P->Y->nfun)
-        return fail(BMI_EINVAL, "matrix is " + std::to_string(out->rows) + " x " + std::to_string(out->cols) +
-                                    ", the plan assembles " + std::to_string(P->X->nfun) + " x " +
-                                    std::to_string(P->Y->nfun));
+    bmi_status st = check_target(P, out);
+    if (st != BMI_OK) return st;
     BMI_HIP(hipSetDevice(P->ctx->device));
     hipStream_t s = P->ctx->stream;
-    const int nbk = (int)P->nblocks;
     BMI_HIP(hipEventRecord(P->ev[0], s));
-    BMI_HIP(launch_classify(P->cargs, 0, s));
-    BMI_HIP(launch_scan(P->d_blk_count.p, P->d_blk_off.p, nbk, P->d_partials.p, P->d_total.p, s));
-    BMI_HIP(launch_classify(P->cargs, 1, s));
-    BMI_HIP(hipEventRecord(P->ev[1], s));
-    const int mixed = P->opts.precision == BMI_PREC_MIXED;
-    BMI_HIP(launch_special(P->sargs, P->capacity, mixed, P->op_kind, s));
+    if ((st = enqueue_prologue(P, out, s, P->ev[1])) != BMI_OK) return st;
     BMI_HIP(hipEventRecord(P->ev[2], s));
-    if (P->X->uncovered || P->Y->uncovered)
-        BMI_HIP(hipMemsetAsync(out->d, 0, (size_t)out->rows * out->cols * (out->dtype == BMI_C64 ? 16 : 8), s));
-
-    PanelArgs B{};
-    B.npanel = P->X->npanel;
-    B.nrows = (int)P->X->nfun;
-    B.sb_off = P->X->d_sb_off.p;
-    B.tgeo = P->d_tgeo.p;
-    B.trs = P->X->d_trs.p;
-    B.tkl = P->X->d_tkl.p;
-    B.tcf = P->X->d_tcf.p;
-    B.bgeo = P->d_bgeo.p;
-    B.bnc = P->Y->d_bnc.p;
-    B.bfn = P->Y->d_bfn.p;
-    B.bfirst = P->Y->d_bfirst.p;
-    B.bcs = P->Y->d_bcs.p;
-    B.bkl = P->Y->d_bkl.p;
-    B.bcf = P->Y->d_bcf.p;
-    B.tslot = P->X->d_tslot.p;
-    B.tscf = P->X->d_tscf.p;
-    B.bslot = P->Y->d_bslot.p;
-    B.bscf = P->Y->d_bscf.p;
-    B.rounds_t = P->X->rounds_t;
-    B.rounds_b = P->Y->rounds_b;
-    B.ncm = P->Y->ncmax;
-    B.rr = P->X->rounds_t * P->Y->rounds_b <= 6;
-    B.fast = P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
-    B.npb = P->npb;
-    B.blk_off = P->d_blk_off.p;
-    B.spec_loc = P->d_spec_loc.p;
-    B.spec_val = P->d_spec_val.p;
-    B.capacity = P->capacity;
-    B.Z = out->d;
-    B.ld = out->rows;
-    B.op = P->opc;
-    B.rule = P->rule;
     int launches = 0;
-    for (int cb = 0; cb < P->Y->ncolour; ++cb) {
-        B.bq_list = P->Y->d_by_colour.p + P->Y->colour_off[cb];
-        B.nbq = P->Y->colour_off[cb + 1] - P->Y->colour_off[cb];
-        BMI_HIP(launch_panels(B, P->op_kind, mixed, out->dtype == BMI_C32, s));
-        ++launches;
-    }
+    if ((st = enqueue_far(P, out, s, &launches)) != BMI_OK) return st;
     BMI_HIP(hipEventRecord(P->ev[3], s));
     BMI_HIP(hipStreamSynchronize(s));
     float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
@@ -1362,6 +1393,340 @@ bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double
     out[0] = std::sqrt(d2);
     out[1] = std::sqrt(r2);
     out[2] = std::sqrt(dm);
+    return BMI_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+
+namespace {
+
+// Problems of a batch that share one far-pair kernel.
+struct BatchBucket {
+    int op_kind = 0, mixed = 0, fast = 0, c32 = 0;
+    std::vector<int> problems;
+    int ncm = 1;
+    int nitems = 0;
+    int npanels = 0;                // global panel ids (done counters)
+    std::vector<int> colour_off;    // items of colour c: [colour_off[c], colour_off[c + 1])
+    DevBuf<PanelArgs> d_args;
+    DevBuf<int4> d_items;           // (problem in bucket, panel, patch, global panel), colour-major
+    DevBuf<int> d_need;
+    DevBuf<int> d_state;            // [0] queue head, [1] error flag, [2..] done counters
+    int mode = 0, wpc = 0;
+    double tuned_ms = 0.0;
+};
+
+std::mutex g_tune_mu;
+std::vector<std::pair<std::string, std::pair<int, int>>> g_tune_cache;
+
+}  // namespace
+
+struct bmi_batch {
+    bmi_context *ctx = nullptr;
+    std::vector<bmi_plan *> plans;
+    std::vector<bmi_matrix *> mats;
+    std::vector<int> singles;
+    std::deque<BatchBucket> buckets;
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork = nullptr, join[kSide] = {}, ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int cus = 0;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+constexpr int kSpinLimit = 1 << 22;
+
+// The far phase of one bucket on stream s with the bucket's schedule.
+bmi_status enqueue_bucket(BatchBucket &B, int mode, int wpc, int cus, hipStream_t s) {
+    if (mode == 1) {
+        BMI_HIP(launch_zero_ints(B.d_state.p, 2 + B.npanels, s));
+        BMI_HIP(launch_panels_queue(B.d_args.p, B.d_items.p, B.d_need.p, B.nitems, B.ncm, B.d_state.p,
+                                    B.d_state.p + 2, B.d_state.p + 1, kSpinLimit, cus * wpc, B.op_kind, B.mixed,
+                                    B.fast, B.c32, s));
+    } else {
+        for (size_t c = 0; c + 1 < B.colour_off.size(); ++c)
+            BMI_HIP(launch_panels_items(B.d_args.p, B.d_items.p + B.colour_off[c],
+                                        B.colour_off[c + 1] - B.colour_off[c], B.ncm, B.op_kind, B.mixed, B.fast,
+                                        B.c32, s));
+    }
+    return BMI_OK;
+}
+
+bmi_status check_queue_error(BatchBucket &B, hipStream_t s) {
+    int err = 0;
+    BMI_HIP(hipMemcpyAsync(&err, B.d_state.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    BMI_HIP(hipStreamSynchronize(s));
+    if (err) return fail(BMI_EDEVICE, "a far-pair queue wait exceeded its spin limit");
+    return BMI_OK;
+}
+
+std::string bucket_signature(const BatchBucket &B) {
+    int lg = 0;
+    while ((1 << lg) < B.nitems) ++lg;
+    return std::to_string(B.op_kind) + "/" + std::to_string(B.mixed) + "/" + std::to_string(B.fast) + "/" +
+           std::to_string(B.c32) + "/" + std::to_string(B.problems.size()) + "/" + std::to_string(lg);
+}
+
+// Builds the colour-major item list of a bucket.  Problems with more blocks
+// come first inside each colour, so the queue's tail holds small blocks.
+bmi_status build_bucket(bmi_batch *Bt, BatchBucket &B) {
+    std::sort(B.problems.begin(), B.problems.end(), [&](int a, int b) {
+        const bmi_plan *pa = Bt->plans[a], *pb = Bt->plans[b];
+        return (int64_t)pa->X->npanel * pa->npb > (int64_t)pb->X->npanel * pb->npb;
+    });
+    std::vector<PanelArgs> args;
+    std::vector<int> pbase;
+    int maxcol = 0;
+    for (int p : B.problems) {
+        const bmi_plan *P = Bt->plans[p];
+        args.push_back(panel_args(P, Bt->mats[p]));
+        pbase.push_back(B.npanels);
+        B.npanels += P->X->npanel;
+        B.ncm = std::max(B.ncm, P->Y->ncmax);
+        maxcol = std::max(maxcol, P->Y->ncolour);
+    }
+    std::vector<int4> items;
+    std::vector<int> need;
+    B.colour_off.assign(1, 0);
+    for (int c = 0; c < maxcol; ++c) {
+        for (size_t k = 0; k < B.problems.size(); ++k) {
+            const bmi_plan *P = Bt->plans[B.problems[k]];
+            if (c >= P->Y->ncolour) continue;
+            const int lower = P->Y->colour_off[c];  // patches of colours < c
+            const std::vector<int> &qs = P->Y->by_colour[c];
+            for (int pan = 0; pan < P->X->npanel; ++pan)
+                for (int q : qs) {
+                    items.push_back(make_int4((int)k, pan, q, pbase[k] + pan));
+                    need.push_back(lower);
+                }
+        }
+        B.colour_off.push_back((int)items.size());
+    }
+    B.nitems = (int)items.size();
+    hipError_t e;
+    if ((e = B.d_args.upload(args)) != hipSuccess || (e = B.d_items.upload(items)) != hipSuccess ||
+        (e = B.d_need.upload(need)) != hipSuccess || (e = B.d_state.alloc(2 + (size_t)B.npanels)) != hipSuccess)
+        return fail(BMI_ENOMEM, std::string("batch buffers: ") + hipGetErrorString(e));
+    return BMI_OK;
+}
+
+// Measures the candidate schedules of a bucket (the prologues have run) and
+// keeps the fastest; a bucket signature seen before reuses its choice.
+bmi_status tune_bucket(bmi_batch *Bt, BatchBucket &B) {
+    const std::string sig = bucket_signature(B);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (auto &kv : g_tune_cache)
+            if (kv.first == sig) {
+                B.mode = kv.second.first;
+                B.wpc = kv.second.second;
+                return BMI_OK;
+            }
+    }
+    hipStream_t s = Bt->ctx->stream;
+    const int maxwpc = (B.op_kind == OPK_DL && !B.mixed) ? 2 : 3;
+    std::vector<std::pair<int, int>> cand = {{0, 0}};
+    for (int w = 1; w <= maxwpc; ++w) cand.push_back({1, w});
+    double best = 1e300;
+    for (auto &c : cand) {
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            BMI_HIP(hipEventRecord(Bt->ev0, s));
+            bmi_status st = enqueue_bucket(B, c.first, c.second, Bt->cus, s);
+            if (st != BMI_OK) return st;
+            BMI_HIP(hipEventRecord(Bt->ev1, s));
+            BMI_HIP(hipStreamSynchronize(s));
+            if (c.first == 1 && (st = check_queue_error(B, s)) != BMI_OK) return st;
+            BMI_HIP(hipEventElapsedTime(&ms, Bt->ev0, Bt->ev1));
+        }
+        if (ms < best) {
+            best = ms;
+            B.mode = c.first;
+            B.wpc = c.second;
+            B.tuned_ms = ms;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache.push_back({sig, {B.mode, B.wpc}});
+    return BMI_OK;
+}
+
+bmi_status capture_batch(bmi_batch *Bt) {
+    hipStream_t s = Bt->ctx->stream;
+    BMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    bmi_status st = BMI_OK;
+    auto body = [&]() -> bmi_status {
+        BMI_HIP(hipEventRecord(Bt->fork, s));
+        for (int k = 0; k < bmi_batch::kSide; ++k) BMI_HIP(hipStreamWaitEvent(Bt->side[k], Bt->fork, 0));
+        for (size_t i = 0; i < Bt->plans.size(); ++i) {
+            bmi_status r = enqueue_prologue(Bt->plans[i], Bt->mats[i], Bt->side[i % bmi_batch::kSide], nullptr);
+            if (r != BMI_OK) return r;
+        }
+        for (int k = 0; k < bmi_batch::kSide; ++k) {
+            BMI_HIP(hipEventRecord(Bt->join[k], Bt->side[k]));
+            BMI_HIP(hipStreamWaitEvent(s, Bt->join[k], 0));
+        }
+        for (auto &B : Bt->buckets) {
+            bmi_status r = enqueue_bucket(B, B.mode, B.wpc, Bt->cus, s);
+            if (r != BMI_OK) return r;
+        }
+        for (int p : Bt->singles) {
+            int launches = 0;
+            bmi_status r = enqueue_far(Bt->plans[p], Bt->mats[p], s, &launches);
+            if (r != BMI_OK) return r;
+        }
+        return BMI_OK;
+    };
+    st = body();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (st != BMI_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return fail(BMI_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    Bt->graph = g;
+    BMI_HIP(hipGraphInstantiate(&Bt->exec, g, nullptr, nullptr, 0));
+    return BMI_OK;
+}
+
+void batch_destroy(bmi_batch *Bt) {
+    if (Bt->exec) (void)hipGraphExecDestroy(Bt->exec);
+    if (Bt->graph) (void)hipGraphDestroy(Bt->graph);
+    for (int k = 0; k < bmi_batch::kSide; ++k) {
+        if (Bt->side[k]) (void)hipStreamDestroy(Bt->side[k]);
+        if (Bt->join[k]) (void)hipEventDestroy(Bt->join[k]);
+    }
+    if (Bt->fork) (void)hipEventDestroy(Bt->fork);
+    if (Bt->ev0) (void)hipEventDestroy(Bt->ev0);
+    if (Bt->ev1) (void)hipEventDestroy(Bt->ev1);
+    delete Bt;
+}
+
+}  // namespace
+
+extern "C" {
+
+bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix *const *matrices, int32_t n,
+                            bmi_batch **out) {
+    if (!ctx || !plans || !matrices || !out || n < 1) return fail(BMI_EINVAL, "bad argument");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!plans[i] || !matrices[i]) return fail(BMI_EINVAL, "NULL plan or matrix");
+        if (plans[i]->ctx != ctx) return fail(BMI_EINVAL, "plans must belong to the batch's context");
+        const bmi_status st = check_target(plans[i], matrices[i]);
+        if (st != BMI_OK) return st;
+        for (int j = 0; j < i; ++j)
+            if (plans[j] == plans[i] || matrices[j] == matrices[i])
+                return fail(BMI_EINVAL, "a plan or matrix appears twice in the batch");
+    }
+    BMI_HIP(hipSetDevice(ctx->device));
+    WorkStream wsc(ctx->stream);
+    std::unique_ptr<bmi_batch, void (*)(bmi_batch *)> Bt(new bmi_batch, batch_destroy);
+    Bt->ctx = ctx;
+    Bt->plans.assign(plans, plans + n);
+    Bt->mats.assign(matrices, matrices + n);
+    BMI_HIP(hipDeviceGetAttribute(&Bt->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    for (int k = 0; k < bmi_batch::kSide; ++k) {
+        BMI_HIP(hipStreamCreateWithFlags(&Bt->side[k], hipStreamNonBlocking));
+        BMI_HIP(hipEventCreateWithFlags(&Bt->join[k], hipEventDisableTiming));
+    }
+    BMI_HIP(hipEventCreateWithFlags(&Bt->fork, hipEventDisableTiming));
+    BMI_HIP(hipEventCreate(&Bt->ev0));
+    BMI_HIP(hipEventCreate(&Bt->ev1));
+    // buckets by far-pair kernel
+    for (int i = 0; i < n; ++i) {
+        const bmi_plan *P = plans[i];
+        const PanelArgs a = panel_args(P, matrices[i]);
+        if (!a.rr) {
+            Bt->singles.push_back(i);
+            continue;
+        }
+        const int mixed = P->opts.precision == BMI_PREC_MIXED, c32 = matrices[i]->dtype == BMI_C32;
+        BatchBucket *B = nullptr;
+        for (auto &b : Bt->buckets)
+            if (b.op_kind == P->op_kind && b.mixed == mixed && b.fast == a.fast && b.c32 == c32) B = &b;
+        if (!B) {
+            if ((int)Bt->buckets.size() == BMI_BATCH_MAX_BUCKETS) {
+                Bt->singles.push_back(i);
+                continue;
+            }
+            Bt->buckets.emplace_back();
+            B = &Bt->buckets.back();
+            B->op_kind = P->op_kind;
+            B->mixed = mixed;
+            B->fast = a.fast;
+            B->c32 = c32;
+        }
+        B->problems.push_back(i);
+    }
+    for (auto &B : Bt->buckets) {
+        const bmi_status st = build_bucket(Bt.get(), B);
+        if (st != BMI_OK) return st;
+    }
+    // prologues once (the far kernels read their special pairs), then tune
+    hipStream_t s = ctx->stream;
+    for (int i = 0; i < n; ++i) {
+        const bmi_status st = enqueue_prologue(plans[i], matrices[i], s, nullptr);
+        if (st != BMI_OK) return st;
+    }
+    BMI_HIP(hipStreamSynchronize(s));
+    for (auto &B : Bt->buckets) {
+        const bmi_status st = tune_bucket(Bt.get(), B);
+        if (st != BMI_OK) return st;
+    }
+    const bmi_status st = capture_batch(Bt.get());
+    if (st != BMI_OK) return st;
+    *out = Bt.release();
+    return BMI_OK;
+}
+
+bmi_status bmi_batch_assemble(bmi_batch *Bt) {
+    if (!Bt) return fail(BMI_EINVAL, "NULL argument");
+    BMI_HIP(hipSetDevice(Bt->ctx->device));
+    hipStream_t s = Bt->ctx->stream;
+    BMI_HIP(hipEventRecord(Bt->ev0, s));
+    BMI_HIP(hipGraphLaunch(Bt->exec, s));
+    BMI_HIP(hipEventRecord(Bt->ev1, s));
+    BMI_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    BMI_HIP(hipEventElapsedTime(&ms, Bt->ev0, Bt->ev1));
+    Bt->last_ms = ms;
+    for (auto &B : Bt->buckets)
+        if (B.mode == 1) {
+            const bmi_status st = check_queue_error(B, s);
+            if (st != BMI_OK) return st;
+        }
+    return BMI_OK;
+}
+
+bmi_status bmi_batch_stats_get(const bmi_batch *Bt, bmi_batch_stats *st) {
+    if (!Bt || !st) return fail(BMI_EINVAL, "NULL argument");
+    *st = bmi_batch_stats{};
+    st->last_ms = Bt->last_ms;
+    st->problems = (int32_t)Bt->plans.size();
+    st->buckets = (int32_t)Bt->buckets.size();
+    st->singles = (int32_t)Bt->singles.size();
+    for (size_t b = 0; b < Bt->buckets.size(); ++b) {
+        st->bucket_problems[b] = (int32_t)Bt->buckets[b].problems.size();
+        st->bucket_mode[b] = Bt->buckets[b].mode;
+        st->bucket_wgs_per_cu[b] = Bt->buckets[b].wpc;
+        st->bucket_tuned_ms[b] = Bt->buckets[b].tuned_ms;
+    }
+    return BMI_OK;
+}
+
+bmi_status bmi_batch_free(bmi_batch *Bt) {
+    if (Bt) {
+        (void)hipSetDevice(Bt->ctx->device);
+        batch_destroy(Bt);
+    }
     return BMI_OK;
 }
 

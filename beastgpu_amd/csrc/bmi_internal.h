@@ -151,6 +151,14 @@ hipError_t launch_classify(const ClassifyArgs &a, int pass, hipStream_t s);
 hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *total, hipStream_t s);
 hipError_t launch_special(const SpecialArgs &a, int capacity, int mixed, int op_kind, hipStream_t s);
 hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hipStream_t s);
+// batched far-pair launches over items (problem, panel, patch, global panel)
+hipError_t launch_panels_items(const PanelArgs *args, const int4 *items, int nitems, int ncm, int op_kind, int mixed,
+                               int fast, int c32, hipStream_t s);
+hipError_t launch_panels_queue(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm,
+                               int *head, int *done, int *err, int spin_limit, int grid, int op_kind, int mixed,
+                               int fast, int c32, hipStream_t s);
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 int scan_partials_needed(int n);
 hipError_t launch_apply(const void *Z, int c32, long long ld, int rows, int cols, const double2 *x, int nrhs,
                         double2 *partial, double2 *y, hipStream_t s);

@@ -1385,35 +1385,44 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
     }
 }
 
-// Variant of k_panels without the LDS pair buffer and the gather: each
-// thread adds its pair's 9 values straight into an LDS tile of the
-// work-group's 64 rows x ncm columns.  Two pairs add to the same entry only if
-// their test cells share the entry's row and their trial cells share its
-// column; giving every slot its rank among the slots of its group with the
-// same row (column), the additions go in rounds (rank_t, rank_b) with a
-// barrier between rounds, so no entry is touched twice inside a round and
-// the summation order is fixed.  RWG functions have at most two cells, so
+// LDS of the rank-round kernels besides the tile.
+struct PanelShared {
+    double sg[2][BMI_GEO];             // test sub-batch geometry, double-buffered
+    double bg[BMI_GEO];                // trial patch geometry
+    int s_tslot[2][BMI_MAX_ROWS];
+    double s_tscf[2][BMI_MAX_ROWS];
+    int s_cfn[BMI_MAX_ROWS];
+    unsigned char s_cfirst[BMI_MAX_ROWS];
+    double2 s_sc[64];                  // sincos table (FP64 path)
+};
+
+// One (test panel P, trial patch q) block of the far-pair assembly with its
+// write-back; the caller has filled sh.s_sc (FP64 path) and passes the tile
+// of BMI_PANEL_ROWS x ncm entries.  Ends after the write-back stores are
+// issued; a caller that reuses the LDS for another block synchronises first.
+//
+// Rank rounds: each thread adds its pair's 9 values straight into the LDS
+// tile of the work-group's 64 rows x ncm columns.  Two pairs add to the same
+// entry only if their test cells share the entry's row and their trial cells
+// share its column; giving every slot its rank among the slots of its group
+// with the same row (column), the additions go in rounds (rank_t, rank_b)
+// with a barrier between rounds, so no entry is touched twice inside a round
+// and the summation order is fixed.  RWG functions have at most two cells, so
 // there are at most 2 x 2 rounds.
 template <int OP, int MIXED, int FAST, typename OutT>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_rr(PanelArgs a) {
-    extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x ncm
-    __shared__ double sg[2][BMI_GEO];             // test sub-batch geometry, double-buffered
-    __shared__ double bg[BMI_GEO];                // trial patch geometry
-    __shared__ int s_tslot[2][BMI_MAX_ROWS];
-    __shared__ double s_tscf[2][BMI_MAX_ROWS];
-    __shared__ int s_cfn[BMI_MAX_ROWS];
-    __shared__ unsigned char s_cfirst[BMI_MAX_ROWS];
-    __shared__ double2 s_sc[64];
-
+__device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, int ncm, double2 *tile,
+                                            PanelShared &sh) {
+    auto &sg = sh.sg;
+    auto &bg = sh.bg;
+    auto &s_tslot = sh.s_tslot;
+    auto &s_tscf = sh.s_tscf;
+    auto &s_cfn = sh.s_cfn;
+    auto &s_cfirst = sh.s_cfirst;
+    const double2 *s_sc = sh.s_sc;
     const int t = threadIdx.x;
-    const int w = blockIdx.x;
-    const int P = w / a.nbq;
-    const int q = a.bq_list[w - P * a.nbq];
     const int r0 = P * BMI_PANEL_ROWS;
     const int nr = min(BMI_PANEL_ROWS, a.nrows - r0);
     const int nc = a.bnc[q];
-    const int ncm = a.ncm;
-    if (!MIXED) fill_sincos_table(s_sc);
     const int kl = t >> 4, ll = t & 15;
     const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
 
@@ -1540,6 +1549,77 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
     }
 }
 
+// One work-group per (test panel P, trial patch q of this launch's colour).
+template <int OP, int MIXED, int FAST, typename OutT>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_rr(PanelArgs a) {
+    extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x ncm
+    __shared__ PanelShared sh;
+    if (!MIXED) fill_sincos_table(sh.s_sc);
+    const int w = blockIdx.x;
+    const int P = w / a.nbq;
+    const int q = a.bq_list[w - P * a.nbq];
+    panel_block<OP, MIXED, FAST, OutT>(a, P, q, a.ncm, tile, sh);
+}
+
+// Batched, one colour at a time: work-group i takes item i = (problem, panel,
+// patch, -) of the launch's colour, across the problems of a batch.
+template <int OP, int MIXED, int FAST, typename OutT>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_items(
+    const PanelArgs *args, const int4 *items, int ncm) {
+    extern __shared__ double2 tile[];
+    __shared__ PanelShared sh;
+    if (!MIXED) fill_sincos_table(sh.s_sc);
+    const int4 it = items[blockIdx.x];
+    panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, ncm, tile, sh);
+}
+
+// Batched, persistent: the work-groups pop items (problem, panel, patch,
+// global panel id) from one queue ordered colour by colour.  An item whose
+// panel has blocks of lower colours waits until those are done (their columns
+// are stored first and then read-modify-written): need[i] blocks of its panel
+// must have been counted in done[global panel id].  Every such block was
+// popped earlier by a running work-group that waits only on blocks popped
+// before it, so the queue always drains; the wait is bounded by spin_limit
+// all the same and a timeout sets *err instead of hanging.
+template <int OP, int MIXED, int FAST, typename OutT>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_queue(
+    const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm, int *head, int *done, int *err,
+    int spin_limit) {
+    extern __shared__ double2 tile[];
+    __shared__ PanelShared sh;
+    __shared__ int s_item;
+    const int t = threadIdx.x;
+    if (!MIXED) fill_sincos_table(sh.s_sc);
+    for (;;) {
+        __syncthreads();  // the previous block is done with the LDS and s_item
+        if (t == 0) s_item = atomicAdd(head, 1);
+        __syncthreads();
+        const int i = s_item;
+        if (i >= nitems) break;
+        const int4 it = items[i];
+        const int nd = need[i];
+        if (nd > 0) {
+            if (t == 0) {
+                int spins = 0;
+                while (__hip_atomic_load(&done[it.w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < nd) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++spins > spin_limit) {
+                        atomicOr(err, 1);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, ncm, tile, sh);
+        // publish this block's stores before counting it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (t == 0) __hip_atomic_fetch_add(&done[it.w], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 template <int OP, int MIXED, int FAST>
 static void launch_panels_t(const PanelArgs &a, int c32, hipStream_t s) {
     const unsigned grid = (unsigned)((long long)a.npanel * a.nbq);
@@ -1575,6 +1655,102 @@ hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hi
     } else {
         if (mixed) launch_panels_m<OPK_HH3D, 1>(a, c32, s);
         else launch_panels_m<OPK_HH3D, 0>(a, c32, s);
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ batched launches
+
+namespace {
+template <int OP, int MIXED, int FAST, typename OutT>
+void items_t(const PanelArgs *args, const int4 *items, int nitems, int ncm, hipStream_t s) {
+    const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * ncm;
+    hipLaunchKernelGGL((k_panels_items<OP, MIXED, FAST, OutT>), dim3(nitems), dim3(BMI_BLOCK_THREADS), lds, s, args,
+                       items, ncm);
+}
+
+template <int OP, int MIXED, int FAST, typename OutT>
+void queue_t(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm, int *head, int *done,
+             int *err, int spin_limit, int grid, hipStream_t s) {
+    const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * ncm;
+    hipLaunchKernelGGL((k_panels_queue<OP, MIXED, FAST, OutT>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, args,
+                       items, need, nitems, ncm, head, done, err, spin_limit);
+}
+
+// f(op, mixed, fast, c32) -> the instantiation
+template <template <int, int, int, typename> class K, typename... A>
+void dispatch4(int op_kind, int mixed, int fast, int c32, A... args) {
+#define BMI_D(O, M, F)                                                                   \
+    if (op_kind == O && mixed == M && fast == F) {                                       \
+        if (c32) K<O, M, F, float2>::run(args...);                                       \
+        else K<O, M, F, double2>::run(args...);                                          \
+        return;                                                                          \
+    }
+    BMI_D(OPK_MW, 0, 0) BMI_D(OPK_MW, 0, 1) BMI_D(OPK_MW, 1, 0) BMI_D(OPK_MW, 1, 1)
+    BMI_D(OPK_DL, 0, 0) BMI_D(OPK_DL, 0, 1) BMI_D(OPK_DL, 1, 0) BMI_D(OPK_DL, 1, 1)
+    BMI_D(OPK_HH3D, 0, 0) BMI_D(OPK_HH3D, 0, 1) BMI_D(OPK_HH3D, 1, 0) BMI_D(OPK_HH3D, 1, 1)
+#undef BMI_D
+}
+
+template <int OP, int MIXED, int FAST, typename OutT>
+struct ItemsK {
+    static void run(const PanelArgs *args, const int4 *items, int nitems, int ncm, hipStream_t s) {
+        items_t<OP, MIXED, FAST, OutT>(args, items, nitems, ncm, s);
+    }
+};
+
+template <int OP, int MIXED, int FAST, typename OutT>
+struct QueueK {
+    static void run(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm, int *head,
+                    int *done, int *err, int spin_limit, int grid, hipStream_t s) {
+        queue_t<OP, MIXED, FAST, OutT>(args, items, need, nitems, ncm, head, done, err, spin_limit, grid, s);
+    }
+};
+
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_fill_zero(uint4 *p, long long n16) {
+    for (long long i = (long long)blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x; i < n16;
+         i += (long long)gridDim.x * BMI_BLOCK_THREADS)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_zero_ints(int *p, int n) {
+    for (int i = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x; i < n; i += gridDim.x * BMI_BLOCK_THREADS) p[i] = 0;
+}
+}  // namespace
+
+hipError_t launch_panels_items(const PanelArgs *args, const int4 *items, int nitems, int ncm, int op_kind, int mixed,
+                               int fast, int c32, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    dispatch4<ItemsK>(op_kind, mixed, fast, c32, args, items, nitems, ncm, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_panels_queue(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm,
+                               int *head, int *done, int *err, int spin_limit, int grid, int op_kind, int mixed,
+                               int fast, int c32, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    dispatch4<QueueK>(op_kind, mixed, fast, c32, args, items, need, nitems, ncm, head, done, err, spin_limit, grid, s);
+    return hipGetLastError();
+}
+
+// p must be 16-byte aligned and bytes a multiple of 8 (matrices are).
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
+    const long long n16 = (long long)(bytes / 16);
+    if (n16 > 0) {
+        const long long g = std::min<long long>((n16 + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, 65536);
+        hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)g), dim3(BMI_BLOCK_THREADS), 0, s, (uint4 *)p, n16);
+    }
+    if (bytes % 16) {  // an 8-byte tail
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(BMI_BLOCK_THREADS), 0, s, (int *)((char *)p + n16 * 16),
+                           (int)((bytes % 16) / 4));
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
+    if (n > 0) {
+        const int g = std::min((n + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, 4096);
+        hipLaunchKernelGGL(k_zero_ints, dim3(g), dim3(BMI_BLOCK_THREADS), 0, s, p, n);
     }
     return hipGetLastError();
 }

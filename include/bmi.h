@@ -172,6 +172,44 @@ bmi_status bmi_matrix_apply(const bmi_matrix *m, const double *x, double *y, int
  * Replaces test(M, M_ref) of test/difference.jl:1-6, which compared one column. */
 bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double out[3]);
 
+/* --------------------------------------------------------------- batching
+ * Many assemblies (plans with their output matrices, one context) run as one
+ * replayed hipGraph: the classification and special-pair launches of the
+ * problems on parallel graph branches, then, per bucket of problems that
+ * share a far-pair kernel (operator, precision, rule, output type), one
+ * persistent launch whose work-groups pop (problem, test panel, trial patch)
+ * blocks from a device queue, or one launch per trial colour over every
+ * problem's blocks -- whichever the autotuner at creation measured faster
+ * (persistent launches with 1, 2 or 3 work-groups per CU, or per-colour
+ * launches; results are cached per bucket signature for the process).  The
+ * matrices are bitwise those of bmi_assemble on each plan.
+ * No counterpart in the reference (its batching precedent is the producer /
+ * Channel tiling of GPU_scheduler.jl:160-206 and the hand sweeps of
+ * testing/determin_amount_of_threads.jl:44-75). */
+typedef struct bmi_batch bmi_batch;
+
+#define BMI_BATCH_MAX_BUCKETS 8
+
+typedef struct {
+    double last_ms;               /* device time of the last bmi_batch_assemble (events around the graph) */
+    int32_t problems;
+    int32_t buckets;
+    int32_t singles;              /* problems assembled with their own per-colour launches (no bucket) */
+    int32_t bucket_problems[BMI_BATCH_MAX_BUCKETS];
+    int32_t bucket_mode[BMI_BATCH_MAX_BUCKETS];        /* 0: one launch per colour, 1: persistent queue */
+    int32_t bucket_wgs_per_cu[BMI_BATCH_MAX_BUCKETS];  /* persistent queue: work-groups per CU */
+    double bucket_tuned_ms[BMI_BATCH_MAX_BUCKETS];     /* far phase of the chosen schedule when tuned */
+} bmi_batch_stats;
+
+/* plans[i] assembles into matrices[i]; every object on ctx.  Runs the
+ * prologues once, tunes, captures and instantiates the graph. */
+bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix *const *matrices, int32_t n,
+                            bmi_batch **out);
+/* Replays the graph; blocking. */
+bmi_status bmi_batch_assemble(bmi_batch *batch);
+bmi_status bmi_batch_stats_get(const bmi_batch *batch, bmi_batch_stats *stats);
+bmi_status bmi_batch_free(bmi_batch *batch);
+
 /* Free and total device memory of the context's device (bytes).
  * No counterpart in the reference; callers that size work by what fits in
  * HBM (admission control, the full-size tests) use it. */
