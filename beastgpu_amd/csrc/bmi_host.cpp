@@ -303,6 +303,12 @@ struct bmi_plan {
     DevBuf<double2> d_spec_val;
     SpecialArgs sargs{};
     ClassifyArgs cargs{};
+    // device copies of the arguments and one-problem prefix tables of the
+    // classify / special kernels (which also take whole batches)
+    DevBuf<ClassifyArgs> d_cargs;
+    DevBuf<SpecialArgs> d_sargs;
+    DevBuf<int> d_gbase, d_cbase, d_ebase;
+    int cs = 256, nchunks = 0;
     int64_t pairs[6] = {0, 0, 0, 0, 0, 0};
     bool pairs_ready = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -841,12 +847,13 @@ bmi_status check_target(const bmi_plan *P, const bmi_matrix *out) {
 // after the classification.
 bmi_status enqueue_prologue(bmi_plan *P, bmi_matrix *out, hipStream_t s, hipEvent_t ev_mid) {
     const int nbk = (int)P->nblocks;
-    BMI_HIP(launch_classify(P->cargs, 0, s));
+    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, 0, s));
     BMI_HIP(launch_scan(P->d_blk_count.p, P->d_blk_off.p, nbk, P->d_partials.p, P->d_total.p, s));
-    BMI_HIP(launch_classify(P->cargs, 1, s));
+    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, 1, s));
     if (ev_mid) BMI_HIP(hipEventRecord(ev_mid, s));
     const int mixed = P->opts.precision == BMI_PREC_MIXED;
-    BMI_HIP(launch_special(P->sargs, P->capacity, mixed, P->op_kind, s));
+    BMI_HIP(launch_special(P->d_sargs.p, P->d_cbase.p, P->d_ebase.p, 1, P->nchunks, P->capacity, P->cs, mixed,
+                           P->op_kind, s));
     // functions without local shapes keep zero rows/columns (a fill kernel:
     // memset nodes misbehave in replayed graphs)
     if (P->X->uncovered || P->Y->uncovered)
@@ -1178,7 +1185,13 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
 
     // first classification: size the special-pair buffers
     hipStream_t s = ctx->stream;
-    BMI_HIP(launch_classify(C, 0, s));
+    {
+        const std::vector<int> gb = {0, P->npa};
+        const std::vector<ClassifyArgs> ca = {C};
+        if ((e = P->d_gbase.upload(gb)) != hipSuccess || (e = P->d_cargs.upload(ca)) != hipSuccess)
+            return fail(BMI_EDEVICE, std::string("plan arguments: ") + hipGetErrorString(e));
+    }
+    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, 0, s));
     BMI_HIP(launch_scan(P->d_blk_count.p, P->d_blk_off.p, nbk, P->d_partials.p, P->d_total.p, s));
     int total = 0;
     BMI_HIP(hipMemcpyAsync(&total, P->d_total.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1194,7 +1207,12 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     C.spec_cls = P->d_spec_cls.p;
     C.spec_loc = P->d_spec_loc.p;
     C.capacity = total;
-    BMI_HIP(launch_classify(C, 1, s));
+    {
+        const std::vector<ClassifyArgs> ca = {C};
+        if ((e = P->d_cargs.upload(ca)) != hipSuccess)
+            return fail(BMI_EDEVICE, std::string("plan arguments: ") + hipGetErrorString(e));
+    }
+    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, 1, s));
     SpecialArgs &S = P->sargs;
     S.X = C.X;
     S.Y = C.Y;
@@ -1225,6 +1243,15 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
                 ++n;
             }
         S.near_np = n;
+    }
+    P->cs = special_chunk(P->capacity);
+    P->nchunks = (P->capacity + P->cs - 1) / P->cs;
+    {
+        const std::vector<SpecialArgs> sa = {S};
+        const std::vector<int> cb = {0, P->nchunks}, eb = {0, P->capacity};
+        if ((e = P->d_sargs.upload(sa)) != hipSuccess || (e = P->d_cbase.upload(cb)) != hipSuccess ||
+            (e = P->d_ebase.upload(eb)) != hipSuccess)
+            return fail(BMI_EDEVICE, std::string("plan arguments: ") + hipGetErrorString(e));
     }
     for (int i = 0; i < 4; ++i) BMI_HIP(hipEventCreate(&P->ev[i]));
     *out = P.release();
@@ -1423,10 +1450,24 @@ std::vector<std::pair<std::string, std::pair<int, int>>> g_tune_cache;
 
 }  // namespace
 
+// Problems of a batch with one operator kind: one launch of the special-pair
+// kernels over all of them.
+struct BatchOpGroup {
+    int op_kind = 0, mixed = 0;
+    std::vector<int> problems;
+    int total_chunks = 0, total_entries = 0, cs = 256;
+    DevBuf<SpecialArgs> d_sargs;
+    DevBuf<int> d_cbase, d_ebase;
+};
+
 struct bmi_batch {
     bmi_context *ctx = nullptr;
     std::vector<bmi_plan *> plans;
     std::vector<bmi_matrix *> mats;
+    DevBuf<ClassifyArgs> d_cargs;   // every problem
+    DevBuf<int> d_gbase;
+    int total_groups = 0;
+    std::deque<BatchOpGroup> opgroups;
     std::vector<int> singles;
     std::deque<BatchBucket> buckets;
     static constexpr int kSide = 4;
@@ -1557,21 +1598,94 @@ bmi_status tune_bucket(bmi_batch *Bt, BatchBucket &B) {
     return BMI_OK;
 }
 
+// Classification of every problem (two batched launches around the
+// per-problem scans), the special pairs per operator group, zero fill of
+// uncovered rows: on stream s, the scans spread over the side streams when
+// side is set (graph capture).
+bmi_status enqueue_batch_prologue(bmi_batch *Bt, hipStream_t s, bool side) {
+    const int n = (int)Bt->plans.size();
+    BMI_HIP(launch_classify(Bt->d_cargs.p, Bt->d_gbase.p, n, Bt->total_groups, 0, s));
+    if (side) {
+        BMI_HIP(hipEventRecord(Bt->fork, s));
+        for (int k = 0; k < bmi_batch::kSide; ++k) BMI_HIP(hipStreamWaitEvent(Bt->side[k], Bt->fork, 0));
+    }
+    for (int i = 0; i < n; ++i) {
+        bmi_plan *P = Bt->plans[i];
+        hipStream_t si = side ? Bt->side[i % bmi_batch::kSide] : s;
+        BMI_HIP(launch_scan(P->d_blk_count.p, P->d_blk_off.p, (int)P->nblocks, P->d_partials.p, P->d_total.p, si));
+    }
+    if (side)
+        for (int k = 0; k < bmi_batch::kSide; ++k) {
+            BMI_HIP(hipEventRecord(Bt->join[k], Bt->side[k]));
+            BMI_HIP(hipStreamWaitEvent(s, Bt->join[k], 0));
+        }
+    BMI_HIP(launch_classify(Bt->d_cargs.p, Bt->d_gbase.p, n, Bt->total_groups, 1, s));
+    for (auto &G : Bt->opgroups)
+        BMI_HIP(launch_special(G.d_sargs.p, G.d_cbase.p, G.d_ebase.p, (int)G.problems.size(), G.total_chunks,
+                               G.total_entries, G.cs, G.mixed, G.op_kind, s));
+    for (int i = 0; i < n; ++i) {
+        const bmi_plan *P = Bt->plans[i];
+        const bmi_matrix *M = Bt->mats[i];
+        if (P->X->uncovered || P->Y->uncovered)
+            BMI_HIP(launch_fill_zero(M->d, (size_t)M->rows * M->cols * (M->dtype == BMI_C64 ? 16 : 8), s));
+    }
+    return BMI_OK;
+}
+
+// Device tables of the batched prologue.
+bmi_status build_prologue_tables(bmi_batch *Bt) {
+    const int n = (int)Bt->plans.size();
+    std::vector<ClassifyArgs> ca;
+    std::vector<int> gb = {0};
+    for (int i = 0; i < n; ++i) {
+        ca.push_back(Bt->plans[i]->cargs);
+        gb.push_back(gb.back() + Bt->plans[i]->npa);
+    }
+    Bt->total_groups = gb.back();
+    hipError_t e;
+    if ((e = Bt->d_cargs.upload(ca)) != hipSuccess || (e = Bt->d_gbase.upload(gb)) != hipSuccess)
+        return fail(BMI_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
+    for (int i = 0; i < n; ++i) {
+        const bmi_plan *P = Bt->plans[i];
+        BatchOpGroup *G = nullptr;
+        for (auto &g : Bt->opgroups)
+            if (g.op_kind == P->op_kind) G = &g;
+        if (!G) {
+            Bt->opgroups.emplace_back();
+            G = &Bt->opgroups.back();
+            G->op_kind = P->op_kind;
+        }
+        G->problems.push_back(i);
+        G->mixed |= P->opts.precision == BMI_PREC_MIXED;
+    }
+    for (auto &G : Bt->opgroups) {
+        int cap = 0;
+        for (int i : G.problems) cap += Bt->plans[i]->capacity;
+        G.cs = special_chunk(cap);
+        std::vector<SpecialArgs> sa;
+        std::vector<int> cb = {0}, eb = {0};
+        for (int i : G.problems) {
+            const bmi_plan *P = Bt->plans[i];
+            sa.push_back(P->sargs);
+            cb.push_back(cb.back() + (P->capacity + G.cs - 1) / G.cs);
+            eb.push_back(eb.back() + P->capacity);
+        }
+        G.total_chunks = cb.back();
+        G.total_entries = eb.back();
+        if ((e = G.d_sargs.upload(sa)) != hipSuccess || (e = G.d_cbase.upload(cb)) != hipSuccess ||
+            (e = G.d_ebase.upload(eb)) != hipSuccess)
+            return fail(BMI_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
+    }
+    return BMI_OK;
+}
+
 bmi_status capture_batch(bmi_batch *Bt) {
     hipStream_t s = Bt->ctx->stream;
     BMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     bmi_status st = BMI_OK;
     auto body = [&]() -> bmi_status {
-        BMI_HIP(hipEventRecord(Bt->fork, s));
-        for (int k = 0; k < bmi_batch::kSide; ++k) BMI_HIP(hipStreamWaitEvent(Bt->side[k], Bt->fork, 0));
-        for (size_t i = 0; i < Bt->plans.size(); ++i) {
-            bmi_status r = enqueue_prologue(Bt->plans[i], Bt->mats[i], Bt->side[i % bmi_batch::kSide], nullptr);
-            if (r != BMI_OK) return r;
-        }
-        for (int k = 0; k < bmi_batch::kSide; ++k) {
-            BMI_HIP(hipEventRecord(Bt->join[k], Bt->side[k]));
-            BMI_HIP(hipStreamWaitEvent(s, Bt->join[k], 0));
-        }
+        bmi_status r0 = enqueue_batch_prologue(Bt, s, true);
+        if (r0 != BMI_OK) return r0;
         for (auto &B : Bt->buckets) {
             bmi_status r = enqueue_bucket(B, B.mode, B.wpc, Bt->cus, s);
             if (r != BMI_OK) return r;
@@ -1672,8 +1786,9 @@ bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix
     }
     // prologues once (the far kernels read their special pairs), then tune
     hipStream_t s = ctx->stream;
-    for (int i = 0; i < n; ++i) {
-        const bmi_status st = enqueue_prologue(plans[i], matrices[i], s, nullptr);
+    {
+        bmi_status st = build_prologue_tables(Bt.get());
+        if (st == BMI_OK) st = enqueue_batch_prologue(Bt.get(), s, false);
         if (st != BMI_OK) return st;
     }
     BMI_HIP(hipStreamSynchronize(s));

@@ -147,9 +147,16 @@ struct SpecialArgs {
 };
 
 // kernel launchers (bmi_kernels.hip)
-hipError_t launch_classify(const ClassifyArgs &a, int pass, hipStream_t s);
+// classification, one problem or a batch: work-group g takes sub-batch
+// g - gbase[i] of problem i (gbase: nprob + 1 prefix of the sub-batch counts)
+hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, int pass,
+                           hipStream_t s);
 hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *total, hipStream_t s);
-hipError_t launch_special(const SpecialArgs &a, int capacity, int mixed, int op_kind, hipStream_t s);
+// special pairs, one problem or a batch of one operator: cbase / ebase are
+// prefixes of the problems' chunk counts (chunks of cs entries) and capacities
+hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *ebase, int nprob, int total_chunks,
+                          int total_entries, int cs, int mixed, int op_kind, hipStream_t s);
+int special_chunk(int capacity);
 hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hipStream_t s);
 // batched far-pair launches over items (problem, panel, patch, global panel)
 hipError_t launch_panels_items(const PanelArgs *args, const int4 *items, int nitems, int ncm, int op_kind, int mixed,

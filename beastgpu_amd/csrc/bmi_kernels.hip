@@ -557,6 +557,20 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
     }
 }
 
+// ------------------------------------------------- batched work mapping
+
+// Problem of work unit u (a group, chunk or entry) under a prefix table
+// base[0..n] of the problems' unit counts (base[n] = total).
+__device__ __forceinline__ int find_problem(const int *base, int n, int u) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------- classification
 
 // One work-group per test patch p.  For each trial patch q the bounding
@@ -571,7 +585,10 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
 // path marks far pairs with close centroids as class 5.
 // pass 0 writes the number of special pairs of every block; pass 1 writes the
 // pairs, in thread order, at the block's offset.
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(ClassifyArgs a, int pass) {
+// Work-group g takes sub-batch g - gbase[i] of problem i (one problem, or
+// every problem of a batch).
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyArgs *args, const int *gbase, int nprob,
+                                                                int pass) {
     __shared__ double tv[BMI_PATCH][9], bv[BMI_PATCH][9];
     __shared__ double tc[BMI_PATCH][3], bc[BMI_PATCH][3];
     __shared__ double barea[BMI_PATCH];
@@ -579,7 +596,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(ClassifyArgs a, 
     __shared__ int nearq[BMI_BLOCK_THREADS];
     __shared__ int wsum[BMI_BLOCK_THREADS / 64];
 
-    const int p = blockIdx.x;
+    const int pi = find_problem(gbase, nprob, (int)blockIdx.x);
+    const ClassifyArgs a = args[pi];  // a copy: the kernel's stores cannot alias it
+    const int p = (int)blockIdx.x - gbase[pi];
     const int t = threadIdx.x;
     const int lane = t & 63, wv = t >> 6;
     if (t < BMI_PATCH) tt[t] = a.X.patch_tri[p * BMI_PATCH + t];
@@ -781,8 +800,11 @@ hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *to
     return hipGetLastError();
 }
 
-hipError_t launch_classify(const ClassifyArgs &a, int pass, hipStream_t s) {
-    hipLaunchKernelGGL(k_classify, dim3((unsigned)a.npa), dim3(BMI_BLOCK_THREADS), 0, s, a, pass);
+hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, int pass,
+                           hipStream_t s) {
+    if (total_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)total_groups), dim3(BMI_BLOCK_THREADS), 0, s, args, gbase, nprob,
+                       pass);
     return hipGetLastError();
 }
 
@@ -917,7 +939,11 @@ __device__ __forceinline__ void ss_map(int cls, int r, double e1, double e2, dou
 // over the hypercube nodes, one Green evaluation each; a butterfly over the
 // wave reduces the accumulators.
 template <int OP>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_special(SpecialArgs a) {
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_special(const SpecialArgs *args,
+                                                                                    const int *cbase, int nprob,
+                                                                                    int cs) {
+    const int pi = find_problem(cbase, nprob, (int)blockIdx.x);
+    const SpecialArgs &a = args[pi];
     __shared__ double s_glx[3][BMI_MAX_GL], s_glw[3][BMI_MAX_GL];
     __shared__ double s_nb1[BMI_MAX_NEAR], s_nb2[BMI_MAX_NEAR], s_nw[BMI_MAX_NEAR];
     __shared__ double2 s_sc[64];
@@ -938,12 +964,13 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
     const int lane = t & 63, wv = t >> 6;
     const int ns = min(*a.n_special, a.capacity);
     const bool lossy = a.op.lossy != 0;
-    // chunks of 256 entries: the threads pick out the touching and near
-    // entries this group owns, then each wave integrates every fourth of them
-    for (int c0 = blockIdx.x * BMI_BLOCK_THREADS; c0 < ns; c0 += gridDim.x * BMI_BLOCK_THREADS) {
+    // one chunk of cs (<= 256) entries: the threads pick out the touching and
+    // near entries the problem owns, then each wave integrates every fourth
+    {
+        const int c0 = ((int)blockIdx.x - cbase[pi]) * cs;
         const int ec = c0 + t;
         int take = 0;
-        if (ec < ns) {
+        if (t < cs && ec < ns) {
             const int cl = a.spec_cls[ec];
             take = !(cl & SPEC_NOT_OWNER) && cl >= CLS_CV && cl <= CLS_NEAR;
         }
@@ -1118,12 +1145,17 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
 }
 
 // One thread per far pair that the mixed path keeps in FP64 (class 5), owner
-// entries only.
+// entries only; thread u takes entry u - ebase[i] of problem i.
 template <int OP>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(SpecialArgs a) {
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *args, const int *ebase, int nprob) {
+    const int u = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
+    if (u >= ebase[nprob]) return;
+    const int pi = find_problem(ebase, nprob, u);
+    const SpecialArgs &a = args[pi];
     const int ns = min(*a.n_special, a.capacity);
-    for (int e = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x; e < ns; e += gridDim.x * BMI_BLOCK_THREADS) {
-        if (a.spec_cls[e] != CLS_FAR64) continue;  // owner entries of class 5 only
+    {
+        const int e = u - ebase[pi];
+        if (e >= ns || a.spec_cls[e] != CLS_FAR64) return;  // owner entries of class 5 only
         const int K = a.spec_K[e], L = a.spec_L[e];
         double xp[3 * BMI_MAX_RULE], yp[3 * BMI_MAX_RULE], xa[9], ya[9], xc[3], yc[3];
 #pragma unroll
@@ -1152,10 +1184,15 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(SpecialArgs a) {
 // in the block (home sub-batch of the test cell, patch of the trial cell);
 // copy its values.  A missing owner (which the classifier rules out: both
 // blocks classify the same pair with the same arithmetic) leaves NaN.
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(SpecialArgs a) {
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *args, const int *ebase, int nprob) {
+    const int u = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
+    if (u >= ebase[nprob]) return;
+    const int pi = find_problem(ebase, nprob, u);
+    const SpecialArgs &a = args[pi];
     const int ns = min(*a.n_special, a.capacity);
-    for (int e = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x; e < ns; e += gridDim.x * BMI_BLOCK_THREADS) {
-        if (!(a.spec_cls[e] & SPEC_NOT_OWNER)) continue;
+    {
+        const int e = u - ebase[pi];
+        if (e >= ns || !(a.spec_cls[e] & SPEC_NOT_OWNER)) return;
         const int K = a.spec_K[e], L = a.spec_L[e];
         const long long b = (long long)a.X.home[K] * a.npb + a.Y.home[L];
         const int o0 = a.blk_off[b], o1 = min(a.blk_off[b + 1], a.capacity);
@@ -1173,20 +1210,29 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(SpecialArgs a) {
     }
 }
 
-hipError_t launch_special(const SpecialArgs &a, int capacity, int mixed, int op_kind, hipStream_t s) {
-    if (capacity <= 0) return hipSuccess;
-    const int grid = std::min((capacity + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, 4096);
-    if (op_kind == OPK_MW) hipLaunchKernelGGL(k_special<OPK_MW>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
-    else if (op_kind == OPK_DL) hipLaunchKernelGGL(k_special<OPK_DL>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
-    else hipLaunchKernelGGL(k_special<OPK_HH3D>, dim3(grid), dim3(BMI_BLOCK_THREADS), 0, s, a);
-    const int g2 = std::min((capacity + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, 2048);
+hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *ebase, int nprob, int total_chunks,
+                          int total_entries, int cs, int mixed, int op_kind, hipStream_t s) {
+    if (total_entries <= 0) return hipSuccess;
+    const dim3 b(BMI_BLOCK_THREADS);
+    if (op_kind == OPK_MW) hipLaunchKernelGGL(k_special<OPK_MW>, dim3(total_chunks), b, 0, s, args, cbase, nprob, cs);
+    else if (op_kind == OPK_DL) hipLaunchKernelGGL(k_special<OPK_DL>, dim3(total_chunks), b, 0, s, args, cbase, nprob, cs);
+    else hipLaunchKernelGGL(k_special<OPK_HH3D>, dim3(total_chunks), b, 0, s, args, cbase, nprob, cs);
+    const int g2 = (total_entries + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
     if (mixed) {
-        if (op_kind == OPK_MW) hipLaunchKernelGGL(k_far64<OPK_MW>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
-        else if (op_kind == OPK_DL) hipLaunchKernelGGL(k_far64<OPK_DL>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
-        else hipLaunchKernelGGL(k_far64<OPK_HH3D>, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
+        if (op_kind == OPK_MW) hipLaunchKernelGGL(k_far64<OPK_MW>, dim3(g2), b, 0, s, args, ebase, nprob);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL(k_far64<OPK_DL>, dim3(g2), b, 0, s, args, ebase, nprob);
+        else hipLaunchKernelGGL(k_far64<OPK_HH3D>, dim3(g2), b, 0, s, args, ebase, nprob);
     }
-    hipLaunchKernelGGL(k_link, dim3(g2), dim3(BMI_BLOCK_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_link, dim3(g2), b, 0, s, args, ebase, nprob);
     return hipGetLastError();
+}
+
+// Entries per chunk of k_special: enough chunks to fill the device
+// (>= 2048 work-groups) down to 16 entries per chunk.
+int special_chunk(int capacity) {
+    int cs = 256;
+    while (cs > 16 && (capacity + cs - 1) / cs < 2048) cs >>= 1;
+    return cs;
 }
 
 // ------------------------------------------------------ far-pair blocks

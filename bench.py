@@ -10,7 +10,10 @@ output matrix resident in HBM.
 Also reported on the same mesh: the mixed-precision path (FP32 far pairs,
 ComplexF32 and ComplexF64 outputs) with its relative Frobenius and Z.x errors
 against the FP64 path, the roofline of the far-pair kernel, and the CPU oracle
-timed on a bounded sample of the same workload.
+timed on a bounded sample of the same workload; then the 64-problem batched
+EFIE/MFIE workload and a 400-request sample of the serving stream.  Other
+workloads (--workload): efie-L3, efie-L3-k4, ellipsoid-60k, hh3d-cubeD,
+batched, serving.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -328,6 +331,8 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=400,
                     help="requests of the serving stream in the default run (0: skip); --workload serving runs 2,000")
     ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--no-batched", dest="batched", action="store_false",
+                    help="skip the 64-problem batched sample (--workload batched runs it alone)")
     ap.add_argument("--serving-total", type=int, default=2000, help="requests of --workload serving")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
@@ -439,6 +444,10 @@ def main():
         result["mixed"] = mixed
 
     asm.close()
+    if args.batched and args.workload == "efie-L5":
+        bt = batched_run(3, 1)
+        print(f"[bench] batched {bt}", file=sys.stderr, flush=True)
+        result["batched"] = bt
     if args.serving_requests > 0 and args.workload == "efie-L5":
         sv = serving_run(args.serving_requests, slots=args.serving_slots)
         print(f"[bench] serving {sv}", file=sys.stderr, flush=True)
