@@ -326,6 +326,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="efie-L5")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--to-host", action="store_true", help="also time assembly plus the copy into a host matrix")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serving-requests", type=int, default=400,
@@ -420,6 +421,20 @@ def main():
         },
     }
 
+    if args.to_host:
+        # end to end as the reference's store-on-GPU path: assembly plus the
+        # copy of the whole matrix into a host ComplexF64 array
+        host = np.empty((n, n), dtype=np.complex128, order="F")
+        from beastgpu_amd import _bmi as _b
+        _b.check(_b.lib().bmi_matrix_to_host(asm.matrix.h, host.ctypes.data, _b.C64))  # touch the pages
+        t0 = time.perf_counter()
+        asm.assemble()
+        _b.check(_b.lib().bmi_matrix_to_host(asm.matrix.h, host.ctypes.data, _b.C64))
+        t_e2e = time.perf_counter() - t0
+        result["end_to_end_with_host_copy_s"] = t_e2e
+        result["host_copy_gbs"] = entries * 16 / (t_e2e - t_step) / 1e9
+        print(f"[bench] assembly + copy to a host ComplexF64 matrix: {t_e2e:.3f} s", file=sys.stderr, flush=True)
+        del host
     if not args.no_mixed:
         rng = np.random.default_rng(33)
         x = np.sqrt(rng.uniform(size=n)) * np.exp(2j * np.pi * rng.uniform(size=n))
