@@ -154,8 +154,9 @@ hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob
 hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *total, hipStream_t s);
 // special pairs, one problem or a batch of one operator: cbase / ebase are
 // prefixes of the problems' chunk counts (chunks of cs entries) and capacities
+// ng5: every problem uses 5 Gauss-Legendre points per dimension for CV, CE, CF
 hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *ebase, int nprob, int total_chunks,
-                          int total_entries, int cs, int mixed, int op_kind, hipStream_t s);
+                          int total_entries, int cs, int mixed, int op_kind, int ng5, hipStream_t s);
 int special_chunk(int capacity);
 hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hipStream_t s);
 // batched far-pair launches over items (problem, panel, patch, global panel)

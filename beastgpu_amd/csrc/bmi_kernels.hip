@@ -938,7 +938,9 @@ __device__ __forceinline__ void ss_map(int cls, int r, double e1, double e2, dou
 // in the sub-batch the cell calls home).  Regions in turn, lanes striding
 // over the hypercube nodes, one Green evaluation each; a butterfly over the
 // wave reduces the accumulators.
-template <int OP>
+// NG: Gauss-Legendre points per hypercube dimension fixed at compile time
+// (the default 5; constant-divisor node decoding), 0: read per class.
+template <int OP, int NG>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_special(const SpecialArgs *args,
                                                                                     const int *cbase, int nprob,
                                                                                     int cs) {
@@ -1074,7 +1076,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
             }
         } else {
             const int g = cls - 1;
-            const int n = a.ss_n[g];
+            const int n = NG ? NG : a.ss_n[g];
             const int n2 = n * n, n4 = n2 * n2;
             const int nreg = cls == CLS_CV ? 2 : (cls == CLS_CE ? 5 : 6);
             // region loop outermost: the region map is uniform over the wave
@@ -1211,12 +1213,19 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *a
 }
 
 hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *ebase, int nprob, int total_chunks,
-                          int total_entries, int cs, int mixed, int op_kind, hipStream_t s) {
+                          int total_entries, int cs, int mixed, int op_kind, int ng5, hipStream_t s) {
     if (total_entries <= 0) return hipSuccess;
     const dim3 b(BMI_BLOCK_THREADS);
-    if (op_kind == OPK_MW) hipLaunchKernelGGL(k_special<OPK_MW>, dim3(total_chunks), b, 0, s, args, cbase, nprob, cs);
-    else if (op_kind == OPK_DL) hipLaunchKernelGGL(k_special<OPK_DL>, dim3(total_chunks), b, 0, s, args, cbase, nprob, cs);
-    else hipLaunchKernelGGL(k_special<OPK_HH3D>, dim3(total_chunks), b, 0, s, args, cbase, nprob, cs);
+    const dim3 g(total_chunks);
+    if (ng5) {
+        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 5>), g, b, 0, s, args, cbase, nprob, cs);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 5>), g, b, 0, s, args, cbase, nprob, cs);
+        else hipLaunchKernelGGL((k_special<OPK_HH3D, 5>), g, b, 0, s, args, cbase, nprob, cs);
+    } else {
+        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 0>), g, b, 0, s, args, cbase, nprob, cs);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0>), g, b, 0, s, args, cbase, nprob, cs);
+        else hipLaunchKernelGGL((k_special<OPK_HH3D, 0>), g, b, 0, s, args, cbase, nprob, cs);
+    }
     const int g2 = (total_entries + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
     if (mixed) {
         if (op_kind == OPK_MW) hipLaunchKernelGGL(k_far64<OPK_MW>, dim3(g2), b, 0, s, args, ebase, nprob);

@@ -235,3 +235,25 @@ def test_double_layer_variants(gpu_ctx, variant):
     err = relfro(Z, Zref)
     print("dl rel err", err)
     assert err <= (MIXED_TOL if variant == "mixed" else FP64_TOL)
+
+
+@pytest.mark.parametrize("kind", ["efie", "hh3d", "mfie"])
+def test_nondefault_ss_orders_match_oracle(gpu_ctx, kind):
+    """Sauter-Schwab orders other than 5 run the kernels that read the order
+    at run time."""
+    m = meshes.perturb_radial(meshes.icosphere(2), 12, 0.03)
+    if kind == "efie":
+        op, X = Maxwell3D.singlelayer(wavenumber=1.7), spaces.raviartthomas(m)
+    elif kind == "mfie":
+        op, X = Maxwell3D.doublelayer(wavenumber=1.7), spaces.raviartthomas(m)
+    else:
+        op, X = Helmholtz3D.singlelayer(wavenumber=1.7), spaces.lagrangec0d1(m)
+    orders = (4, 6, 3)
+    print(f"case ss orders {kind}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(ss_order=orders), ctx=gpu_ctx)
+    Z = a.assemble().to_host()
+    a.close()
+    Zref, _ = O.assemble(op, X, X, ss_order=orders)
+    err = relfro(Z, Zref)
+    print("rel err", err)
+    assert err <= FP64_TOL
