@@ -1573,6 +1573,12 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
             e1 = min(pe1, a.capacity);
             if (s + 2 < sb1) fetch(s + 2);
         }
+        // With one test rank (the round-robin sub-batches), different test
+        // cells of a sub-batch own different rows, so two updates of one
+        // entry come from lanes of the same 16-lane group, i.e. of one wave:
+        // the trial-rank rounds need no barrier, as one wave's LDS accesses
+        // are performed in program order.  More test ranks put colliding
+        // rows in different waves: a barrier after each test round.
         for (int ra = 0; ra < a.rounds_t; ++ra) {
             for (int rb = 0; rb < a.rounds_b; ++rb) {
 #pragma unroll
@@ -1588,8 +1594,10 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
                         z.y = fma(cc, loc[3 * i + j].y, z.y);
                     }
                 }
-                __syncthreads();
             }
+            // the next test round; after the last one, the next sub-batch's
+            // records (stashed above) become visible and the tile is complete
+            __syncthreads();
         }
     }
 
