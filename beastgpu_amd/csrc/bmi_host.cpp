@@ -296,7 +296,7 @@ struct bmi_plan {
     DevBuf<double> d_xpts, d_ypts, d_tgeo, d_bgeo;
     int npa = 0, npb = 0;
     long long nblocks = 0;
-    DevBuf<int> d_blk_count, d_blk_off, d_partials, d_total;
+    DevBuf<int> d_blk_cnt, d_blk_off, d_total;  // d_total: the classifier's entry counter
     int capacity = 0;
     DevBuf<int> d_spec_K, d_spec_L;
     DevBuf<unsigned char> d_spec_cls, d_spec_loc;
@@ -851,10 +851,8 @@ int gl5(const bmi_plan *P) {
 // no host synchronisation (graph-capturable).  ev_mid (optional) is recorded
 // after the classification.
 bmi_status enqueue_prologue(bmi_plan *P, bmi_matrix *out, hipStream_t s, hipEvent_t ev_mid) {
-    const int nbk = (int)P->nblocks;
-    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, 0, s));
-    BMI_HIP(launch_scan(P->d_blk_count.p, P->d_blk_off.p, nbk, P->d_partials.p, P->d_total.p, s));
-    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, 1, s));
+    BMI_HIP(launch_zero_ints(P->d_total.p, 1, s));
+    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, s));
     if (ev_mid) BMI_HIP(hipEventRecord(ev_mid, s));
     const int mixed = P->opts.precision == BMI_PREC_MIXED;
     BMI_HIP(launch_special(P->d_sargs.p, P->d_cbase.p, P->d_ebase.p, 1, P->nchunks, P->capacity, P->cs, mixed,
@@ -894,6 +892,7 @@ PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) {
     B.fast = P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
     B.npb = P->npb;
     B.blk_off = P->d_blk_off.p;
+    B.blk_cnt = P->d_blk_cnt.p;
     B.spec_loc = P->d_spec_loc.p;
     B.spec_val = P->d_spec_val.p;
     B.capacity = P->capacity;
@@ -1171,8 +1170,7 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     P->nblocks = (long long)P->npa * P->npb;
     if (P->nblocks >= (1LL << 31) - 1) return fail(BMI_EUNSUPPORTED, "too many patch blocks");
     const int nbk = (int)P->nblocks;
-    if ((e = P->d_blk_count.alloc(nbk)) != hipSuccess || (e = P->d_blk_off.alloc(nbk + 1)) != hipSuccess ||
-        (e = P->d_partials.alloc(scan_partials_needed(nbk) + 1)) != hipSuccess ||
+    if ((e = P->d_blk_cnt.alloc(nbk)) != hipSuccess || (e = P->d_blk_off.alloc(nbk)) != hipSuccess ||
         (e = P->d_total.alloc(1)) != hipSuccess)
         return fail(BMI_ENOMEM, std::string("plan buffers: ") + hipGetErrorString(e));
 
@@ -1185,8 +1183,9 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     const double h = std::max(test->mesh->hmax, trial->mesh->hmax);
     C.near_d2 = o.precision == BMI_PREC_MIXED ? (o.near_factor * h) * (o.near_factor * h) : -1.0;
     C.near_d = o.precision == BMI_PREC_MIXED ? o.near_factor * h : -1.0;
-    C.blk_count = P->d_blk_count.p;
+    C.blk_cnt = P->d_blk_cnt.p;
     C.blk_off = P->d_blk_off.p;
+    C.counter = P->d_total.p;
 
     // first classification: size the special-pair buffers
     hipStream_t s = ctx->stream;
@@ -1196,8 +1195,9 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
         if ((e = P->d_gbase.upload(gb)) != hipSuccess || (e = P->d_cargs.upload(ca)) != hipSuccess)
             return fail(BMI_EDEVICE, std::string("plan arguments: ") + hipGetErrorString(e));
     }
-    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, 0, s));
-    BMI_HIP(launch_scan(P->d_blk_count.p, P->d_blk_off.p, nbk, P->d_partials.p, P->d_total.p, s));
+    // counting run: no entry buffer yet (capacity 0), the counter gives the size
+    BMI_HIP(launch_zero_ints(P->d_total.p, 1, s));
+    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, s));
     int total = 0;
     BMI_HIP(hipMemcpyAsync(&total, P->d_total.p, sizeof(int), hipMemcpyDeviceToHost, s));
     BMI_HIP(hipStreamSynchronize(s));
@@ -1217,7 +1217,8 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
         if ((e = P->d_cargs.upload(ca)) != hipSuccess)
             return fail(BMI_EDEVICE, std::string("plan arguments: ") + hipGetErrorString(e));
     }
-    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, 1, s));
+    BMI_HIP(launch_zero_ints(P->d_total.p, 1, s));
+    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, s));
     SpecialArgs &S = P->sargs;
     S.X = C.X;
     S.Y = C.Y;
@@ -1226,6 +1227,7 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     S.spec_cls = P->d_spec_cls.p;
     S.n_special = P->d_total.p;
     S.blk_off = P->d_blk_off.p;
+    S.blk_cnt = P->d_blk_cnt.p;
     S.npb = P->npb;
     S.capacity = total;
     S.spec_val = P->d_spec_val.p;
@@ -1471,13 +1473,12 @@ struct bmi_batch {
     std::vector<bmi_matrix *> mats;
     DevBuf<ClassifyArgs> d_cargs;   // every problem
     DevBuf<int> d_gbase;
+    DevBuf<int *> d_counters;       // each problem's classifier counter
     int total_groups = 0;
     std::deque<BatchOpGroup> opgroups;
     std::vector<int> singles;
     std::deque<BatchBucket> buckets;
-    static constexpr int kSide = 4;
-    hipStream_t side[kSide] = {};
-    hipEvent_t fork = nullptr, join[kSide] = {}, ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int cus = 0;
@@ -1603,28 +1604,12 @@ bmi_status tune_bucket(bmi_batch *Bt, BatchBucket &B) {
     return BMI_OK;
 }
 
-// Classification of every problem (two batched launches around the
-// per-problem scans), the special pairs per operator group, zero fill of
-// uncovered rows: on stream s, the scans spread over the side streams when
-// side is set (graph capture).
-bmi_status enqueue_batch_prologue(bmi_batch *Bt, hipStream_t s, bool side) {
+// Classification of every problem (one launch), the special pairs per
+// operator group, zero fill of uncovered rows: on stream s.
+bmi_status enqueue_batch_prologue(bmi_batch *Bt, hipStream_t s) {
     const int n = (int)Bt->plans.size();
-    BMI_HIP(launch_classify(Bt->d_cargs.p, Bt->d_gbase.p, n, Bt->total_groups, 0, s));
-    if (side) {
-        BMI_HIP(hipEventRecord(Bt->fork, s));
-        for (int k = 0; k < bmi_batch::kSide; ++k) BMI_HIP(hipStreamWaitEvent(Bt->side[k], Bt->fork, 0));
-    }
-    for (int i = 0; i < n; ++i) {
-        bmi_plan *P = Bt->plans[i];
-        hipStream_t si = side ? Bt->side[i % bmi_batch::kSide] : s;
-        BMI_HIP(launch_scan(P->d_blk_count.p, P->d_blk_off.p, (int)P->nblocks, P->d_partials.p, P->d_total.p, si));
-    }
-    if (side)
-        for (int k = 0; k < bmi_batch::kSide; ++k) {
-            BMI_HIP(hipEventRecord(Bt->join[k], Bt->side[k]));
-            BMI_HIP(hipStreamWaitEvent(s, Bt->join[k], 0));
-        }
-    BMI_HIP(launch_classify(Bt->d_cargs.p, Bt->d_gbase.p, n, Bt->total_groups, 1, s));
+    BMI_HIP(launch_zero_each(Bt->d_counters.p, n, s));
+    BMI_HIP(launch_classify(Bt->d_cargs.p, Bt->d_gbase.p, n, Bt->total_groups, s));
     for (auto &G : Bt->opgroups)
         BMI_HIP(launch_special(G.d_sargs.p, G.d_cbase.p, G.d_ebase.p, (int)G.problems.size(), G.total_chunks,
                                G.total_entries, G.cs, G.mixed, G.op_kind, G.ng5, s));
@@ -1647,8 +1632,11 @@ bmi_status build_prologue_tables(bmi_batch *Bt) {
         gb.push_back(gb.back() + Bt->plans[i]->npa);
     }
     Bt->total_groups = gb.back();
+    std::vector<int *> counters;
+    for (int i = 0; i < n; ++i) counters.push_back(Bt->plans[i]->d_total.p);
     hipError_t e;
-    if ((e = Bt->d_cargs.upload(ca)) != hipSuccess || (e = Bt->d_gbase.upload(gb)) != hipSuccess)
+    if ((e = Bt->d_cargs.upload(ca)) != hipSuccess || (e = Bt->d_gbase.upload(gb)) != hipSuccess ||
+        (e = Bt->d_counters.upload(counters)) != hipSuccess)
         return fail(BMI_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
     for (int i = 0; i < n; ++i) {
         const bmi_plan *P = Bt->plans[i];
@@ -1690,7 +1678,7 @@ bmi_status capture_batch(bmi_batch *Bt) {
     BMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     bmi_status st = BMI_OK;
     auto body = [&]() -> bmi_status {
-        bmi_status r0 = enqueue_batch_prologue(Bt, s, true);
+        bmi_status r0 = enqueue_batch_prologue(Bt, s);
         if (r0 != BMI_OK) return r0;
         for (auto &B : Bt->buckets) {
             bmi_status r = enqueue_bucket(B, B.mode, B.wpc, Bt->cus, s);
@@ -1719,11 +1707,6 @@ bmi_status capture_batch(bmi_batch *Bt) {
 void batch_destroy(bmi_batch *Bt) {
     if (Bt->exec) (void)hipGraphExecDestroy(Bt->exec);
     if (Bt->graph) (void)hipGraphDestroy(Bt->graph);
-    for (int k = 0; k < bmi_batch::kSide; ++k) {
-        if (Bt->side[k]) (void)hipStreamDestroy(Bt->side[k]);
-        if (Bt->join[k]) (void)hipEventDestroy(Bt->join[k]);
-    }
-    if (Bt->fork) (void)hipEventDestroy(Bt->fork);
     if (Bt->ev0) (void)hipEventDestroy(Bt->ev0);
     if (Bt->ev1) (void)hipEventDestroy(Bt->ev1);
     delete Bt;
@@ -1753,11 +1736,6 @@ bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix
     Bt->plans.assign(plans, plans + n);
     Bt->mats.assign(matrices, matrices + n);
     BMI_HIP(hipDeviceGetAttribute(&Bt->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    for (int k = 0; k < bmi_batch::kSide; ++k) {
-        BMI_HIP(hipStreamCreateWithFlags(&Bt->side[k], hipStreamNonBlocking));
-        BMI_HIP(hipEventCreateWithFlags(&Bt->join[k], hipEventDisableTiming));
-    }
-    BMI_HIP(hipEventCreateWithFlags(&Bt->fork, hipEventDisableTiming));
     BMI_HIP(hipEventCreate(&Bt->ev0));
     BMI_HIP(hipEventCreate(&Bt->ev1));
     // buckets by far-pair kernel
@@ -1794,7 +1772,7 @@ bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix
     hipStream_t s = ctx->stream;
     {
         bmi_status st = build_prologue_tables(Bt.get());
-        if (st == BMI_OK) st = enqueue_batch_prologue(Bt.get(), s, false);
+        if (st == BMI_OK) st = enqueue_batch_prologue(Bt.get(), s);
         if (st != BMI_OK) return st;
     }
     BMI_HIP(hipStreamSynchronize(s));

@@ -106,7 +106,8 @@ struct PanelArgs {
     const int *bq_list;          // trial patches of this launch's colour
     int nbq;
     int npb;                     // trial patches in total (block id = sub-batch * npb + q)
-    const int *blk_off;          // special-pair offsets per block
+    const int *blk_off;          // special-pair entries of each block: first ...
+    const int *blk_cnt;          // ... and count
     const unsigned char *spec_loc;
     const double2 *spec_val;
     int capacity;                // special-pair entries allocated
@@ -122,8 +123,9 @@ struct ClassifyArgs {
     double k2x16;        // 16 |gamma|^2
     double near_d2;      // (near_factor * max edge)^2, mixed precision only; < 0 disables
     double near_d;       // near_factor * max edge, mixed precision only; < 0 disables
-    int *blk_count;      // pass 0 output
-    const int *blk_off;  // pass 1 input
+    int *blk_off;        // output: first entry of each block ...
+    int *blk_cnt;        // ... and its number of entries
+    int *counter;        // entries reserved so far (zeroed before the launch)
     int *spec_K, *spec_L;
     unsigned char *spec_cls, *spec_loc;
     int capacity;
@@ -134,7 +136,7 @@ struct SpecialArgs {
     const int *spec_K, *spec_L;
     const unsigned char *spec_cls;
     const int *n_special;
-    const int *blk_off;
+    const int *blk_off, *blk_cnt;
     int npb;
     int capacity;        // special-pair entries allocated
     double2 *spec_val;
@@ -149,9 +151,8 @@ struct SpecialArgs {
 // kernel launchers (bmi_kernels.hip)
 // classification, one problem or a batch: work-group g takes sub-batch
 // g - gbase[i] of problem i (gbase: nprob + 1 prefix of the sub-batch counts)
-hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, int pass,
-                           hipStream_t s);
-hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *total, hipStream_t s);
+hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, hipStream_t s);
+hipError_t launch_zero_each(int *const *ptrs, int n, hipStream_t s);
 // special pairs, one problem or a batch of one operator: cbase / ebase are
 // prefixes of the problems' chunk counts (chunks of cs entries) and capacities
 // ng5: every problem uses 5 Gauss-Legendre points per dimension for CV, CE, CF
@@ -167,7 +168,6 @@ hipError_t launch_panels_queue(const PanelArgs *args, const int4 *items, const i
                                int fast, int c32, hipStream_t s);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
-int scan_partials_needed(int n);
 hipError_t launch_apply(const void *Z, int c32, long long ld, int rows, int cols, const double2 *x, int nrhs,
                         double2 *partial, double2 *y, hipStream_t s);
 hipError_t launch_compare(const void *a, int a32, const void *b, int b32, long long n, int groups, double *out,

@@ -4,7 +4,7 @@
 //
 // Reference path replaced (file:line in the reference):
 //   quadrule_determine_type          quadrule_determine.jl:54-77      -> k_classify
-//   sort_data (compaction)           nonMainCaseQuadratures.jl:13-32  -> k_classify pass 1 + scan
+//   sort_data (compaction)           nonMainCaseQuadratures.jl:13-32  -> k_classify (atomic block ranges)
 //   test_toll! + SS kernel           pulledback_integrand_gpu.jl:22-228,
 //                                    SauterSchwab.jl:212-298,
 //                                    Integrand_gpu.jl:58-132          -> k_special
@@ -573,7 +573,7 @@ __device__ __forceinline__ int find_problem(const int *base, int n, int u) {
 
 // ------------------------------------------------------- classification
 
-// One work-group per test patch p.  For each trial patch q the bounding
+// One work-group per test sub-batch p.  For each trial patch q the bounding
 // spheres give a lower bound D on the distance between any cell of p and any
 // cell of q; a block can hold a special pair only if D is below the largest
 // distance at which a pair can touch (0), be near (dmin < 0.8 sqrt(A_trial),
@@ -583,18 +583,19 @@ __device__ __forceinline__ int find_problem(const int *base, int n, int u) {
 // hits = vertex pairs closer than 1e3 eps (compared as squared distances);
 // otherwise near iff dmin^2 max(16|gamma|^2, 1/A_trial) < 0.64; the mixed
 // path marks far pairs with close centroids as class 5.
-// pass 0 writes the number of special pairs of every block; pass 1 writes the
-// pairs, in thread order, at the block's offset.
+// A block's special pairs are written, in thread order, to a range of entries
+// reserved with one atomic add (its first entry and count are recorded per
+// block); culled blocks record a count of zero.
 // Work-group g takes sub-batch g - gbase[i] of problem i (one problem, or
 // every problem of a batch).
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyArgs *args, const int *gbase, int nprob,
-                                                                int pass) {
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyArgs *args, const int *gbase, int nprob) {
     __shared__ double tv[BMI_PATCH][9], bv[BMI_PATCH][9];
     __shared__ double tc[BMI_PATCH][3], bc[BMI_PATCH][3];
     __shared__ double barea[BMI_PATCH];
     __shared__ int tt[BMI_PATCH], bt[BMI_PATCH];
     __shared__ int nearq[BMI_BLOCK_THREADS];
     __shared__ int wsum[BMI_BLOCK_THREADS / 64];
+    __shared__ int s_off;
 
     const int pi = find_problem(gbase, nprob, (int)blockIdx.x);
     const ClassifyArgs a = args[pi];  // a copy: the kernel's stores cannot alias it
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
             double reach = fmax(1e-9, 0.8 * sqrt(a.Y.patch_amax[q]));
             if (a.near_d >= 0.0) reach = fmax(reach, a.near_d);
             cand = D < 1.0001 * reach + 1e-12;
-            if (pass == 0 && !cand) a.blk_count[(long long)p * a.npb + q] = 0;
+            if (!cand) a.blk_cnt[(long long)p * a.npb + q] = 0;
         }
         const unsigned long long m = __ballot(cand);
         if (lane == 0) wsum[wv] = __popcll(m);
@@ -698,10 +699,17 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
                 sbase += i < wv ? wsum[i] : 0;
                 total += wsum[i];
             }
-            if (pass == 0) {
-                if (t == 0) a.blk_count[b] = total;
-            } else if (special) {
-                const int e = a.blk_off[b] + sbase + before;
+            // the block's entries go to a range reserved with one atomic: the
+            // ranges' order varies from run to run, their contents do not
+            if (t == 0) {
+                const int off = total > 0 ? atomicAdd(a.counter, total) : 0;
+                s_off = off;
+                a.blk_off[b] = off;
+                a.blk_cnt[b] = total;
+            }
+            __syncthreads();
+            if (special) {
+                const int e = s_off + sbase + before;
                 if (e < a.capacity) {
                     a.spec_K[e] = K;
                     a.spec_L[e] = L;
@@ -715,96 +723,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
     }
 }
 
-// -------------------------------------------------------------- scan
-
-#define SCAN_PER_THREAD 8
-#define SCAN_CHUNK (BMI_BLOCK_THREADS * SCAN_PER_THREAD)
-
-int scan_partials_needed(int n) { return (n + SCAN_CHUNK - 1) / SCAN_CHUNK; }
-
-__device__ __forceinline__ int block_exclusive_scan(int v, int &total) {
-    __shared__ int ws[BMI_BLOCK_THREADS / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) ws[wv] = x;
-    __syncthreads();
-    int base = 0;
-    total = 0;
-#pragma unroll
-    for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
-        base += i < wv ? ws[i] : 0;
-        total += ws[i];
-    }
-    __syncthreads();
-    return base + x - v;
-}
-
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_scan_reduce(const int *in, int n, int *partials) {
-    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_PER_THREAD; ++i) s += (base + i < n) ? in[base + i] : 0;
-    int total;
-    block_exclusive_scan(s, total);
-    if (threadIdx.x == 0) partials[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_scan_top(int *partials, int np, int *off, int n, int *total_out) {
-    const int per = (np + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
-    const int b0 = threadIdx.x * per;
-    int s = 0;
-    for (int i = 0; i < per; ++i) s += (b0 + i < np) ? partials[b0 + i] : 0;
-    int total;
-    int run = block_exclusive_scan(s, total);
-    for (int i = 0; i < per; ++i) {
-        if (b0 + i < np) {
-            const int v = partials[b0 + i];
-            partials[b0 + i] = run;
-            run += v;
-        }
-    }
-    if (threadIdx.x == 0) {
-        off[n] = total;
-        *total_out = total;
-    }
-}
-
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_scan_apply(const int *in, int n, const int *partials, int *out) {
-    const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
-    int v[SCAN_PER_THREAD];
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_PER_THREAD; ++i) {
-        v[i] = (base + i < n) ? in[base + i] : 0;
-        s += v[i];
-    }
-    int total;
-    int run = block_exclusive_scan(s, total) + partials[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < SCAN_PER_THREAD; ++i) {
-        if (base + i < n) out[base + i] = run;
-        run += v[i];
-    }
-}
-
-hipError_t launch_scan(const int *count, int *off, int n, int *partials, int *total, hipStream_t s) {
-    const int np = scan_partials_needed(n);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(np), dim3(BMI_BLOCK_THREADS), 0, s, count, n, partials);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BMI_BLOCK_THREADS), 0, s, partials, np, off, n, total);
-    hipLaunchKernelGGL(k_scan_apply, dim3(np), dim3(BMI_BLOCK_THREADS), 0, s, count, n, partials, off);
-    return hipGetLastError();
-}
-
-hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, int pass,
-                           hipStream_t s) {
+hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, hipStream_t s) {
     if (total_groups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_classify, dim3((unsigned)total_groups), dim3(BMI_BLOCK_THREADS), 0, s, args, gbase, nprob,
-                       pass);
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)total_groups), dim3(BMI_BLOCK_THREADS), 0, s, args, gbase, nprob);
     return hipGetLastError();
 }
 
@@ -1197,7 +1118,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *a
         if (e >= ns || !(a.spec_cls[e] & SPEC_NOT_OWNER)) return;
         const int K = a.spec_K[e], L = a.spec_L[e];
         const long long b = (long long)a.X.home[K] * a.npb + a.Y.home[L];
-        const int o0 = a.blk_off[b], o1 = min(a.blk_off[b + 1], a.capacity);
+        const int o0 = a.blk_off[b], o1 = min(o0 + a.blk_cnt[b], a.capacity);
         int o = -1;
         for (int u = o0; u < o1; ++u)
             if (a.spec_K[u] == K && a.spec_L[u] == L) {
@@ -1343,7 +1264,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
         }
         const long long b = (long long)s * a.npb + q;
         pe0 = a.blk_off[b];
-        pe1 = a.blk_off[b + 1];
+        pe1 = pe0 + a.blk_cnt[b];
     };
     if (sb0 < sb1) fetch(sb0);
 
@@ -1513,7 +1434,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
         }
         const long long b = (long long)s * a.npb + q;
         pe0 = a.blk_off[b];
-        pe1 = a.blk_off[b + 1];
+        pe1 = pe0 + a.blk_cnt[b];
     };
     auto stash = [&](int buf) {
         sg[buf][t] = pg0;
@@ -1776,6 +1697,11 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_fill_zero(uint4 *p, long 
         p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_zero_each(int *const *ptrs, int n) {
+    const int i = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
+    if (i < n) *ptrs[i] = 0;
+}
+
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_zero_ints(int *p, int n) {
     for (int i = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x; i < n; i += gridDim.x * BMI_BLOCK_THREADS) p[i] = 0;
 }
@@ -1807,6 +1733,13 @@ hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
         hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(BMI_BLOCK_THREADS), 0, s, (int *)((char *)p + n16 * 16),
                            (int)((bytes % 16) / 4));
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_each(int *const *ptrs, int n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_zero_each, dim3((n + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS), dim3(BMI_BLOCK_THREADS),
+                           0, s, ptrs, n);
     return hipGetLastError();
 }
 

@@ -174,8 +174,8 @@ bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double
 
 /* --------------------------------------------------------------- batching
  * Many assemblies (plans with their output matrices, one context) run as one
- * replayed hipGraph: the classification and special-pair launches of the
- * problems on parallel graph branches, then, per bucket of problems that
+ * replayed hipGraph: one classification launch over every problem, one
+ * special-pair launch per operator, then, per bucket of problems that
  * share a far-pair kernel (operator, precision, rule, output type), one
  * persistent launch whose work-groups pop (problem, test panel, trial patch)
  * blocks from a device queue, or one launch per trial colour over every
