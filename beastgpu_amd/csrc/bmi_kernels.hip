@@ -133,6 +133,40 @@ __device__ __forceinline__ void green64(const OpConst &op, bool lossy, double w4
     hi = -h * s;
 }
 
+// Smooth remainder of G for the near-pair subtraction, times w:
+// h = w4pi (exp(-gamma R) - 1 - gamma^2 R^2 / 2) / R = w4pi (-gamma + O(R^2)),
+// exp(-gamma R) - 1 taken as exp(-gr R)(-2 sin^2(kR/2) - i sin kR) + expm1(-gr R)
+// so that no term loses its leading digits for small R.
+template <bool TAB>
+__device__ __forceinline__ void green_smooth64(const OpConst &op, bool lossy, double w4pi, double dx, double dy,
+                                               double dz, double &hr, double &hi, const double2 *tab) {
+    const double R2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double k = op.k2pi * 6.283185307179586476925;
+    if (R2 == 0.0) {
+        hr = -w4pi * op.gamma_re;
+        hi = -w4pi * k;
+        return;
+    }
+    const double rinv = rsqrt64(R2);
+    const double R = R2 * rinv;
+    double sh, ch;
+    if (TAB) sincos_turns64_tab(0.5 * R * op.k2pi, tab, sh, ch);
+    else sincos_turns64(0.5 * R * op.k2pi, sh, ch);
+    const double s = 2.0 * sh * ch, c1 = -2.0 * sh * sh;  // sin kR, cos kR - 1
+    double re, im;
+    if (lossy) {
+        const double gr = op.gamma_re, E = exp(-gr * R);
+        re = fma(E, c1, expm1(-gr * R)) - 0.5 * (gr * gr - k * k) * R2;
+        im = -fma(E, s, gr * k * R2);
+    } else {
+        re = fma(0.5 * k * k, R2, c1);
+        im = -s;
+    }
+    const double h = w4pi * rinv;
+    hr = h * re;
+    hi = h * im;
+}
+
 // grad_x G factor for one point pair in FP64: grad_x G(r) = q(R) r with
 // q = -(1 + gamma R) exp(-gamma R) / (4 pi R^3); returns h = 4 pi w4pi q as (re, im).
 template <bool TAB>
@@ -744,7 +778,9 @@ struct CellG {
     double t2[3];  // v2 - v3
 };
 
-template <int OP>
+// SMOOTH: the smooth remainder of G (near pairs of the single layers) in
+// place of G.
+template <int OP, bool SMOOTH = false>
 __device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const double2 *tab, SpecAcc &acc, const CellG &X, const CellG &Y,
                                            const double *dc, double bx1, double bx2, double by1, double by2,
                                            double w, const double *nrm) {
@@ -786,8 +822,12 @@ __device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const 
         return;
     }
     double hr, hi;
-    green64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1], dc[2] + x[2] - y[2], hr, hi,
-                  tab);
+    if (SMOOTH)
+        green_smooth64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1],
+                             dc[2] + x[2] - y[2], hr, hi, tab);
+    else
+        green64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1],
+                      dc[2] + x[2] - y[2], hr, hi, tab);
     if (OP == OPK_MW) {
         acc.v[0] += hr;
         acc.v[1] += hi;
@@ -854,17 +894,163 @@ __device__ __forceinline__ void ss_map(int cls, int r, double e1, double e2, dou
     }
 }
 
-// One wave per touching or near pair (owner entries only:
-// a pair whose test cell belongs to several sub-batches is integrated once,
-// in the sub-batch the cell calls home).  Regions in turn, lanes striding
-// over the hypercube nodes, one Green evaluation each; a butterfly over the
-// wave reduces the accumulators.
+// Closed-form integrals over the flat trial triangle w[0..2] (unit normal n)
+// at the point x, R = |x - y|, xp the projection of x onto its plane:
+//   I0 = int 1/R,  I1 = int (y - xp)/R,  K1 = int R,  K1v = int (y - xp) R.
+// Per edge a -> b (counter-clockwise about n): l the unit edge vector, m = l x n
+// its outward in-plane normal, t+- = (b, a - xp).l, p = (a - xp).m, h = |x - xp|,
+// r+- = |x - b|, |x - a|, r0^2 = p^2 + h^2, L = ln((r+ + t+)/(r- + t-)); the
+// edge integrals E1 = int_e R = [t r + r0^2 L]/2 and
+// E3 = int_e R^3 = [t r^3]/4 + 3 r0^2 [t r]/8 + 3 r0^4 L/8 ([f] = f(+) - f(-))
+// give I1 = sum m E1, K1v = sum m E3/3, K1 = (sum p E1 + h^2 I0)/3 and
+// I0 = sum p L - h [atan(p t+ / (r0^2 + h r+)) - atan(p t- / (r0^2 + h r-))]
+// (Wilton et al., IEEE TAP 32, 1984; Graglia, IEEE TAP 41, 1993).  L is taken
+// in the one of three equal forms that subtracts no nearly equal numbers.
+__device__ __forceinline__ void tri_potentials64(const double (&w)[3][3], const double (&n)[3], const double (&x)[3],
+                                                 double &I0, double (&I1)[3], double &K1, double (&K1v)[3],
+                                                 double (&xp)[3]) {
+    const double h0 = (x[0] - w[0][0]) * n[0] + (x[1] - w[0][1]) * n[1] + (x[2] - w[0][2]) * n[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xp[d] = x[d] - h0 * n[d];
+    const double h = fabs(h0);
+    double i0 = 0.0, k1 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) I1[d] = K1v[d] = 0.0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const int e1 = e == 2 ? 0 : e + 1;
+        double l[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) l[d] = w[e1][d] - w[e][d];
+        const double il = rsqrt64(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) l[d] *= il;
+        const double m[3] = {l[1] * n[2] - l[2] * n[1], l[2] * n[0] - l[0] * n[2], l[0] * n[1] - l[1] * n[0]};
+        double tp = 0.0, tm = 0.0, p = 0.0, rp2 = 0.0, rm2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            tp = fma(w[e1][d] - xp[d], l[d], tp);
+            tm = fma(w[e][d] - xp[d], l[d], tm);
+            p = fma(w[e][d] - xp[d], m[d], p);
+            rp2 = fma(x[d] - w[e1][d], x[d] - w[e1][d], rp2);
+            rm2 = fma(x[d] - w[e][d], x[d] - w[e][d], rm2);
+        }
+        const double rp = sqrt(rp2), rm = sqrt(rm2);
+        const double r0sq = fma(p, p, h * h);
+        double lg = 0.0;
+        if (tm >= 0.0) lg = log((rp + tp) / (rm + tm));
+        else if (tp <= 0.0) lg = log((rm - tm) / (rp - tp));
+        else if (r0sq > 0.0) lg = log((rp + tp) * (rm - tm) / r0sq);
+        if (fabs(p) * il > 1e-14)
+            i0 += p * lg - h * (atan(p * tp / fma(h, rp, r0sq)) - atan(p * tm / fma(h, rm, r0sq)));
+        const double tr = tp * rp - tm * rm;
+        const double E1 = 0.5 * fma(r0sq, lg, tr);
+        const double E3 = 0.25 * (tp * rp * rp2 - tm * rm * rm2) + 0.375 * r0sq * fma(r0sq, lg, tr);
+        k1 = fma(p, E1, k1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            I1[d] = fma(E1, m[d], I1[d]);
+            K1v[d] = fma(E3 * (1.0 / 3.0), m[d], K1v[d]);
+        }
+    }
+    I0 = i0;
+    K1 = fma(h * h, i0, k1) * (1.0 / 3.0);
+}
+
+// Near pair of a single layer, outer point (b1, b2) of weight nw on the test
+// cell: adds the trial-cell integrals of the singular terms
+// [1/R + (gamma^2/2) R]/(4 pi) of G, in closed form, to the accumulators in the
+// form spec_point gives them (reference weights: sum_J nw_J f(y_J) stands for
+// int_L f / (2 A_L)).  The remainder is left to spec_point<OP, true>.
+template <int OP>
+__device__ __forceinline__ void near_static(const OpConst &op, SpecAcc &acc, const CellG &X, const CellG &Y,
+                                            const double *dc, double b1, double b2, double nw) {
+    double x[3], xl[3], w[3][3], n[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[d] = X.o[d] + X.t1[d] * b1 + X.t2[d] * b2;  // relative to the test centroid
+        xl[d] = x[d] + dc[d];                        // relative to the trial centroid
+        w[0][d] = Y.o[d] + Y.t1[d];
+        w[1][d] = Y.o[d] + Y.t2[d];
+        w[2][d] = Y.o[d];
+    }
+    n[0] = Y.t1[1] * Y.t2[2] - Y.t1[2] * Y.t2[1];
+    n[1] = Y.t1[2] * Y.t2[0] - Y.t1[0] * Y.t2[2];
+    n[2] = Y.t1[0] * Y.t2[1] - Y.t1[1] * Y.t2[0];
+    const double nn2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    const double inv = rsqrt64(nn2);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) n[d] *= inv;
+    const double two_area = nn2 * inv;
+    double I0, I1[3], K1, K1v[3], xp[3];
+    tri_potentials64(w, n, xl, I0, I1, K1, K1v, xp);
+    // gamma^2 / 2
+    const double k = op.k2pi * 6.283185307179586476925, gr = op.gamma_re;
+    const double g2r = 0.5 * (gr * gr - k * k), g2i = gr * k;
+    const double sc = nw / (two_area * 4.0 * M_PI);
+    const double s0r = sc * fma(g2r, K1, I0), s0i = sc * g2i * K1;
+    double s1r[3], s1i[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double v = fma(xp[d], K1, K1v[d]);
+        s1r[d] = sc * (fma(xp[d], I0, I1[d]) + g2r * v);
+        s1i[d] = sc * g2i * v;
+    }
+    if (OP == OPK_MW) {
+        acc.v[0] += s0r;
+        acc.v[1] += s0i;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            acc.v[2 + d] = fma(s0r, x[d], acc.v[2 + d]);
+            acc.v[5 + d] = fma(s0i, x[d], acc.v[5 + d]);
+            acc.v[8 + d] += s1r[d];
+            acc.v[11 + d] += s1i[d];
+        }
+        acc.v[14] += x[0] * s1r[0] + x[1] * s1r[1] + x[2] * s1r[2];
+        acc.v[15] += x[0] * s1i[0] + x[1] * s1i[1] + x[2] * s1i[2];
+    } else {
+        // barycentric coordinates of the trial chart as affine functions
+        // lambda_j(y) = c0_j + c_j . y (Gram-matrix inverse of t1, t2)
+        const double g11 = Y.t1[0] * Y.t1[0] + Y.t1[1] * Y.t1[1] + Y.t1[2] * Y.t1[2];
+        const double g12 = Y.t1[0] * Y.t2[0] + Y.t1[1] * Y.t2[1] + Y.t1[2] * Y.t2[2];
+        const double g22 = Y.t2[0] * Y.t2[0] + Y.t2[1] * Y.t2[1] + Y.t2[2] * Y.t2[2];
+        const double idet = 1.0 / (g11 * g22 - g12 * g12);
+        double c[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            c[0][d] = (g22 * Y.t1[d] - g12 * Y.t2[d]) * idet;
+            c[1][d] = (g11 * Y.t2[d] - g12 * Y.t1[d]) * idet;
+            c[2][d] = -c[0][d] - c[1][d];
+        }
+        const double f[3] = {b1, b2, 1.0 - b1 - b2};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double c0 = (j == 2 ? 1.0 : 0.0) - (c[j][0] * Y.o[0] + c[j][1] * Y.o[1] + c[j][2] * Y.o[2]);
+            const double glr = c0 * s0r + c[j][0] * s1r[0] + c[j][1] * s1r[1] + c[j][2] * s1r[2];
+            const double gli = c0 * s0i + c[j][0] * s1i[0] + c[j][1] * s1i[1] + c[j][2] * s1i[2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                acc.v[2 * (3 * i + j)] = fma(f[i], glr, acc.v[2 * (3 * i + j)]);
+                acc.v[2 * (3 * i + j) + 1] = fma(f[i], gli, acc.v[2 * (3 * i + j) + 1]);
+            }
+        }
+    }
+}
+
+// One wave per touching pair (NEARK false) or per near pair (NEARK true),
+// owner entries only: a pair whose test cell belongs to several sub-batches
+// is integrated once, in the sub-batch the cell calls home.  Touching pairs:
+// regions in turn, lanes striding over the hypercube nodes, one Green
+// evaluation each.  Near pairs: closed-form singular terms, one outer point
+// per lane, then lanes striding over the product rule.  A butterfly over the
+// wave reduces the accumulators.  The two kinds are separate instantiations
+// so that the near-pair code does not add to the registers of the
+// Sauter-Schwab loop.
 // NG: Gauss-Legendre points per hypercube dimension fixed at compile time
 // (the default 5; constant-divisor node decoding), 0: read per class.
-template <int OP, int NG>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_special(const SpecialArgs *args,
-                                                                                    const int *cbase, int nprob,
-                                                                                    int cs) {
+template <int OP, int NG, bool NEARK>
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2) void k_special(
+    const SpecialArgs *args, const int *cbase, int nprob, int cs) {
     const int pi = find_problem(cbase, nprob, (int)blockIdx.x);
     const SpecialArgs &a = args[pi];
     __shared__ double s_glx[3][BMI_MAX_GL], s_glw[3][BMI_MAX_GL];
@@ -887,15 +1073,15 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
     const int lane = t & 63, wv = t >> 6;
     const int ns = min(*a.n_special, a.capacity);
     const bool lossy = a.op.lossy != 0;
-    // one chunk of cs (<= 256) entries: the threads pick out the touching and
-    // near entries the problem owns, then each wave integrates every fourth
+    // one chunk of cs (<= 256) entries: the threads pick out the touching (or
+    // near) entries the problem owns, then each wave integrates every fourth
     {
         const int c0 = ((int)blockIdx.x - cbase[pi]) * cs;
         const int ec = c0 + t;
         int take = 0;
         if (t < cs && ec < ns) {
             const int cl = a.spec_cls[ec];
-            take = !(cl & SPEC_NOT_OWNER) && cl >= CLS_CV && cl <= CLS_NEAR;
+            take = !(cl & SPEC_NOT_OWNER) && (NEARK ? cl == CLS_NEAR : (cl >= CLS_CV && cl <= CLS_CF));
         }
         const unsigned long long m = __ballot(take);
         __syncthreads();  // s_items, s_wsum of the previous chunk are consumed
@@ -916,7 +1102,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
         // vertex permutations of the two cells (identity for near pairs):
         // chart vertex c of the test cell is its vertex PT[c]
         int PT[3] = {0, 1, 2}, PS[3] = {0, 1, 2};
-        if (cls != CLS_NEAR) {
+        if constexpr (!NEARK) {
             // vertex correspondence (pulledback_integrand_gpu.jl:22-155)
             int mi[3] = {0, 0, 0}, mj[3] = {0, 0, 0}, ne = 0;
 #pragma unroll
@@ -987,13 +1173,18 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, OP == OPK_MW ? 3 : 2) void k_spe
 #pragma unroll
         for (int i = 0; i < 18; ++i) acc.v[i] = 0.0;
 
-        if (cls == CLS_NEAR) {
+        if constexpr (NEARK) {
+            // single layers: singular terms in closed form, one outer point
+            // per lane, then the smooth remainder by the product rule; the
+            // double layer keeps the plain product rule
             const int np = a.near_np, tot = np * np;
+            constexpr bool SUB = OP != OPK_DL;
+            if (SUB && lane < np) near_static<OP>(a.op, acc, X, Y, dc, s_nb1[lane], s_nb2[lane], s_nw[lane]);
 #pragma unroll 1
             for (int idx = lane; idx < tot; idx += 64) {
                 const int I = idx / np, J = idx - np * (idx / np);
-                spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, s_nb1[I], s_nb2[I], s_nb1[J], s_nb2[J],
-                               s_nw[I] * s_nw[J], nrm);
+                spec_point<OP, SUB>(a.op, lossy, s_sc, acc, X, Y, dc, s_nb1[I], s_nb2[I], s_nb1[J], s_nb2[J],
+                                    s_nw[I] * s_nw[J], nrm);
             }
         } else {
             const int g = cls - 1;
@@ -1139,14 +1330,17 @@ hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *
     const dim3 b(BMI_BLOCK_THREADS);
     const dim3 g(total_chunks);
     if (ng5) {
-        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 5>), g, b, 0, s, args, cbase, nprob, cs);
-        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 5>), g, b, 0, s, args, cbase, nprob, cs);
-        else hipLaunchKernelGGL((k_special<OPK_HH3D, 5>), g, b, 0, s, args, cbase, nprob, cs);
+        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
+        else hipLaunchKernelGGL((k_special<OPK_HH3D, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
     } else {
-        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 0>), g, b, 0, s, args, cbase, nprob, cs);
-        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0>), g, b, 0, s, args, cbase, nprob, cs);
-        else hipLaunchKernelGGL((k_special<OPK_HH3D, 0>), g, b, 0, s, args, cbase, nprob, cs);
+        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
+        else hipLaunchKernelGGL((k_special<OPK_HH3D, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
     }
+    if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
+    else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
+    else hipLaunchKernelGGL((k_special<OPK_HH3D, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
     const int g2 = (total_entries + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
     if (mixed) {
         if (op_kind == OPK_MW) hipLaunchKernelGGL(k_far64<OPK_MW>, dim3(g2), b, 0, s, args, ebase, nprob);

@@ -26,9 +26,17 @@
  * n the unit normal of the test cell ((v1 - v3) x (v2 - v3) direction).
  *
  * Pair classes whose rule the reference leaves open: class-4 ("near") pairs are
- * dropped by the reference (doubleQuadRuleMainCase.jl:154); here they are
- * integrated with a Duffy-collapsed Gauss-Legendre product rule of
- * near_order^2 points per triangle.
+ * dropped by the reference (doubleQuadRuleMainCase.jl:154).  Here the single
+ * layers use singularity subtraction in the manner of BEAST's WiltonSE rule:
+ * outer points of a Duffy-collapsed Gauss-Legendre rule of near_order^2 points
+ * on the test cell; for each, the first two singular terms of the expansion
+ * G = [1/R + (gamma^2/2) R + ...]/(4 pi) - gamma/(4 pi) are integrated over the
+ * trial cell in closed form (the potentials of Wilton et al., IEEE TAP 32
+ * (1984), and Graglia, IEEE TAP 41 (1993): R^-1 and R and the in-plane vectors
+ * (y - x_p) R^-1 and (y - x_p) R), and the smooth remainder
+ * (exp(-gamma R) - 1 - gamma^2 R^2/2)/(4 pi R) = -gamma/(4 pi) + O(R^2) with the
+ * same collapsed rule on the trial cell.  The double layer keeps the plain
+ * collapsed product rule.
  *
  * Parity status of each part, and the checks that pin it, are listed in the
  * header of oracle/oracle.py.
@@ -326,6 +334,138 @@ static void double_rule(const orc_operator *op, const orc_space *X, const orc_sp
     }
 }
 
+/* ------------------------------------------------- near pairs (class 4) */
+
+/* Closed-form potentials of the flat triangle s at the point x:
+ *   I0 = int_s dA / R,   I1 = int_s (y - xp) dA / R,
+ *   K1 = int_s R dA,     K1v = int_s (y - xp) R dA,      R = |x - y|,
+ * xp the projection of x onto the plane of s (I1, K1v lie in that plane).
+ * Per edge (from vertex a to b, counter-clockwise about the normal n):
+ * l = unit edge vector, m = l x n its outward in-plane normal, t+- the
+ * coordinates of b and a along l from xp's foot, p = (a - xp).m the signed
+ * in-plane distance to the edge line, h = |(x - a).n|, r+- = |x - b|, |x - a|,
+ * r0^2 = p^2 + h^2, L = ln((r+ + t+)/(r- + t-)); the edge integrals
+ *   E-1 = int_e dl/R = L,   E1 = int_e R dl = [t r + r0^2 L]/2,
+ *   E3 = int_e R^3 dl = [t r^3]/4 + 3 r0^2 [t r]/8 + 3 r0^4 L/8
+ * ([f] = f(+) - f(-)) give, by the in-plane divergence theorem and the
+ * recursion (n + 2) int R^n = sum p E_n + n h^2 int R^(n-2),
+ *   I0 = sum p L - h [atan(p t+ / (r0^2 + h r+)) - atan(p t- / (r0^2 + h r-))],
+ *   I1 = sum m E1,   K1 = (sum p E1 + h^2 I0)/3,   K1v = sum m E3 / 3.
+ * The logarithm is evaluated in one of three equal forms chosen by the signs
+ * of t+-, so that no form subtracts nearly equal numbers. */
+static void tri_potentials(const cell *s, const double *x, double *I0, double I1[3], double *K1, double K1v[3]) {
+    const double *n = s->n;
+    const double h0 = (x[0] - s->v[0][0]) * n[0] + (x[1] - s->v[0][1]) * n[1] + (x[2] - s->v[0][2]) * n[2];
+    const double xp[3] = {x[0] - h0 * n[0], x[1] - h0 * n[1], x[2] - h0 * n[2]};
+    const double h = fabs(h0);
+    double i0 = 0.0, k1 = 0.0;
+    for (int d = 0; d < 3; d++) I1[d] = K1v[d] = 0.0;
+    for (int e = 0; e < 3; e++) {
+        const double *a = s->v[e], *b = s->v[(e + 1) % 3];
+        double l[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double len = sqrt(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+        for (int d = 0; d < 3; d++) l[d] /= len;
+        const double m[3] = {l[1] * n[2] - l[2] * n[1], l[2] * n[0] - l[0] * n[2], l[0] * n[1] - l[1] * n[0]};
+        const double tp = (b[0] - xp[0]) * l[0] + (b[1] - xp[1]) * l[1] + (b[2] - xp[2]) * l[2];
+        const double tm = (a[0] - xp[0]) * l[0] + (a[1] - xp[1]) * l[1] + (a[2] - xp[2]) * l[2];
+        const double p = (a[0] - xp[0]) * m[0] + (a[1] - xp[1]) * m[1] + (a[2] - xp[2]) * m[2];
+        const double rp = sqrt((x[0] - b[0]) * (x[0] - b[0]) + (x[1] - b[1]) * (x[1] - b[1]) + (x[2] - b[2]) * (x[2] - b[2]));
+        const double rm = sqrt((x[0] - a[0]) * (x[0] - a[0]) + (x[1] - a[1]) * (x[1] - a[1]) + (x[2] - a[2]) * (x[2] - a[2]));
+        const double r0sq = p * p + h * h;
+        // ln((r+ + t+)/(r- + t-)) in a form without cancellation: t- < t+;
+        // (r + t)(r - t) = p^2 + h^2 for both ends
+        double lg = 0.0;
+        if (tm >= 0.0) lg = log((rp + tp) / (rm + tm));
+        else if (tp <= 0.0) lg = log((rm - tm) / (rp - tp));
+        else if (r0sq > 0.0) lg = log((rp + tp) * (rm - tm) / r0sq);
+        if (fabs(p) > 1e-14 * len)
+            i0 += p * lg - h * (atan(p * tp / (r0sq + h * rp)) - atan(p * tm / (r0sq + h * rm)));
+        const double tr = tp * rp - tm * rm;
+        const double e1 = 0.5 * (tr + r0sq * lg);
+        const double e3 = 0.25 * (tp * rp * rp * rp - tm * rm * rm * rm) + 0.375 * r0sq * (tr + r0sq * lg);
+        k1 += p * e1;
+        for (int d = 0; d < 3; d++) {
+            I1[d] += e1 * m[d];
+            K1v[d] += e3 * m[d] / 3.0;
+        }
+    }
+    *I0 = i0;
+    *K1 = (k1 + h * h * i0) / 3.0;
+}
+
+/* (exp(-gamma R) - 1 - gamma^2 R^2 / 2) / (4 pi R); exp(-gamma R) - 1 as
+ * exp(-gr R)(cos kR - 1 - i sin kR) + expm1(-gr R), cos kR - 1 = -2 sin^2(kR/2) */
+static cplx green_smooth(const orc_operator *op, double R) {
+    const double gr = op->gamma[0], k = op->gamma[1];
+    const cplx g = gr + I * k;
+    const double sh = sin(0.5 * k * R);
+    const cplx em1 = exp(-gr * R) * (-2.0 * sh * sh - I * sin(k * R)) + expm1(-gr * R);
+    return R > 0.0 ? (em1 - 0.5 * g * g * R * R) / (4.0 * M_PI * R) : -g / (4.0 * M_PI);
+}
+
+/* barycentric coordinates of the chart, as affine functions of y:
+ * lambda_j(y) = c0[j] + c[j] . y (lambda_0 = b1, lambda_1 = b2, lambda_2 = 1 - b1 - b2) */
+static void barycentric_affine(const cell *s, double c0[3], double c[3][3]) {
+    const double g11 = s->t1[0] * s->t1[0] + s->t1[1] * s->t1[1] + s->t1[2] * s->t1[2];
+    const double g12 = s->t1[0] * s->t2[0] + s->t1[1] * s->t2[1] + s->t1[2] * s->t2[2];
+    const double g22 = s->t2[0] * s->t2[0] + s->t2[1] * s->t2[1] + s->t2[2] * s->t2[2];
+    const double det = g11 * g22 - g12 * g12;
+    for (int d = 0; d < 3; d++) {
+        c[0][d] = (g22 * s->t1[d] - g12 * s->t2[d]) / det;
+        c[1][d] = (g11 * s->t2[d] - g12 * s->t1[d]) / det;
+        c[2][d] = -c[0][d] - c[1][d];
+    }
+    for (int j = 0; j < 3; j++)
+        c0[j] = (j == 2 ? 1.0 : 0.0) - (c[j][0] * s->v[2][0] + c[j][1] * s->v[2][1] + c[j][2] * s->v[2][2]);
+}
+
+/* Near pair of a single layer by singularity subtraction (see the header). */
+static void near_subtraction(const orc_operator *op, const orc_space *X, const orc_space *Y, const cell *t,
+                             const cell *s, const tri_rule *rn, cplx Z[3][3]) {
+    const cplx a = op->alpha[0] + I * op->alpha[1], bt = op->beta[0] + I * op->beta[1];
+    const cplx g = op->gamma[0] + I * op->gamma[1];
+    double c0[3], c[3][3];
+    barycentric_affine(s, c0, c);
+    for (int ia = 0; ia < rn->n; ia++) {
+        double x[3];
+        chart_point(t, rn->b1[ia], rn->b2[ia], x);
+        shapes f;
+        eval_shapes(X->kind, t, rn->b1[ia], rn->b2[ia], x, &f);
+        const double wx = rn->w[ia] * 2.0 * t->area;
+        /* S0 = int_s G dy, S1 = int_s G y dy */
+        double I0, I1[3], K1, K1v[3];
+        tri_potentials(s, x, &I0, I1, &K1, K1v);
+        const double h0 = (x[0] - s->v[0][0]) * s->n[0] + (x[1] - s->v[0][1]) * s->n[1] + (x[2] - s->v[0][2]) * s->n[2];
+        const cplx g2h = 0.5 * g * g;
+        cplx S0 = (I0 + g2h * K1) / (4.0 * M_PI), S1[3];
+        for (int d = 0; d < 3; d++) {
+            const double xp = x[d] - h0 * s->n[d];
+            S1[d] = (xp * I0 + I1[d] + g2h * (xp * K1 + K1v[d])) / (4.0 * M_PI);
+        }
+        for (int jb = 0; jb < rn->n; jb++) {
+            double y[3];
+            chart_point(s, rn->b1[jb], rn->b2[jb], y);
+            const double R = sqrt((x[0] - y[0]) * (x[0] - y[0]) + (x[1] - y[1]) * (x[1] - y[1]) + (x[2] - y[2]) * (x[2] - y[2]));
+            const cplx gw = rn->w[jb] * 2.0 * s->area * green_smooth(op, R);
+            S0 += gw;
+            for (int d = 0; d < 3; d++) S1[d] += gw * y[d];
+        }
+        for (int j = 0; j < 3; j++) {
+            if (op->op == ORC_HH3D_SL) {
+                const cplx gl = c0[j] * S0 + c[j][0] * S1[0] + c[j][1] * S1[1] + c[j][2] * S1[2];
+                for (int i = 0; i < 3; i++) Z[i][j] += wx * a * f.sval[i] * gl;
+            } else {
+                cplx gv[3];
+                for (int d = 0; d < 3; d++) gv[d] = (S1[d] - s->v[j][d] * S0) / (2.0 * s->area);
+                const cplx gd = S0 / s->area;
+                for (int i = 0; i < 3; i++)
+                    Z[i][j] += wx * (a * (f.val[i][0] * gv[0] + f.val[i][1] * gv[1] + f.val[i][2] * gv[2]) +
+                                     bt * f.div[i] * gd);
+            }
+        }
+    }
+}
+
 /* ---------------------------------------------------------- Sauter-Schwab */
 
 /* reference simplex vertices used for the permuted charts
@@ -452,7 +592,8 @@ int orc_pair_block(const orc_operator *op, const orc_space *X, const orc_space *
     } else if (cls == ORC_NEAR) {
         tri_rule rn;
         if (make_collapsed_rule(q->near_order, &rn)) return -1;
-        double_rule(op, X, Y, &t, &s, &rn, &rn, Z);
+        if (op->kernel == ORC_KERNEL_GREEN && op->op != ORC_MW_DL) near_subtraction(op, X, Y, &t, &s, &rn, Z);
+        else double_rule(op, X, Y, &t, &s, &rn, &rn, Z);
     } else if (cls <= 3) {
         int n = q->ss_order[cls - 1];
         if (n < 1 || n > 64) return -1;
@@ -605,4 +746,27 @@ int orc_collapsed_rule(int order, double *b1, double *b2, double *w) {
     if (make_collapsed_rule(order, &r)) return -1;
     for (int i = 0; i < r.n; i++) { b1[i] = r.b1[i]; b2[i] = r.b2[i]; w[i] = r.w[i]; }
     return r.n;
+}
+
+/* Closed-form triangle potentials, exposed for their own tests: v holds the
+ * three vertices, out = (I0, I1x, I1y, I1z, K1, K1vx, K1vy, K1vz) at each of
+ * the m points x. */
+void orc_tri_potentials(const double *v, const double *x, int64_t m, double *out) {
+    cell c;
+    for (int i = 0; i < 3; i++)
+        for (int d = 0; d < 3; d++) c.v[i][d] = v[3 * i + d];
+    for (int d = 0; d < 3; d++) {
+        c.t1[d] = c.v[0][d] - c.v[2][d];
+        c.t2[d] = c.v[1][d] - c.v[2][d];
+    }
+    const double n0 = c.t1[1] * c.t2[2] - c.t1[2] * c.t2[1];
+    const double n1 = c.t1[2] * c.t2[0] - c.t1[0] * c.t2[2];
+    const double n2 = c.t1[0] * c.t2[1] - c.t1[1] * c.t2[0];
+    const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    c.area = 0.5 * nn;
+    c.n[0] = n0 / nn;
+    c.n[1] = n1 / nn;
+    c.n[2] = n2 / nn;
+    for (int64_t k = 0; k < m; k++)
+        tri_potentials(&c, x + 3 * k, out + 8 * k, out + 8 * k + 1, out + 8 * k + 4, out + 8 * k + 5);
 }

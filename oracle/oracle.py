@@ -32,11 +32,20 @@ Parity status of each oracle function, and what pins it (tests named):
   and convergence in the Sauter-Schwab order (test_dl_ss_convergence).  BEAST's
   own MWDoubleLayer3D integrand is not importable here: its sign and
   normalisation conventions are parity unpinned.
+* near pairs (class 4, which the reference drops: doubleQuadRuleMainCase.jl:154)
+  of the single layers by singularity subtraction -- the closed-form triangle
+  integrals of 1/R, (y - xp)/R, R and (y - xp) R are pinned against the
+  independent closed form of Wilton et al. and a composite Duffy cubature at
+  points above, in and beside the plane (test_triangle_potentials_vs_cubature);
+  the static P1 block's row and column sums against the closed-form potential
+  and cubature (test_near_static_vs_analytic); whole blocks, lossless and
+  lossy, RWG and P1, against the same outer rule with a composite Duffy inner
+  integral, with the error falling in the order (test_near_vs_bruteforce).
 * BEAST conventions that only an import of BEAST 2.6.0 / CompScienceMeshes
   0.9.3 / SauterSchwabQuadrature 2.4.1 could fix (triangle-rule tables and
-  point order, RT orientation signs and edge numbering, near-pair WiltonSE
-  rule) are parity unpinned; the symmetric rules used here make the point
-  order irrelevant up to rounding.
+  point order, RT orientation signs and edge numbering, the exact orders and
+  expansion terms of BEAST's WiltonSE near rule) are parity unpinned; the
+  symmetric rules used here make the point order irrelevant up to rounding.
 """
 from __future__ import annotations
 
@@ -114,6 +123,8 @@ def lib():
         L.orc_collapsed_rule.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_gauss_legendre01.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_gauss_legendre01.restype = None
+        L.orc_tri_potentials.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.orc_tri_potentials.restype = None
         _lib = L
     return _lib
 
@@ -217,3 +228,14 @@ def gauss_legendre01(n):
     x, w = np.zeros(n), np.zeros(n)
     lib().orc_gauss_legendre01(n, x.ctypes.data, w.ctypes.data)
     return x, w
+
+
+def tri_potentials(v, x):
+    """Closed-form integrals over the flat triangle v (3, 3) at the points x
+    (m, 3), R = |x - y|, xp the projection of x onto the triangle's plane:
+    returns (int 1/R, int (y - xp)/R, int R, int (y - xp) R)."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(3, 3)
+    x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+    out = np.zeros((x.shape[0], 8))
+    lib().orc_tri_potentials(v.ctypes.data, x.ctypes.data, x.shape[0], out.ctypes.data)
+    return out[:, 0], out[:, 1:4], out[:, 4], out[:, 5:8]

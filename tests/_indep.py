@@ -87,3 +87,36 @@ def triangle_potential(v, r):
         term = np.where(np.abs(P0) < 1e-14, 0.0, term)
         out += term
     return out
+
+
+def split_rule(v, x, n, m=4):
+    """Composite rule on the flat triangle v (3, 3) for integrands singular or
+    nearly singular at the projection xp of the point x: the triangle is split
+    into the 3m triangles (xp, a, b) over m pieces of each edge, each with
+    the collapsed n^2 rule collapsed at xp (the Duffy transform removes a 1/R
+    singularity there).  xp may lie outside v: the pieces then carry signed
+    weights.  Returns points (k, 3) and weights (k,)."""
+    nrm = np.cross(v[0] - v[2], v[1] - v[2])
+    nrm = nrm / np.linalg.norm(nrm)
+    xp = x - ((x - v[0]) @ nrm) * nrm
+    b1, b2, w = collapsed(n)
+    P, W = [], []
+    for e in range(3):
+        a0, a1 = v[e], v[(e + 1) % 3]
+        for k in range(m):
+            a = a0 + (a1 - a0) * k / m
+            b = a0 + (a1 - a0) * (k + 1) / m
+            sa = 0.5 * np.cross(a - xp, b - xp) @ nrm  # signed area
+            if sa == 0.0:
+                continue
+            P.append(cell_points(np.array([xp, a, b]), b1, b2))
+            W.append(w * 2.0 * sa)
+    return np.concatenate(P), np.concatenate(W)
+
+
+def barycentric(v, P):
+    """Barycentric coordinates (b1, b2, 1 - b1 - b2) of points P (k, 3) in the
+    plane of v, in the chart x = v3 + b1 (v1 - v3) + b2 (v2 - v3)."""
+    M = np.stack([v[0] - v[2], v[1] - v[2]], axis=1)
+    b = np.linalg.lstsq(M, (P - v[2]).T, rcond=None)[0].T
+    return np.stack([b[:, 0], b[:, 1], 1.0 - b[:, 0] - b[:, 1]], axis=1)

@@ -257,3 +257,43 @@ def test_nondefault_ss_orders_match_oracle(gpu_ctx, kind):
     err = relfro(Z, Zref)
     print("rel err", err)
     assert err <= FP64_TOL
+
+
+def _near_mesh(which):
+    """Meshes with class-4 (near) pairs: two concentric spheres 3% apart (one
+    mesh, two closed components; every cell is near its copy), or the
+    jittered ellipsoid, whose thin cells come near cells they do not touch."""
+    if which == "shells":
+        a = meshes.icosphere(2)
+        nv = a.vertices.shape[0]
+        return meshes.Mesh(np.vstack([a.vertices, 1.03 * a.vertices]), np.vstack([a.faces, a.faces + nv]))
+    return meshes.ellipsoid(cells=(16, 10, 10))
+
+
+@pytest.mark.parametrize("which", ["shells", "ellipsoid"])
+@pytest.mark.parametrize("kind,precision,near_order", [
+    ("efie", "fp64", 4), ("hh3d", "fp64", 4), ("mfie", "fp64", 4), ("efie-lossy", "fp64", 4),
+    ("efie", "fp64", 7), ("hh3d", "mixed", 4)])
+def test_near_pairs_match_oracle(gpu_ctx, which, kind, precision, near_order):
+    """Near pairs: closed-form singular terms plus the product rule on the
+    remainder for the single layers, the plain product rule for the double
+    layer, against the oracle."""
+    m = _near_mesh(which)
+    if kind == "efie":
+        op, X = Maxwell3D.singlelayer(wavenumber=3.0), spaces.raviartthomas(m)
+    elif kind == "efie-lossy":
+        op, X = Maxwell3D.singlelayer(gamma=complex(0.8, 3.0)), spaces.raviartthomas(m)
+    elif kind == "mfie":
+        op, X = Maxwell3D.doublelayer(wavenumber=3.0), spaces.raviartthomas(m)
+    else:
+        op, X = Helmholtz3D.singlelayer(wavenumber=3.0, alpha=0.7 - 0.2j), spaces.lagrangec0d1(m)
+    print(f"case near {which} {kind} {precision} order {near_order} nfun={X.nfun}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(precision=precision, near_order=near_order), ctx=gpu_ctx)
+    Z = a.assemble().to_host()
+    st = a.stats()
+    a.close()
+    Zref, counts = O.assemble(op, X, X, near_order=near_order)
+    err = relfro(Z, Zref)
+    print("near rel err", err, st)
+    assert st["pairs"]["near"] == counts[4] > 0
+    assert err <= (MIXED_TOL if precision == "mixed" else FP64_TOL)

@@ -218,6 +218,145 @@ def test_ss_convergence(kind):
         assert errs[1] < 5e-4 and errs[2] < 5e-6, errs
 
 
+# ------------------------------------------------------ near pairs (class 4)
+
+def _potential_points(v):
+    nrm = np.cross(v[0] - v[2], v[1] - v[2])
+    nrm = nrm / np.linalg.norm(nrm)
+    c = v.mean(axis=0)
+    return np.array([
+        c + 0.1 * nrm,                               # above the centroid
+        c,                                           # in the plane, inside
+        c - 0.37 * nrm,                              # below
+        0.3 * v[0] + 0.7 * v[1] + 0.4 * (v[1] - v[2]),  # in the plane, outside
+        2.0 * v[1] - v[0],                           # on an edge line, outside the edge
+        2.0 * v[1] - v[0] + 0.3 * nrm,               # above that point
+        v[0] + np.array([0.01, 0.02, 0.03]),         # near a vertex
+        v[2] + 1.5 * (v[2] - c) - 0.2 * nrm,         # beyond a vertex
+    ])
+
+
+def test_triangle_potentials_vs_cubature():
+    """Closed-form int 1/R, int (y - xp)/R, int R and int (y - xp) R over a
+    triangle against the independent closed form of 1/R (Wilton et al. 1984)
+    and against a composite Duffy rule collapsed at the point's projection,
+    at points above, in and beside the plane, on an edge line and near a
+    vertex."""
+    v = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.3, 0.9, 0.05]])
+    xs = _potential_points(v)
+    I0, I1, K1, K1v = O.tri_potentials(v, xs)
+    assert np.allclose(I0, ind.triangle_potential(v, xs), rtol=1e-13, atol=0)
+    nrm = np.cross(v[0] - v[2], v[1] - v[2])
+    nrm = nrm / np.linalg.norm(nrm)
+    for k, x in enumerate(xs):
+        P, W = ind.split_rule(v, x, 40)
+        R = np.linalg.norm(P - x, axis=1)
+        xp = x - ((x - v[0]) @ nrm) * nrm
+        q0 = np.sum(W / R)
+        q1 = (W / R) @ (P - xp)
+        k1 = np.sum(W * R)
+        k1v = (W * R) @ (P - xp)
+        assert abs(I0[k] - q0) <= 1e-8 * abs(q0), (k, I0[k], q0)
+        assert np.abs(I1[k] - q1).max() <= 1e-8 * np.abs(q1).max(), (k, I1[k], q1)
+        assert abs(K1[k] - k1) <= 1e-8 * abs(k1), (k, K1[k], k1)
+        assert np.abs(K1v[k] - k1v).max() <= 1e-8 * np.abs(k1v).max(), (k, K1v[k], k1v)
+        # the vector integrals lie in the plane
+        assert abs(I1[k] @ nrm) <= 1e-14 * np.abs(I1[k]).max() + 1e-300
+        assert abs(K1v[k] @ nrm) <= 1e-14 * np.abs(K1v[k]).max() + 1e-300
+
+
+def _shells(scale):
+    """A coarse sphere and a scaled copy: cell q of the copy faces cell q of
+    the sphere across a gap of (scale - 1) times the radius."""
+    a = meshes.icosphere(1)
+    return a, meshes.Mesh(a.vertices * scale, a.faces)
+
+
+@pytest.mark.parametrize("scale", [1.02, 1.1])
+def test_near_static_vs_analytic(scale):
+    """gamma = 0, P1, a near pair: the row sums of the block are the outer
+    rule applied to the closed-form potential of the trial cell (exact up to
+    rounding), the column sums the outer rule applied to int_L lambda_j / R
+    by a composite Duffy rule."""
+    a, b = _shells(scale)
+    X, Y = spaces.lagrangec0d1(a), spaces.lagrangec0d1(b)
+    op = Helmholtz3D.singlelayer(gamma=0.0)
+    tv, sv = a.vertices[a.faces[0]], b.vertices[b.faces[0]]
+    Zo, cls = O.pair_block(op, X, Y, 0, 0)
+    assert cls == O.NEAR
+    b1, b2, w = ind.collapsed(4)
+    xs = ind.cell_points(tv, b1, b2)
+    wx = w * 2 * ind.area(tv)
+    lam = np.stack([b1, b2, 1 - b1 - b2], axis=1)
+    row = (wx * ind.triangle_potential(sv, xs)) @ lam / (4 * np.pi)
+    col = np.zeros(3)
+    for I, x in enumerate(xs):
+        P, W = ind.split_rule(sv, x, 40)
+        R = np.linalg.norm(P - x, axis=1)
+        col += wx[I] * (W / R) @ ind.barycentric(sv, P) / (4 * np.pi)
+    assert np.allclose(Zo.sum(axis=1).real, row, rtol=1e-13, atol=0), (Zo.sum(axis=1), row)
+    assert np.allclose(Zo.sum(axis=0).real, col, rtol=1e-11, atol=0), (Zo.sum(axis=0), col)
+    assert np.abs(Zo.imag).max() == 0.0
+
+
+def _near_reference(op, kind, tv, sv, n, inner):
+    """The near block with the outer collapsed n^2 rule on the test cell and
+    the inner integral by the composite Duffy rule (inner = "split") or by the
+    plain collapsed n^2 rule (inner = "plain"), in numpy."""
+    b1, b2, w = ind.collapsed(n)
+    xs = ind.cell_points(tv, b1, b2)
+    wx = w * 2 * ind.area(tv)
+    At, As = ind.area(tv), ind.area(sv)
+    Z = np.zeros((3, 3), dtype=complex)
+    for I, x in enumerate(xs):
+        if inner == "split":
+            P, W = ind.split_rule(sv, x, 40)
+        else:
+            P, W = ind.cell_points(sv, b1, b2), w * 2 * As
+        R = np.linalg.norm(P - x, axis=1)
+        G = W * np.exp(-op.gamma * R) / (4 * np.pi * R)
+        if kind == "hh3d":
+            fx = np.array([b1[I], b2[I], 1 - b1[I] - b2[I]])
+            Z += wx[I] * op.alpha * np.outer(fx, G @ ind.barycentric(sv, P))
+        else:
+            for i in range(3):
+                fi = (x - tv[i]) / (2 * At)
+                for j in range(3):
+                    gj = (P - sv[j]) / (2 * As)
+                    Z[i, j] += wx[I] * (op.alpha * (G @ (gj @ fi)) + op.beta * G.sum() / (At * As))
+    return Z
+
+
+@pytest.mark.parametrize("kind", ["efie", "efie-lossy", "hh3d"])
+def test_near_vs_bruteforce(kind):
+    """Near blocks against the same outer rule with the inner integral by a
+    composite Duffy rule: the subtraction leaves only the smooth remainder to
+    the inner rule, so the error is small at near_order 4 and falls fast with
+    the order, where the plain product rule is off by tens of percent for the
+    facing cells."""
+    a, b = _shells(1.02)
+    if kind == "efie":
+        op, X, Y = Maxwell3D.singlelayer(wavenumber=3.0), spaces.raviartthomas(a), spaces.raviartthomas(b)
+    elif kind == "efie-lossy":
+        op, X, Y = Maxwell3D.singlelayer(gamma=0.8 + 3.0j), spaces.raviartthomas(a), spaces.raviartthomas(b)
+    else:
+        op = Helmholtz3D.singlelayer(wavenumber=3.0, alpha=0.7 - 0.2j)
+        X, Y = spaces.lagrangec0d1(a), spaces.lagrangec0d1(b)
+    k = "hh3d" if kind == "hh3d" else "efie"
+    tv = a.vertices[a.faces[0]]
+    for q in (0, 1, 5):
+        sv = b.vertices[b.faces[q]]
+        for n, tol in ((4, 2e-5), (8, 1e-6)):
+            Zo, cls = O.pair_block(op, X, Y, 0, q, near_order=n)
+            assert cls == O.NEAR
+            Zr = _near_reference(op, k, tv, sv, n, "split")
+            err = np.abs(Zo - Zr).max() / np.abs(Zr).max()
+            assert err <= tol, (q, n, err)
+            if q == 0 and n == 4:
+                Zp = _near_reference(op, k, tv, sv, n, "plain")
+                assert np.abs(Zp - Zr).max() / np.abs(Zr).max() > 1e4 * err
+
+
 # ------------------------------------------------------------- classifier
 
 def test_classifier_euler_counts():
