@@ -19,13 +19,13 @@
 
 // ------------------------------------------------------------------ math
 
-// 1/sqrt(x) in FP64: hardware estimate plus two Newton steps.
+// 1/sqrt(x) in FP64: the hardware estimate y (relative error below 1e-7)
+// and one third-order correction: with e = 1 - x y^2,
+// 1/sqrt(x) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + O(e^3)).
 __device__ __forceinline__ double rsqrt64(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double hx = 0.5 * x;
-    y = y * fma(-hx * y, y, 1.5);
-    const double r = fma(-x * y, y, 1.0);
-    return fma(0.5 * y, r, y);
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 // sin and cos of 2*pi*t (t in turns), FP64.  Reduction to a quarter turn
@@ -84,23 +84,28 @@ __device__ __forceinline__ void sincos_turns32(float f, float &s, float &c) {
     c = ((qi + 1) & 2) ? -b : b;
 }
 
-// sin and cos of 2*pi*t, FP64, with a table of the 64 angles 2*pi*m/64:
-// t = m/64 + f exactly (|f| <= 1/128, Sterbenz), x = 2*pi*f, |x| <= pi/64,
-// Taylor polynomials to x^7 / x^8 (truncation below 5e-18), then the angle
-// addition formulas with tab[m mod 64] = (sin, cos).
-__device__ __forceinline__ void sincos_turns64_tab(double t, const double2 *tab, double &s, double &c) {
-    const double m = rint(64.0 * t);
-    const double f = fma(m, -1.0 / 64.0, t);
-    const double x = f * 6.283185307179586476925;
-    const double x2 = x * x;
-    double ps = fma(x2, 1.0 / 362880.0, -1.0 / 5040.0);
-    ps = fma(x2, ps, 1.0 / 120.0);
-    ps = fma(x2, ps, -1.0 / 6.0);
-    const double sx = fma(x * x2, ps, x);
-    double pc = fma(x2, 1.0 / 40320.0, -1.0 / 720.0);
-    pc = fma(x2, pc, 1.0 / 24.0);
-    pc = fma(x2, pc, -0.5);
-    const double cx = fma(x2, pc, 1.0);
+// sin and cos of 2*pi*u/64 (u in 64ths of a turn), FP64, with a table of the
+// 64 angles 2*pi*m/64: u = m + g exactly (m = rint(u), |g| <= 1/2), the angle
+// of g is x = (pi/32) g, |x| <= pi/64; Taylor polynomials to x^7 / x^8
+// (truncation below 5e-18) with the powers of pi/32 folded into the
+// coefficients, then the angle addition formulas with tab[m mod 64] = (sin, cos).
+__device__ __forceinline__ void sincos_64ths_tab(double u, const double2 *tab, double &s, double &c) {
+    constexpr double W = 0.098174770424681038701957605727484465;  // pi/32
+    constexpr double W2 = W * W;
+    constexpr double S1 = W, S3 = -W * W2 / 6.0, S5 = W * W2 * W2 / 120.0, S7 = -W * W2 * W2 * W2 / 5040.0;
+    constexpr double C2 = -W2 / 2.0, C4 = W2 * W2 / 24.0, C6 = -W2 * W2 * W2 / 720.0,
+                     C8 = W2 * W2 * W2 * W2 / 40320.0;
+    const double m = rint(u);
+    const double g = u - m;
+    const double g2 = g * g;
+    double ps = fma(g2, S7, S5);
+    ps = fma(g2, ps, S3);
+    ps = fma(g2, ps, S1);
+    const double sx = g * ps;
+    double pc = fma(g2, C8, C6);
+    pc = fma(g2, pc, C4);
+    pc = fma(g2, pc, C2);
+    const double cx = fma(g2, pc, 1.0);
     const double2 sc = tab[((int)m) & 63];
     s = fma(sc.x, cx, sc.y * sx);
     c = fma(sc.y, cx, -(sc.x * sx));
@@ -125,7 +130,7 @@ __device__ __forceinline__ void green64(const OpConst &op, bool lossy, double w4
     const double rinv = rsqrt64(R2);
     const double R = R2 * rinv;
     double s, c;
-    if (TAB) sincos_turns64_tab(R * op.k2pi, tab, s, c);
+    if (TAB) sincos_64ths_tab(R * (64.0 * op.k2pi), tab, s, c);
     else sincos_turns64(R * op.k2pi, s, c);
     double h = w4pi * rinv;
     if (lossy) h *= exp(-op.gamma_re * R);
@@ -150,7 +155,7 @@ __device__ __forceinline__ void green_smooth64(const OpConst &op, bool lossy, do
     const double rinv = rsqrt64(R2);
     const double R = R2 * rinv;
     double sh, ch;
-    if (TAB) sincos_turns64_tab(0.5 * R * op.k2pi, tab, sh, ch);
+    if (TAB) sincos_64ths_tab(R * (32.0 * op.k2pi), tab, sh, ch);
     else sincos_turns64(0.5 * R * op.k2pi, sh, ch);
     const double s = 2.0 * sh * ch, c1 = -2.0 * sh * sh;  // sin kR, cos kR - 1
     double re, im;
@@ -175,11 +180,16 @@ __device__ __forceinline__ void green_grad64(const OpConst &op, bool lossy, doub
     const double R2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double rinv = rsqrt64(R2);
     const double R = R2 * rinv;
-    const double tt = R * op.k2pi;
-    double s, c;
-    if (TAB) sincos_turns64_tab(tt, tab, s, c);
-    else sincos_turns64(tt, s, c);
-    const double kr = tt * 6.283185307179586476925;  // kappa R
+    double s, c, kr;  // kr = kappa R
+    if (TAB) {
+        const double u = R * (64.0 * op.k2pi);
+        sincos_64ths_tab(u, tab, s, c);
+        kr = u * (6.283185307179586476925 / 64.0);
+    } else {
+        const double tt = R * op.k2pi;
+        sincos_turns64(tt, s, c);
+        kr = tt * 6.283185307179586476925;
+    }
     double a = 1.0, h = -w4pi * rinv * rinv * rinv;
     if (lossy) {
         a = fma(op.gamma_re, R, 1.0);
