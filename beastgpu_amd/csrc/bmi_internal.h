@@ -68,6 +68,12 @@ struct RuleConst {
 #define BMI_GEO_AREA (BMI_GEO_CEN + 3 * BMI_PATCH)     // 16 areas (0 = no cell)
 #define BMI_GEO_VREL (BMI_GEO_AREA + BMI_PATCH)        // 16 x 9 vertices, relative to the centroid
 #define BMI_GEO (BMI_GEO_VREL + 9 * BMI_PATCH)         // 400
+// FP32 copy of the points and vertices in the front of a record's LDS (the
+// mixed far kernels): float index of the vertices; it ends before the
+// centroids, which stay FP64 in place
+#define BMI_GEOF_VREL (BMI_PATCH * 3 * BMI_MAX_RULE)
+static_assert((BMI_GEOF_VREL + 9 * BMI_PATCH) * sizeof(float) <= BMI_GEO_CEN * sizeof(double),
+              "FP32 geometry overlaps the centroids");
 
 // Far-pair assembly.  Test side: panels of BMI_PANEL_ROWS consecutive
 // functions, each with the cells that carry them (cells of several panels are

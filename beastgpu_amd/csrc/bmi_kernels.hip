@@ -206,11 +206,10 @@ __device__ __forceinline__ void green_grad32(const OpConst &op, bool lossy, floa
     const float R2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = __builtin_amdgcn_rsqf(R2);
     const float R = R2 * rinv;
-    const double tt = (double)R * op.k2pi;
-    const float f = (float)(tt - rint(tt));
+    const float f = (float)__builtin_amdgcn_fract((double)R * op.k2pi);  // turns, 0 <= f < 1
     const float s = __builtin_amdgcn_sinf(f);
     const float c = __builtin_amdgcn_cosf(f);
-    const float kr = (float)(tt * 6.283185307179586476925);
+    const float kr = R * (float)(op.k2pi * 6.283185307179586476925);
     float a = 1.0f, h = -w4pi * rinv * rinv * rinv;
     if (lossy) {
         a = fmaf((float)op.gamma_re, R, 1.0f);
@@ -311,9 +310,9 @@ __device__ __forceinline__ void hh_epilogue(const OpConst &op, const double *pr,
 // so eight moments of h = w w' q carry the pair; the 1/(2A) of each RWG
 // shape cancels the Jacobian 2A of the point weights.  S is the arithmetic
 // type (double, or float for the mixed path).
-template <typename S, int FAST, bool TAB>
-__device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule_, const double *xp, const double *xa,
-                                        const double *xc, const double *yp, const double *ya, const double *yc,
+template <typename S, int FAST, bool TAB, typename G = double>
+__device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule_, const G *xp, const G *xa,
+                                        const double *xc, const G *yp, const G *ya, const double *yc,
                                         double2 loc[9], const double2 *tab) {
     const FarRule<FAST> rule{rule_, op};
     const bool lossy = rule.lossy();
@@ -321,8 +320,8 @@ __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule
     // unit normal of the test cell, (v1 - v3) x (v2 - v3) direction
     double nx, ny, nz;
     {
-        const double e1x = xa[0] - xa[6], e1y = xa[1] - xa[7], e1z = xa[2] - xa[8];
-        const double e2x = xa[3] - xa[6], e2y = xa[4] - xa[7], e2z = xa[5] - xa[8];
+        const double e1x = (double)xa[0] - xa[6], e1y = (double)xa[1] - xa[7], e1z = (double)xa[2] - xa[8];
+        const double e2x = (double)xa[3] - xa[6], e2y = (double)xa[4] - xa[7], e2z = (double)xa[5] - xa[8];
         nx = e1y * e2z - e1z * e2y;
         ny = e1z * e2x - e1x * e2z;
         nz = e1x * e2y - e1y * e2x;
@@ -494,9 +493,8 @@ __device__ __forceinline__ void green32(const OpConst &op, bool lossy, float w4p
     const float R2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float rinv = __builtin_amdgcn_rsqf(R2);
     const float R = R2 * rinv;
-    const double tt = (double)R * op.k2pi;
-    const float f = (float)(tt - rint(tt));
-    // v_sin_f32 / v_cos_f32 take the angle in turns; |f| <= 1/2
+    // v_sin_f32 / v_cos_f32 take the angle in turns; 0 <= f < 1
+    const float f = (float)__builtin_amdgcn_fract((double)R * op.k2pi);
     const float s = __builtin_amdgcn_sinf(f);
     const float c = __builtin_amdgcn_cosf(f);
     float h = w4pi * rinv;
@@ -505,12 +503,14 @@ __device__ __forceinline__ void green32(const OpConst &op, bool lossy, float w4p
     hi = -h * s;
 }
 
-template <int OP, int FAST>
-__device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &rule_, const double *xp,
-                                           const double *xa, const double *xc, double xs, const double *yp,
-                                           const double *ya, const double *yc, double ys, double2 loc[9]) {
+// G: type of the rule points and vertices (float when the caller keeps an
+// FP32 copy of the geometry, double otherwise).
+template <int OP, int FAST, typename G = double>
+__device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &rule_, const G *xp,
+                                           const G *xa, const double *xc, double xs, const G *yp,
+                                           const G *ya, const double *yc, double ys, double2 loc[9]) {
     if constexpr (OP == OPK_DL) {
-        pair_dl<float, FAST, false>(op, rule_, xp, xa, xc, yp, ya, yc, loc, nullptr);
+        pair_dl<float, FAST, false, G>(op, rule_, xp, xa, xc, yp, ya, yc, loc, nullptr);
         return;
     }
     const FarRule<FAST> rule{rule_, op};
@@ -1576,6 +1576,12 @@ struct PanelShared {
     double2 s_sc[64];                  // sincos table (FP64 path)
 };
 
+// Tile entry (row r, column c) of a panel block: column-major, 64 rows per
+// column, the row index XOR-ed with c mod 16, so that the 16 lanes of a rank
+// round (one row, up to 16 columns) fall into different LDS banks while each
+// column stays one contiguous 1 KB segment (one wave's direct-to-LDS load).
+__device__ __forceinline__ int tile_ix(int r, int c) { return c * BMI_PANEL_ROWS + (r ^ (c & 15)); }
+
 // One (test panel P, trial patch q) block of the far-pair assembly with its
 // write-back; the caller has filled sh.s_sc (FP64 path) and passes the tile
 // of BMI_PANEL_ROWS x ncm entries.  Ends after the write-back stores are
@@ -1606,11 +1612,19 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
     const int kl = t >> 4, ll = t & 15;
     const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
 
-    for (int i = t; i < BMI_PANEL_ROWS * ncm; i += BMI_BLOCK_THREADS) tile[i] = make_double2(0.0, 0.0);
+    const int lane = t & 63, wv = t >> 6;
+    for (int c = wv; c < nc; c += BMI_BLOCK_THREADS / 64) tile[c * BMI_PANEL_ROWS + lane] = make_double2(0.0, 0.0);
+    // MIXED: the rule points and vertices of a geometry record are kept in
+    // FP32 in the front of the record's LDS (geo_put), centroids and areas in
+    // FP64 in place
+    auto geo_put = [&](double *rec, int i, double v) {
+        if (!MIXED || (i >= BMI_GEO_CEN && i < BMI_GEO_VREL)) rec[i] = v;
+        else ((float *)rec)[i < BMI_GEO_CEN ? i : BMI_GEOF_VREL + (i - BMI_GEO_VREL)] = (float)v;
+    };
     {
         const double *g = a.bgeo + (long long)q * BMI_GEO;
-        bg[t] = g[t];
-        if (t < BMI_GEO - BMI_BLOCK_THREADS) bg[BMI_BLOCK_THREADS + t] = g[BMI_BLOCK_THREADS + t];
+        geo_put(bg, t, g[t]);
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) geo_put(bg, BMI_BLOCK_THREADS + t, g[BMI_BLOCK_THREADS + t]);
         if (t < BMI_MAX_ROWS) {
             s_cfn[t] = a.bfn[q * BMI_MAX_ROWS + t];
             s_cfirst[t] = a.bfirst[q * BMI_MAX_ROWS + t];
@@ -1627,7 +1641,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
 
     // registers with the records of the next sub-batch
     double pg0 = 0.0, pg1 = 0.0, pcf = 0.0;
-    int psl = 0xff, pe0 = 0, pe1 = 0;
+    int psl = 0xff, pe0 = 0, pcnt = 0;
     auto fetch = [&](int s) {
         const double *g = a.tgeo + (long long)s * BMI_GEO;
         pg0 = g[t];
@@ -1638,11 +1652,11 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
         }
         const long long b = (long long)s * a.npb + q;
         pe0 = a.blk_off[b];
-        pe1 = pe0 + a.blk_cnt[b];
+        pcnt = a.blk_cnt[b];  // added to pe0 where it is used: no wait here
     };
     auto stash = [&](int buf) {
-        sg[buf][t] = pg0;
-        if (t < BMI_GEO - BMI_BLOCK_THREADS) sg[buf][BMI_BLOCK_THREADS + t] = pg1;
+        geo_put(sg[buf], t, pg0);
+        if (t < BMI_GEO - BMI_BLOCK_THREADS) geo_put(sg[buf], BMI_BLOCK_THREADS + t, pg1);
         if (t < BMI_MAX_ROWS) {
             s_tslot[buf][t] = psl;
             s_tscf[buf][t] = pcf;
@@ -1653,9 +1667,9 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
         fetch(sb0);
         stash(0);
         e0 = pe0;
-        e1 = min(pe1, a.capacity);
-        if (sb0 + 1 < sb1) fetch(sb0 + 1);
+        e1 = min(pe0 + pcnt, a.capacity);
     }
+    if (sb0 + 1 < sb1) fetch(sb0 + 1);
     __syncthreads();
 
     for (int s = sb0; s < sb1; ++s) {
@@ -1663,14 +1677,19 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
         double2 loc[9];
         const double xs = sg[cur][BMI_GEO_AREA + kl], ys = bg[BMI_GEO_AREA + ll];
         if (xs > 0.0 && ys > 0.0) {
-            const double *xp = &sg[cur][BMI_GEO_PTS + 3 * BMI_MAX_RULE * kl];
             const double *xc = &sg[cur][BMI_GEO_CEN + 3 * kl];
-            const double *xa = &sg[cur][BMI_GEO_VREL + 9 * kl];
-            const double *yp = &bg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * ll];
             const double *yc = &bg[BMI_GEO_CEN + 3 * ll];
-            const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
-            if (MIXED) pair_mixed<OP, FAST>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc);
-            else pair_fp64<OP, FAST, true>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
+            if (MIXED) {
+                const float *xf = (const float *)sg[cur], *yf = (const float *)bg;
+                pair_mixed<OP, FAST, float>(a.op, a.rule, xf + 3 * BMI_MAX_RULE * kl, xf + BMI_GEOF_VREL + 9 * kl, xc,
+                                            xs, yf + 3 * BMI_MAX_RULE * ll, yf + BMI_GEOF_VREL + 9 * ll, yc, ys, loc);
+            } else {
+                const double *xp = &sg[cur][BMI_GEO_PTS + 3 * BMI_MAX_RULE * kl];
+                const double *xa = &sg[cur][BMI_GEO_VREL + 9 * kl];
+                const double *yp = &bg[BMI_GEO_PTS + 3 * BMI_MAX_RULE * ll];
+                const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
+                pair_fp64<OP, FAST, true>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
+            }
         } else {
 #pragma unroll
             for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
@@ -1695,7 +1714,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
         if (s + 1 < sb1) {
             stash(cur ^ 1);
             e0 = pe0;
-            e1 = min(pe1, a.capacity);
+            e1 = min(pe0 + pcnt, a.capacity);
             if (s + 2 < sb1) fetch(s + 2);
         }
         // With one test rank (the round-robin sub-batches), different test
@@ -1714,7 +1733,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
                     for (int j = 0; j < 3; ++j) {
                         if ((bsl[j] & 0xff) == 0xff || (bsl[j] >> 8) != rb) continue;
                         const double cc = tcf[i] * bcf[j];
-                        double2 &z = tile[row * ncm + (bsl[j] & 0xff)];
+                        double2 &z = tile[tile_ix(row, bsl[j] & 0xff)];
                         z.x = fma(cc, loc[3 * i + j].x, z.x);
                         z.y = fma(cc, loc[3 * i + j].y, z.y);
                     }
@@ -1726,13 +1745,30 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
         }
     }
 
-    // write-back: consecutive threads take consecutive rows of one column
+    // write-back, one column per wave at a time, lane l taking position l of
+    // the column's segment (row l ^ (c mod 16)).  The stored values of all of
+    // the wave's read-modify-write columns are loaded first, so that the
+    // loads are in flight together, then added and stored.
     OutT *Z = (OutT *)a.Z;
-    for (int e = t; e < BMI_PANEL_ROWS * nc; e += BMI_BLOCK_THREADS) {
-        const int c = e / BMI_PANEL_ROWS, r = e - c * BMI_PANEL_ROWS;
-        if (r < nr) {
-            const double2 v = tile[r * ncm + c];
-            out_write(Z, (long long)(r0 + r) + (long long)s_cfn[c] * a.ld, v.x, v.y, s_cfirst[c]);
+    constexpr int NCW = (BMI_MAX_COLS + BMI_BLOCK_THREADS / 64 - 1) / (BMI_BLOCK_THREADS / 64);
+    OutT old[NCW];
+#pragma unroll
+    for (int k = 0; k < NCW; ++k) {
+        const int c = wv + k * (BMI_BLOCK_THREADS / 64);
+        const int r = lane ^ (c & 15);
+        old[k] = OutT{};
+        if (c < nc && r < nr && !s_cfirst[c]) old[k] = Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld];
+    }
+#pragma unroll
+    for (int k = 0; k < NCW; ++k) {
+        const int c = wv + k * (BMI_BLOCK_THREADS / 64);
+        const int r = lane ^ (c & 15);
+        if (c < nc && r < nr) {
+            const double2 v = tile[c * BMI_PANEL_ROWS + lane];
+            OutT o;
+            o.x = (decltype(o.x))((double)old[k].x + v.x);
+            o.y = (decltype(o.y))((double)old[k].y + v.y);
+            Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld] = o;
         }
     }
 }
