@@ -1203,28 +1203,35 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             const int nreg = cls == CLS_CV ? 2 : (cls == CLS_CE ? 5 : 6);
             // region loop outermost: the region map is uniform over the wave
             for (int r = 0; r < nreg; ++r) {
+                // U nodes per lane and iteration, so that independent
+                // evaluations overlap their latencies (the double layer's
+                // registers allow one); past the last node the extra ones
+                // repeat it with weight zero
+                constexpr int U = OP == OPK_DL ? 1 : 2;
 #pragma unroll 1
-                for (int node = lane; node < n4; node += 64) {
-                    const int i1 = node / (n2 * n), r1 = node - i1 * n2 * n;
-                    const int i2 = r1 / n2, r2 = r1 - i2 * n2;
-                    const int i3 = r2 / n, i4 = r2 - i3 * n;
-                    const double wgt = s_glw[g][i1] * s_glw[g][i2] * s_glw[g][i3] * s_glw[g][i4];
-                    double u1, u2, v1, v2, m;
-                    ss_map(cls, r, s_glx[g][i1], s_glx[g][i2], s_glx[g][i3], s_glx[g][i4], wgt, u1, u2, v1, v2, m);
-                    spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
+                for (int node = lane; node < n4; node += 64 * U) {
+#pragma unroll
+                    for (int h2 = 0; h2 < U; ++h2) {
+                        const int nd = min(node + 64 * h2, n4 - 1);
+                        const int i1 = nd / (n2 * n), r1 = nd - i1 * n2 * n;
+                        const int i2 = r1 / n2, r2 = r1 - i2 * n2;
+                        const int i3 = r2 / n, i4 = r2 - i3 * n;
+                        const double wgt = (node + 64 * h2 < n4 ? 1.0 : 0.0) * s_glw[g][i1] * s_glw[g][i2] *
+                                           s_glw[g][i3] * s_glw[g][i4];
+                        double u1, u2, v1, v2, m;
+                        ss_map(cls, r, s_glx[g][i1], s_glx[g][i2], s_glx[g][i3], s_glx[g][i4], wgt, u1, u2, v1, v2, m);
+                        spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
+                    }
                 }
             }
         }
-        // butterfly over the 64 lanes, one accumulator at a time (a
-        // scheduling barrier keeps the 18 exchanges from being interleaved,
-        // which would hold all their temporaries at once)
+        // butterfly over the 64 lanes, all accumulators at each step: six
+        // dependent rounds of lane exchanges in all (one accumulator after
+        // another would chain 108 exchanges, each waiting for the last)
 #pragma unroll
-        for (int i = 0; i < 18; ++i) {
-            double v = acc.v[i];
+        for (int o = 32; o >= 1; o >>= 1) {
 #pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-            acc.v[i] = v;
-            __builtin_amdgcn_sched_barrier(0);
+            for (int i = 0; i < 18; ++i) acc.v[i] += __shfl_xor(acc.v[i], o);
         }
         if (lane == 0) {
             double2 loc[9];
