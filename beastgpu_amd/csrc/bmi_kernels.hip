@@ -1066,6 +1066,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
     __shared__ double s_glx[3][BMI_MAX_GL], s_glw[3][BMI_MAX_GL];
     __shared__ double s_nb1[BMI_MAX_NEAR], s_nb2[BMI_MAX_NEAR], s_nw[BMI_MAX_NEAR];
     __shared__ double2 s_sc[64];
+    __shared__ double4 s_node[NG ? 64 * ((NG * NG * NG + 63) / 64) : 1];
     __shared__ int s_items[BMI_BLOCK_THREADS];
     __shared__ int s_wsum[BMI_BLOCK_THREADS / 64];
     const int t = threadIdx.x;
@@ -1083,6 +1084,19 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
     const int lane = t & 63, wv = t >> 6;
     const int ns = min(*a.n_special, a.capacity);
     const bool lossy = a.op.lossy != 0;
+    // compile-time order: node table over (i2, i3, i4), entry u = (x[i2],
+    // x[i3], x[i4], w[i2] w[i3] w[i4]) for u < NG^3, padded to whole waves
+    // with the last node at weight zero
+    constexpr int N3 = NG * NG * NG, NH3 = NG ? (N3 + 63) / 64 : 1;
+    if constexpr (NG != 0) {
+        for (int u = t; u < 64 * NH3; u += BMI_BLOCK_THREADS) {
+            const int nd = min(u, N3 - 1);
+            const int i2 = nd / (NG * NG), i3 = (nd / NG) % NG, i4 = nd % NG;
+            s_node[u] = make_double4(s_glx[0][i2], s_glx[0][i3], s_glx[0][i4],
+                                     (u < N3 ? 1.0 : 0.0) * s_glw[0][i2] * s_glw[0][i3] * s_glw[0][i4]);
+        }
+        __syncthreads();
+    }
     // one chunk of cs (<= 256) entries: the threads pick out the touching (or
     // near) entries the problem owns, then each wave integrates every fourth
     {
@@ -1196,9 +1210,33 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                 spec_point<OP, SUB>(a.op, lossy, s_sc, acc, X, Y, dc, s_nb1[I], s_nb2[I], s_nb1[J], s_nb2[J],
                                     s_nw[I] * s_nw[J], nrm);
             }
+        } else if constexpr (NG != 0) {
+            // compile-time order (every class the same Gauss-Legendre rule):
+            // lanes over the (i2, i3, i4) node table, the region and i1
+            // uniform over the wave; the Helmholtz kernel takes both halves of
+            // the table in one iteration (two independent evaluations; the
+            // other operators' registers are better spent on one)
+            const int nreg = cls == CLS_CV ? 2 : (cls == CLS_CE ? 5 : 6);
+            constexpr int U = OP == OPK_HH3D ? NH3 : 1;
+            for (int r = 0; r < nreg; ++r) {
+#pragma unroll 1
+                for (int i1 = 0; i1 < NG; ++i1) {
+                    const double e1 = s_glx[0][i1], w1 = s_glw[0][i1];
+#pragma unroll 1
+                    for (int h0 = 0; h0 < NH3; h0 += U) {
+#pragma unroll
+                        for (int h2 = 0; h2 < U; ++h2) {
+                            const double4 nd = s_node[lane + 64 * (h0 + h2)];
+                            double u1, u2, v1, v2, m;
+                            ss_map(cls, r, e1, nd.x, nd.y, nd.z, w1 * nd.w, u1, u2, v1, v2, m);
+                            spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
+                        }
+                    }
+                }
+            }
         } else {
             const int g = cls - 1;
-            const int n = NG ? NG : a.ss_n[g];
+            const int n = a.ss_n[g];
             const int n2 = n * n, n4 = n2 * n2;
             const int nreg = cls == CLS_CV ? 2 : (cls == CLS_CE ? 5 : 6);
             // region loop outermost: the region map is uniform over the wave
