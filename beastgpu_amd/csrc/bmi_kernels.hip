@@ -775,10 +775,13 @@ hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob
 
 // ------------------------------------------------------- special pairs
 
-// Accumulator of one lane for a special pair (moment form for RWG, 3x3
-// shape products for P1).
+// Accumulator of one lane for a special pair: moment form for RWG (16 of
+// the 18 for the single layer, 32 for the double layer), 3x3 shape products
+// for P1.
+constexpr int spec_nacc(int op) { return op == OPK_DL ? 32 : 18; }
+template <int OP>
 struct SpecAcc {
-    double v[18];
+    double v[spec_nacc(OP)];
 };
 
 
@@ -791,7 +794,7 @@ struct CellG {
 // SMOOTH: the smooth remainder of G (near pairs of the single layers) in
 // place of G.
 template <int OP, bool SMOOTH = false>
-__device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const double2 *tab, SpecAcc &acc, const CellG &X, const CellG &Y,
+__device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const double2 *tab, SpecAcc<OP> &acc, const CellG &X, const CellG &Y,
                                            const double *dc, double bx1, double bx2, double by1, double by2,
                                            double w, const double *nrm) {
     double x[3], y[3];
@@ -802,32 +805,38 @@ __device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const 
     }
     if constexpr (OP == OPK_DL) {
         // (n x a_c).(grad G x b_c') = q [(n.r)(a_c.b_c') - (n.b_c')(a_c.r)] for the
-        // chart vertices c, c' (a_c = x - chart vertex c, relative to the centroid)
+        // chart vertices c, c' (a_c = x - X_c, b_c' = y - Y_c', relative to the
+        // centroids); summed over the nodes it is a combination of eight
+        // moments of h = w q (dl_epilogue):
+        //   M1 = h n.r, M2 = h (n.r)(x.y), M3 = h (n.r) y, M4 = h (n.r) x,
+        //   M5 = h (n.y)(x.r), M6 = h (n.y) r, M7 = h x.r, M8 = h r
         const double r0 = dc[0] + x[0] - y[0], r1 = dc[1] + x[1] - y[1], r2 = dc[2] + x[2] - y[2];
         double hr, hi;
         green_grad64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), r0, r1, r2, hr, hi, tab);
         const double nr = nrm[0] * r0 + nrm[1] * r1 + nrm[2] * r2;
-        double a[3][3], b[3][3];
+        const double ny = nrm[0] * y[0] + nrm[1] * y[1] + nrm[2] * y[2];
+        const double xy = x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+        const double xr = x[0] * r0 + x[1] * r1 + x[2] * r2;
+        const double r[3] = {r0, r1, r2};
+        const double hnr = hr * nr, hni = hi * nr, hyr = hr * ny, hyi = hi * ny;
+        acc.v[0] += hnr;
+        acc.v[1] += hni;
+        acc.v[2] = fma(hnr, xy, acc.v[2]);
+        acc.v[3] = fma(hni, xy, acc.v[3]);
+        acc.v[4] = fma(hyr, xr, acc.v[4]);
+        acc.v[5] = fma(hyi, xr, acc.v[5]);
+        acc.v[6] = fma(hr, xr, acc.v[6]);
+        acc.v[7] = fma(hi, xr, acc.v[7]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            a[0][d] = x[d] - (X.o[d] + X.t1[d]);
-            a[1][d] = x[d] - (X.o[d] + X.t2[d]);
-            a[2][d] = x[d] - X.o[d];
-            b[0][d] = y[d] - (Y.o[d] + Y.t1[d]);
-            b[1][d] = y[d] - (Y.o[d] + Y.t2[d]);
-            b[2][d] = y[d] - Y.o[d];
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double ar = a[c][0] * r0 + a[c][1] * r1 + a[c][2] * r2;
-#pragma unroll
-            for (int c2 = 0; c2 < 3; ++c2) {
-                const double ab = a[c][0] * b[c2][0] + a[c][1] * b[c2][1] + a[c][2] * b[c2][2];
-                const double nb = nrm[0] * b[c2][0] + nrm[1] * b[c2][1] + nrm[2] * b[c2][2];
-                const double v = nr * ab - nb * ar;
-                acc.v[2 * (3 * c + c2)] = fma(hr, v, acc.v[2 * (3 * c + c2)]);
-                acc.v[2 * (3 * c + c2) + 1] = fma(hi, v, acc.v[2 * (3 * c + c2) + 1]);
-            }
+            acc.v[8 + d] = fma(hnr, y[d], acc.v[8 + d]);
+            acc.v[11 + d] = fma(hni, y[d], acc.v[11 + d]);
+            acc.v[14 + d] = fma(hnr, x[d], acc.v[14 + d]);
+            acc.v[17 + d] = fma(hni, x[d], acc.v[17 + d]);
+            acc.v[20 + d] = fma(hyr, r[d], acc.v[20 + d]);
+            acc.v[23 + d] = fma(hyi, r[d], acc.v[23 + d]);
+            acc.v[26 + d] = fma(hr, r[d], acc.v[26 + d]);
+            acc.v[29 + d] = fma(hi, r[d], acc.v[29 + d]);
         }
         return;
     }
@@ -973,7 +982,7 @@ __device__ __forceinline__ void tri_potentials64(const double (&w)[3][3], const 
 // form spec_point gives them (reference weights: sum_J nw_J f(y_J) stands for
 // int_L f / (2 A_L)).  The remainder is left to spec_point<OP, true>.
 template <int OP>
-__device__ __forceinline__ void near_static(const OpConst &op, SpecAcc &acc, const CellG &X, const CellG &Y,
+__device__ __forceinline__ void near_static(const OpConst &op, SpecAcc<OP> &acc, const CellG &X, const CellG &Y,
                                             const double *dc, double b1, double b2, double nw) {
     double x[3], xl[3], w[3][3], n[3];
 #pragma unroll
@@ -1193,9 +1202,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             nrm[1] *= ninv;
             nrm[2] *= ninv;
         }
-        SpecAcc acc;
+        SpecAcc<OP> acc;
 #pragma unroll
-        for (int i = 0; i < 18; ++i) acc.v[i] = 0.0;
+        for (int i = 0; i < spec_nacc(OP); ++i) acc.v[i] = 0.0;
 
         if constexpr (NEARK) {
             // single layers: singular terms in closed form, one outer point
@@ -1265,11 +1274,12 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
         }
         // butterfly over the 64 lanes, all accumulators at each step: six
         // dependent rounds of lane exchanges in all (one accumulator after
-        // another would chain 108 exchanges, each waiting for the last)
+        // another would chain six exchanges per accumulator, each waiting
+        // for the last)
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
 #pragma unroll
-            for (int i = 0; i < 18; ++i) acc.v[i] += __shfl_xor(acc.v[i], o);
+            for (int i = 0; i < spec_nacc(OP); ++i) acc.v[i] += __shfl_xor(acc.v[i], o);
         }
         if (lane == 0) {
             double2 loc[9];
@@ -1292,16 +1302,52 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                     }
                 mw_epilogue(a.op, mm, xa, ya, loc);
             } else {
-                // accumulators are in chart order: (c, c') belongs to the
-                // local shapes (PT[c], PS[c'])
+                // entry (c, c') in chart order belongs to the local shapes
+                // (PT[c], PS[c'])
                 double pr[9], pi[9];
+                if constexpr (OP == OPK_DL) {
+                    // chart vertices relative to the centroids
+                    double Xc[3][3], Yc[3][3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int c2 = 0; c2 < 3; ++c2) {
-                        pr[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2)];
-                        pi[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2) + 1];
+                    for (int d = 0; d < 3; ++d) {
+                        Xc[0][d] = X.o[d] + X.t1[d];
+                        Xc[1][d] = X.o[d] + X.t2[d];
+                        Xc[2][d] = X.o[d];
+                        Yc[0][d] = Y.o[d] + Y.t1[d];
+                        Yc[1][d] = Y.o[d] + Y.t2[d];
+                        Yc[2][d] = Y.o[d];
                     }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        // terms that depend on c only
+                        double tr = acc.v[2] - acc.v[4], ti = acc.v[3] - acc.v[5];
+                        double ur[3], ui[3], qr = acc.v[6], qi = acc.v[7];
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            tr += Xc[c][d] * (acc.v[20 + d] - acc.v[8 + d]);
+                            ti += Xc[c][d] * (acc.v[23 + d] - acc.v[11 + d]);
+                            ur[d] = acc.v[14 + d] - Xc[c][d] * acc.v[0];
+                            ui[d] = acc.v[17 + d] - Xc[c][d] * acc.v[1];
+                            qr -= Xc[c][d] * acc.v[26 + d];
+                            qi -= Xc[c][d] * acc.v[29 + d];
+                        }
+#pragma unroll
+                        for (int c2 = 0; c2 < 3; ++c2) {
+                            const double nY = nrm[0] * Yc[c2][0] + nrm[1] * Yc[c2][1] + nrm[2] * Yc[c2][2];
+                            // M2 - X.M3 - Y.M4 + (X.Y) M1 - M5 + X.M6 + (n.Y)(M7 - X.M8)
+                            pr[3 * PT[c] + PS[c2]] = tr - (Yc[c2][0] * ur[0] + Yc[c2][1] * ur[1] + Yc[c2][2] * ur[2]) + nY * qr;
+                            pi[3 * PT[c] + PS[c2]] = ti - (Yc[c2][0] * ui[0] + Yc[c2][1] * ui[1] + Yc[c2][2] * ui[2]) + nY * qi;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int c2 = 0; c2 < 3; ++c2) {
+                            pr[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2)];
+                            pi[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2) + 1];
+                        }
+                }
                 // Helmholtz: Jacobian 4 A_K A_L; double layer: it cancels the
                 // 1/(2A) of the two RWG shapes
                 hh_epilogue(a.op, pr, pi, OP == OPK_DL ? 1.0 : 4.0 * a.X.area[K] * a.Y.area[L], loc);
