@@ -311,6 +311,7 @@ struct bmi_plan {
     int cs = 256, nchunks = 0;
     int64_t pairs[6] = {0, 0, 0, 0, 0, 0};
     bool pairs_ready = false;
+    mutable bool filled = false;  // the special-pair entries have been written
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bmi_stats last{};
 };
@@ -586,11 +587,17 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
             int n = 0;
             while (i < nt && n < BMI_PATCH) {
                 const int K = order[i];
-                for (int j = 0; j < 3; ++j)
-                    if (gidx[3 * (int64_t)K + j] >= 0) fs.push_back(gidx[3 * (int64_t)K + j]);
-                std::sort(fs.begin(), fs.end());
-                fs.erase(std::unique(fs.begin(), fs.end()), fs.end());
-                if (n > 0 && (int)fs.size() > BMI_MAX_COLS) break;
+                // the functions this cell would add to the run
+                int64_t add[3];
+                int na = 0;
+                for (int j = 0; j < 3; ++j) {
+                    const int64_t f = gidx[3 * (int64_t)K + j];
+                    if (f >= 0 && std::find(fs.begin(), fs.end(), f) == fs.end() &&
+                        std::find(add, add + na, f) == add + na)
+                        add[na++] = f;
+                }
+                if (n > 0 && (int)fs.size() + na > BMI_MAX_COLS) break;
+                fs.insert(fs.end(), add, add + na);
                 s->patch_tri.push_back(K);
                 ++n;
                 ++i;
@@ -752,39 +759,6 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
     return BMI_OK;
 }
 
-// Packed geometry (BMI_GEO doubles) of groups of BMI_PATCH cells.
-std::vector<double> geo_records(const bmi_mesh *m, const std::vector<double> &pts, const std::vector<int> &cells,
-                                int ngroups) {
-    std::vector<double> g((size_t)ngroups * BMI_GEO, 0.0);
-    for (int b = 0; b < ngroups; ++b) {
-        double *r = &g[(size_t)b * BMI_GEO];
-        for (int c = 0; c < BMI_PATCH; ++c) {
-            const int K = cells[(size_t)b * BMI_PATCH + c];
-            if (K < 0) continue;
-            for (int e = 0; e < 3 * BMI_MAX_RULE; ++e)
-                r[BMI_GEO_PTS + 3 * BMI_MAX_RULE * c + e] = pts[(size_t)K * 3 * BMI_MAX_RULE + e];
-            for (int d = 0; d < 3; ++d) r[BMI_GEO_CEN + 3 * c + d] = m->cen[3 * (size_t)K + d];
-            r[BMI_GEO_AREA + c] = m->area[K];
-            for (int e = 0; e < 9; ++e) r[BMI_GEO_VREL + 9 * c + e] = m->vtx[9 * (size_t)K + e] - m->cen[3 * (size_t)K + e % 3];
-        }
-    }
-    return g;
-}
-
-std::vector<double> rule_points(const bmi_mesh *m, int n, const double l[BMI_MAX_RULE][3]) {
-    std::vector<double> pts((size_t)m->nt * 3 * BMI_MAX_RULE, 0.0);
-    for (int64_t k = 0; k < m->nt; ++k) {
-        const double *v = &m->vtx[9 * k];
-        for (int I = 0; I < n; ++I)
-            for (int d = 0; d < 3; ++d) {
-                // x = v3 + b1 (v1 - v3) + b2 (v2 - v3), relative to the centroid
-                const double x = v[6 + d] + l[I][0] * (v[d] - v[6 + d]) + l[I][1] * (v[3 + d] - v[6 + d]);
-                pts[(size_t)k * 3 * BMI_MAX_RULE + 3 * I + d] = x - m->cen[3 * k + d];
-            }
-    }
-    return pts;
-}
-
 }  // namespace
 
 hipStream_t context_stream(const bmi_context *ctx) { return ctx->stream; }
@@ -853,6 +827,7 @@ int gl5(const bmi_plan *P) {
 bmi_status enqueue_prologue(bmi_plan *P, bmi_matrix *out, hipStream_t s, hipEvent_t ev_mid) {
     BMI_HIP(launch_zero_ints(P->d_total.p, 1, s));
     BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, s));
+    P->filled = true;
     if (ev_mid) BMI_HIP(hipEventRecord(ev_mid, s));
     const int mixed = P->opts.precision == BMI_PREC_MIXED;
     BMI_HIP(launch_special(P->d_sargs.p, P->d_cbase.p, P->d_ebase.p, 1, P->nchunks, P->capacity, P->cs, mixed,
@@ -1157,12 +1132,22 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
         for (int J = 0; J < BMI_MAX_RULE; ++J) R.w4pi[I][J] = wt[I] * wb[J] / (4.0 * kPi);
 
     hipError_t e;
+    hipStream_t s = ctx->stream;
+    // rule points of every cell and the packed geometry records of the test
+    // sub-batches and trial patches, built on the device from the meshes
+    if ((e = P->d_xpts.alloc((size_t)test->mesh->nt * 3 * BMI_MAX_RULE)) != hipSuccess ||
+        (e = P->d_ypts.alloc((size_t)trial->mesh->nt * 3 * BMI_MAX_RULE)) != hipSuccess ||
+        (e = P->d_tgeo.alloc((size_t)test->nsb * BMI_GEO)) != hipSuccess ||
+        (e = P->d_bgeo.alloc((size_t)trial->npatch * BMI_GEO)) != hipSuccess)
+        return fail(BMI_ENOMEM, std::string("rule points: ") + hipGetErrorString(e));
     {
-        const std::vector<double> xp = rule_points(test->mesh, nt, lt), yp = rule_points(trial->mesh, nb, lb);
-        if ((e = P->d_xpts.upload(xp)) != hipSuccess || (e = P->d_ypts.upload(yp)) != hipSuccess ||
-            (e = P->d_tgeo.upload(geo_records(test->mesh, xp, test->sb_cells, test->nsb))) != hipSuccess ||
-            (e = P->d_bgeo.upload(geo_records(trial->mesh, yp, trial->patch_tri, trial->npatch))) != hipSuccess)
-            return fail(BMI_EDEVICE, std::string("rule points: ") + hipGetErrorString(e));
+        const bmi_mesh *mt = test->mesh, *mb = trial->mesh;
+        BMI_HIP(launch_rule_points(mt->d_vtx.p, mt->d_cen.p, mt->nt, nt, lt, P->d_xpts.p, s));
+        BMI_HIP(launch_rule_points(mb->d_vtx.p, mb->d_cen.p, mb->nt, nb, lb, P->d_ypts.p, s));
+        BMI_HIP(launch_geo_records(P->d_xpts.p, mt->d_vtx.p, mt->d_cen.p, mt->d_area.p, test->d_sb_cells.p, test->nsb,
+                                   P->d_tgeo.p, s));
+        BMI_HIP(launch_geo_records(P->d_ypts.p, mb->d_vtx.p, mb->d_cen.p, mb->d_area.p, trial->d_patch_tri.p,
+                                   trial->npatch, P->d_bgeo.p, s));
     }
 
     P->npa = test->nsb;
@@ -1188,7 +1173,6 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     C.counter = P->d_total.p;
 
     // first classification: size the special-pair buffers
-    hipStream_t s = ctx->stream;
     {
         const std::vector<int> gb = {0, P->npa};
         const std::vector<ClassifyArgs> ca = {C};
@@ -1217,8 +1201,8 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
         if ((e = P->d_cargs.upload(ca)) != hipSuccess)
             return fail(BMI_EDEVICE, std::string("plan arguments: ") + hipGetErrorString(e));
     }
-    BMI_HIP(launch_zero_ints(P->d_total.p, 1, s));
-    BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, s));
+    // the entries themselves are written by each assembly's classification
+    // (or by bmi_plan_stats if it comes first)
     SpecialArgs &S = P->sargs;
     S.X = C.X;
     S.Y = C.Y;
@@ -1311,6 +1295,11 @@ bmi_status bmi_plan_stats(const bmi_plan *Pc, bmi_stats *st) {
         // has one owner entry (the other entries of a cell repeated in
         // several panels carry SPEC_NOT_OWNER); the rest are far pairs
         BMI_HIP(hipSetDevice(P->ctx->device));
+        if (!P->filled) {
+            BMI_HIP(launch_zero_ints(P->d_total.p, 1, P->ctx->stream));
+            BMI_HIP(launch_classify(P->d_cargs.p, P->d_gbase.p, 1, P->npa, P->ctx->stream));
+            P->filled = true;
+        }
         std::vector<unsigned char> cls(std::max(P->capacity, 1), 0);
         if (P->capacity > 0) {
             BMI_HIP(hipMemcpyAsync(cls.data(), P->d_spec_cls.p, P->capacity, hipMemcpyDeviceToHost, P->ctx->stream));
@@ -1610,6 +1599,7 @@ bmi_status enqueue_batch_prologue(bmi_batch *Bt, hipStream_t s) {
     const int n = (int)Bt->plans.size();
     BMI_HIP(launch_zero_each(Bt->d_counters.p, n, s));
     BMI_HIP(launch_classify(Bt->d_cargs.p, Bt->d_gbase.p, n, Bt->total_groups, s));
+    for (const bmi_plan *P : Bt->plans) P->filled = true;
     for (auto &G : Bt->opgroups)
         BMI_HIP(launch_special(G.d_sargs.p, G.d_cbase.p, G.d_ebase.p, (int)G.problems.size(), G.total_chunks,
                                G.total_entries, G.cs, G.mixed, G.op_kind, G.ng5, s));

@@ -157,6 +157,10 @@ struct SpecialArgs {
 // kernel launchers (bmi_kernels.hip)
 // classification, one problem or a batch: work-group g takes sub-batch
 // g - gbase[i] of problem i (gbase: nprob + 1 prefix of the sub-batch counts)
+hipError_t launch_rule_points(const double *vtx, const double *cen, long long nt, int n, const double l[][3],
+                              double *pts, hipStream_t s);
+hipError_t launch_geo_records(const double *pts, const double *vtx, const double *cen, const double *area,
+                              const int *cells, int ngroups, double *g, hipStream_t s);
 hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, hipStream_t s);
 hipError_t launch_zero_each(int *const *ptrs, int n, hipStream_t s);
 // special pairs, one problem or a batch of one operator: cbase / ebase are

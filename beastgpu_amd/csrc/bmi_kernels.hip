@@ -767,6 +767,76 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
     }
 }
 
+// ------------------------------------------------- plan geometry on device
+
+// Far-rule points of every cell relative to its centroid,
+// x = v3 + b1 (v1 - v3) + b2 (v2 - v3) (assamble_chunk_gpu.jl:75 chart);
+// thread = (cell, point, coordinate).  Products are not fused, so the points
+// equal a host evaluation of the same expression.
+__global__ void k_rule_points(const double *vtx, const double *cen, long long nt, int n, double l0, double l1,
+                              double l2, double l3, double l4, double l5, double l6, double l7, double *pts) {
+#pragma clang fp contract(off)
+    const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nt * 3 * BMI_MAX_RULE) return;
+    const long long k = u / (3 * BMI_MAX_RULE);
+    const int I = (int)(u % (3 * BMI_MAX_RULE)) / 3, d = (int)(u % 3);
+    if (I >= n) {
+        pts[u] = 0.0;
+        return;
+    }
+    const double b1[BMI_MAX_RULE] = {l0, l2, l4, l6}, b2[BMI_MAX_RULE] = {l1, l3, l5, l7};
+    const double *v = vtx + 9 * k;
+    const double x = v[6 + d] + b1[I] * (v[d] - v[6 + d]) + b2[I] * (v[3 + d] - v[6 + d]);
+    pts[u] = x - cen[3 * k + d];
+}
+
+// Packed geometry records (BMI_GEO doubles) of groups of BMI_PATCH cells:
+// rule points, centroids, areas, vertices relative to the centroid; zero for
+// empty slots.  One work-group per group.
+__global__ void k_geo_records(const double *pts, const double *vtx, const double *cen, const double *area,
+                              const int *cells, int ngroups, double *g) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.x;
+    if (b >= ngroups) return;
+    for (int i = threadIdx.x; i < BMI_GEO; i += blockDim.x) {
+        double v = 0.0;
+        int c, e;
+        if (i < BMI_GEO_CEN) { c = i / (3 * BMI_MAX_RULE); e = i % (3 * BMI_MAX_RULE); }
+        else if (i < BMI_GEO_AREA) { c = (i - BMI_GEO_CEN) / 3; e = (i - BMI_GEO_CEN) % 3; }
+        else if (i < BMI_GEO_VREL) { c = i - BMI_GEO_AREA; e = 0; }
+        else { c = (i - BMI_GEO_VREL) / 9; e = (i - BMI_GEO_VREL) % 9; }
+        const long long K = cells[(long long)b * BMI_PATCH + c];
+        if (K >= 0) {
+            if (i < BMI_GEO_CEN) v = pts[K * 3 * BMI_MAX_RULE + e];
+            else if (i < BMI_GEO_AREA) v = cen[3 * K + e];
+            else if (i < BMI_GEO_VREL) v = area[K];
+            else v = vtx[9 * K + e] - cen[3 * K + e % 3];
+        }
+        g[(long long)b * BMI_GEO + i] = v;
+    }
+}
+
+hipError_t launch_rule_points(const double *vtx, const double *cen, long long nt, int n, const double l[][3],
+                              double *pts, hipStream_t s) {
+    const long long tot = nt * 3 * BMI_MAX_RULE;
+    if (tot <= 0) return hipSuccess;
+    double b[8] = {};
+    for (int I = 0; I < n && I < BMI_MAX_RULE; ++I) {
+        b[2 * I] = l[I][0];
+        b[2 * I + 1] = l[I][1];
+    }
+    hipLaunchKernelGGL(k_rule_points, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, vtx, cen, nt, n, b[0], b[1],
+                       b[2], b[3], b[4], b[5], b[6], b[7], pts);
+    return hipGetLastError();
+}
+
+hipError_t launch_geo_records(const double *pts, const double *vtx, const double *cen, const double *area,
+                              const int *cells, int ngroups, double *g, hipStream_t s) {
+    if (ngroups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_geo_records, dim3((unsigned)ngroups), dim3(256), 0, s, pts, vtx, cen, area, cells, ngroups, g);
+    return hipGetLastError();
+}
+
 hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, hipStream_t s) {
     if (total_groups <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_classify, dim3((unsigned)total_groups), dim3(BMI_BLOCK_THREADS), 0, s, args, gbase, nprob);
