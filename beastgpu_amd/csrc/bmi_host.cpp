@@ -5,6 +5,7 @@
 #include "bmi_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -376,6 +377,26 @@ void group_sphere(const bmi_mesh *m, const int *cells, int n, double *sph, doubl
     sph[3] = std::sqrt(r2) * (1.0 + 1e-12);
 }
 
+// f(0) ... f(n - 1) over up to 8 host threads (the calling thread and
+// others started for the call), in chunks of 8 indices taken in turn.
+template <class F>
+void parallel_for(int n, F f) {
+    const int nth = (int)std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 64 || nth <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i; (i = next.fetch_add(8)) < n;)
+            for (int j = i; j < std::min(n, i + 8); ++j) f(j);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+}
+
 // Panels: functions [64 P, 64 P + 64) and every cell carrying one of them,
 // ordered by Morton rank, cut into sub-batches of BMI_PATCH cells.  For each
 // sub-batch and each of the 64 rows: the slots (local cell * 3 + shape,
@@ -413,14 +434,17 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
             }
         }
     }
-    s->sb_off.assign(np + 1, 0);
-    s->sb_cells.clear();
-    s->sb_row_ptr.clear();
-    s->sb_slot_kl.clear();
-    s->sb_slot_cf.clear();
+    // per panel (in parallel): its sub-batches' cells and their slot lists
+    // by row; then concatenated in panel order
     struct Slot { int r, kl; double cf; };
-    std::vector<Slot> sl;
-    for (int P = 0; P < np; ++P) {
+    struct PanelOut {
+        std::vector<int> cells, row_ptr;  // row_ptr relative to the panel's slots
+        std::vector<unsigned char> kl;
+        std::vector<double> cf;
+    };
+    std::vector<PanelOut> outs(np);
+    parallel_for(np, [&](int P) {
+        PanelOut &o = outs[P];
         std::sort(cells.begin() + cnt[P], cells.begin() + cnt[P + 1],
                   [&](int a, int b) { return morton_rank[a] < morton_rank[b]; });
         // Round-robin packing: the panel's cells, in Morton order, are dealt
@@ -457,12 +481,12 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
             if (ok) break;
         }
         if (ncell == 0) bins.clear();
-        const int nsb = (int)bins.size();
-        for (int b = 0; b < nsb; ++b) {
+        std::vector<Slot> sl;
+        for (size_t b = 0; b < bins.size(); ++b) {
             sl.clear();
             for (int k = 0; k < BMI_PATCH; ++k) {
                 const int K = k < (int)bins[b].size() ? bins[b][k] : -1;
-                s->sb_cells.push_back(K);
+                o.cells.push_back(K);
                 if (K < 0) continue;
                 for (int i = 0; i < 3; ++i) {
                     const int64_t f = gidx[3 * (int64_t)K + i];
@@ -475,20 +499,35 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
             });
             size_t u = 0;
             for (int r = 0; r <= R; ++r) {
-                s->sb_row_ptr.push_back((int)s->sb_slot_kl.size());
+                o.row_ptr.push_back((int)o.kl.size());
                 while (u < sl.size() && sl[u].r == r) {
-                    s->sb_slot_kl.push_back((unsigned char)sl[u].kl);
-                    s->sb_slot_cf.push_back(sl[u].cf);
+                    o.kl.push_back((unsigned char)sl[u].kl);
+                    o.cf.push_back(sl[u].cf);
                     ++u;
                 }
             }
         }
-        s->sb_off[P + 1] = s->sb_off[P] + nsb;
+    });
+    s->sb_off.assign(np + 1, 0);
+    s->sb_cells.clear();
+    s->sb_row_ptr.clear();
+    s->sb_slot_kl.clear();
+    s->sb_slot_cf.clear();
+    for (int P = 0; P < np; ++P) {
+        const PanelOut &o = outs[P];
+        const int base = (int)s->sb_slot_kl.size();
+        s->sb_cells.insert(s->sb_cells.end(), o.cells.begin(), o.cells.end());
+        for (int v : o.row_ptr) s->sb_row_ptr.push_back(base + v);
+        s->sb_slot_kl.insert(s->sb_slot_kl.end(), o.kl.begin(), o.kl.end());
+        s->sb_slot_cf.insert(s->sb_slot_cf.end(), o.cf.begin(), o.cf.end());
+        s->sb_off[P + 1] = s->sb_off[P] + (int)(o.cells.size() / BMI_PATCH);
     }
     s->nsb = s->sb_off[np];
     s->sb_sph.assign((size_t)s->nsb * 4, 0.0);
     s->sb_amax.assign(s->nsb, 0.0);
-    for (int b = 0; b < s->nsb; ++b) group_sphere(m, &s->sb_cells[(size_t)b * BMI_PATCH], BMI_PATCH, &s->sb_sph[4 * b], &s->sb_amax[b]);
+    parallel_for(s->nsb, [&](int b) {
+        group_sphere(m, &s->sb_cells[(size_t)b * BMI_PATCH], BMI_PATCH, &s->sb_sph[4 * b], &s->sb_amax[b]);
+    });
     // fixed-size slot records per sub-batch
     std::vector<int> trs((size_t)s->nsb * (R + 1), 0);
     std::vector<unsigned char> tkl((size_t)s->nsb * BMI_MAX_ROWS, 0);
@@ -610,36 +649,41 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
 
     s->patch_sph.assign((size_t)np * 4, 0.0);
     s->patch_amax.assign(np, 0.0);
-    for (int p = 0; p < np; ++p)
+    parallel_for(np, [&](int p) {
         group_sphere(m, &s->patch_tri[(size_t)p * BMI_PATCH], BMI_PATCH, &s->patch_sph[4 * p], &s->patch_amax[p]);
+    });
 
+    // per patch (in parallel): its slots sorted by function; then
+    // concatenated in patch order
+    struct Slot { int64_t fn; int kl; double cf; };
+    std::vector<std::vector<Slot>> psl(np);
+    parallel_for(np, [&](int p) {
+        std::vector<Slot> &sl = psl[p];
+        for (int k = 0; k < BMI_PATCH; ++k) {
+            const int K = s->patch_tri[(size_t)p * BMI_PATCH + k];
+            if (K < 0) continue;
+            for (int i = 0; i < 3; ++i) {
+                const int64_t fn = gidx[3 * (int64_t)K + i];
+                if (fn >= 0) sl.push_back({fn, 3 * k + i, coeff[3 * (int64_t)K + i]});
+            }
+        }
+        std::stable_sort(sl.begin(), sl.end(), [](const Slot &a, const Slot &b) {
+            return a.fn != b.fn ? a.fn < b.fn : a.kl < b.kl;
+        });
+    });
     s->row_ptr.assign(np + 1, 0);
     s->row_fn.clear();
     s->row_slot_ptr.clear();
     s->slot_kl.clear();
     s->slot_cf.clear();
     std::vector<char> covered(s->nfun, 0);
-    struct Slot { int64_t fn; int kl; double cf; };
-    std::vector<Slot> sl;
     for (int p = 0; p < np; ++p) {
-        sl.clear();
-        for (int k = 0; k < BMI_PATCH; ++k) {
-            const int K = s->patch_tri[(size_t)p * BMI_PATCH + k];
-            if (K < 0) continue;
-            for (int i = 0; i < 3; ++i) {
-                const int64_t fn = gidx[3 * (int64_t)K + i];
-                if (fn < 0) continue;
-                sl.push_back({fn, 3 * k + i, coeff[3 * (int64_t)K + i]});
-                covered[fn] = 1;
-            }
-        }
-        std::stable_sort(sl.begin(), sl.end(), [](const Slot &a, const Slot &b) {
-            return a.fn != b.fn ? a.fn < b.fn : a.kl < b.kl;
-        });
+        const std::vector<Slot> &sl = psl[p];
         for (size_t u = 0; u < sl.size(); ++u) {
             if (u == 0 || sl[u].fn != sl[u - 1].fn) {
                 s->row_fn.push_back((int)sl[u].fn);
                 s->row_slot_ptr.push_back((int)s->slot_kl.size());
+                covered[sl[u].fn] = 1;
             }
             s->slot_kl.push_back((unsigned char)sl[u].kl);
             s->slot_cf.push_back(sl[u].cf);
