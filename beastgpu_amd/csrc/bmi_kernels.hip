@@ -2175,16 +2175,25 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_apply(const T *Z, long lo
     double ar[BMI_MAX_RHS], ai[BMI_MAX_RHS];
 #pragma unroll
     for (int r = 0; r < BMI_MAX_RHS; ++r) ar[r] = ai[r] = 0.0;
-    for (int c = c0; c < c1; ++c) {
-        const T z = Z[row + (long long)c * ld];
-        const double zr = (double)z.x, zi = (double)z.y;
-        const double2 *xc = &sx[(c - c0) * nrhs];
+    // eight columns' entries are loaded before any is used, so that each
+    // wave keeps 8 KB of the matrix in flight
+    constexpr int U = 8;
+    for (int cb = c0; cb < c1; cb += U) {
+        T zz[U];
 #pragma unroll
-        for (int r = 0; r < BMI_MAX_RHS; ++r) {
-            if (r >= nrhs) break;
-            const double2 v = xc[r];
-            ar[r] = fma(zr, v.x, fma(-zi, v.y, ar[r]));
-            ai[r] = fma(zr, v.y, fma(zi, v.x, ai[r]));
+        for (int u = 0; u < U; ++u) zz[u] = cb + u < c1 ? Z[row + (long long)(cb + u) * ld] : T{};
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (cb + u >= c1) break;
+            const double zr = (double)zz[u].x, zi = (double)zz[u].y;
+            const double2 *xc = &sx[(cb + u - c0) * nrhs];
+#pragma unroll
+            for (int r = 0; r < BMI_MAX_RHS; ++r) {
+                if (r >= nrhs) break;
+                const double2 v = xc[r];
+                ar[r] = fma(zr, v.x, fma(-zi, v.y, ar[r]));
+                ai[r] = fma(zr, v.y, fma(zi, v.x, ai[r]));
+            }
         }
     }
     for (int r = 0; r < nrhs; ++r)
