@@ -152,7 +152,9 @@ bmi_status bmi_matrix_device_ptr(const bmi_matrix *m, void **ptr, int64_t *ld);
  * Replaces assemble_gpu's compute path (assamble_gpu.jl:25-87). */
 bmi_status bmi_assemble(bmi_plan *plan, bmi_matrix *out);
 
-/* Timings and pair counts of the last bmi_assemble on this plan. */
+/* Timings of the last bmi_assemble on this plan (zero before the first) and
+ * the pair counts by class (the first request before any assembly runs the
+ * classification once to obtain them). */
 bmi_status bmi_plan_stats(const bmi_plan *plan, bmi_stats *stats);
 
 /* Copy a device matrix to caller-owned host memory as C64 or C32.
