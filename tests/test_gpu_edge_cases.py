@@ -98,3 +98,21 @@ def test_boundary_errors(gpu_ctx):
         a.matrix.apply(np.zeros((RWG.nfun, 9)))
     wrong.close()
     a.close()
+
+
+def test_stats_before_and_after_the_first_assembly(gpu_ctx):
+    """Pair counts asked for before any assembly (the plan then classifies
+    once) equal the oracle's and the counts after assembling."""
+    m = meshes.ellipsoid(cells=(16, 10, 10))
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.singlelayer(wavenumber=3.0)
+    names = ("far", "common_vertex", "common_edge", "common_face", "near")
+    print("case stats before assembly", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(), ctx=gpu_ctx)
+    before = [a.stats()["pairs"][n] for n in names]
+    Z = a.assemble().to_host()
+    after = [a.stats()["pairs"][n] for n in names]
+    a.close()
+    Zr, counts = O.assemble(op, X, X)
+    assert before == after == list(counts)
+    assert np.linalg.norm(Z - Zr) <= 1e-12 * np.linalg.norm(Zr)
