@@ -1884,19 +1884,49 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
         // the trial-rank rounds need no barrier, as one wave's LDS accesses
         // are performed in program order.  More test ranks put colliding
         // rows in different waves: a barrier after each test round.
+        // Within a round a thread's entries are distinct and no other thread
+        // touches them, so the mixed kernel (which has the registers) issues
+        // all of its reads before its writes: one LDS wait per round instead
+        // of one per entry; unused slots read entry 0 and write nothing.
         for (int ra = 0; ra < a.rounds_t; ++ra) {
             for (int rb = 0; rb < a.rounds_b; ++rb) {
+                if constexpr (MIXED && FAST) {
+                    int ix[9];
+                    bool act[9];
+                    double2 z[9];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    if ((tsl[i] & 0xff) == 0xff || (tsl[i] >> 8) != ra) continue;
-                    const int row = tsl[i] & 0xff;
+                    for (int i = 0; i < 3; ++i)
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        if ((bsl[j] & 0xff) == 0xff || (bsl[j] >> 8) != rb) continue;
-                        const double cc = tcf[i] * bcf[j];
-                        double2 &z = tile[tile_ix(row, bsl[j] & 0xff)];
-                        z.x = fma(cc, loc[3 * i + j].x, z.x);
-                        z.y = fma(cc, loc[3 * i + j].y, z.y);
+                        for (int j = 0; j < 3; ++j) {
+                            act[3 * i + j] = (tsl[i] & 0xff) != 0xff && (tsl[i] >> 8) == ra &&
+                                             (bsl[j] & 0xff) != 0xff && (bsl[j] >> 8) == rb;
+                            ix[3 * i + j] = act[3 * i + j] ? tile_ix(tsl[i] & 0xff, bsl[j] & 0xff) : 0;
+                            z[3 * i + j] = tile[ix[3 * i + j]];
+                        }
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            if (!act[3 * i + j]) continue;
+                            const double cc = tcf[i] * bcf[j];
+                            double2 v = z[3 * i + j];
+                            v.x = fma(cc, loc[3 * i + j].x, v.x);
+                            v.y = fma(cc, loc[3 * i + j].y, v.y);
+                            tile[ix[3 * i + j]] = v;
+                        }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        if ((tsl[i] & 0xff) == 0xff || (tsl[i] >> 8) != ra) continue;
+                        const int row = tsl[i] & 0xff;
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            if ((bsl[j] & 0xff) == 0xff || (bsl[j] >> 8) != rb) continue;
+                            const double cc = tcf[i] * bcf[j];
+                            double2 &z = tile[tile_ix(row, bsl[j] & 0xff)];
+                            z.x = fma(cc, loc[3 * i + j].x, z.x);
+                            z.y = fma(cc, loc[3 * i + j].y, z.y);
+                        }
                     }
                 }
             }
