@@ -1735,6 +1735,7 @@ struct PanelShared {
     int s_cfn[BMI_MAX_ROWS];
     unsigned char s_cfirst[BMI_MAX_ROWS];
     double2 s_sc[64];                  // sincos table (FP64 path)
+    short s_ent[BMI_BLOCK_THREADS];    // special entry of each thread's pair (offset in the block), -1 none
 };
 
 // Tile entry (row r, column c) of a panel block: column-major, 64 rows per
@@ -1775,6 +1776,8 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
 
     const int lane = t & 63, wv = t >> 6;
     for (int c = wv; c < nc; c += BMI_BLOCK_THREADS / 64) tile[c * BMI_PANEL_ROWS + lane] = make_double2(0.0, 0.0);
+    short *s_ent = sh.s_ent;
+    s_ent[t] = -1;
     // MIXED: the rule points and vertices of a geometry record are kept in
     // FP32 in the front of the record's LDS (geo_put), centroids and areas in
     // FP64 in place
@@ -1836,6 +1839,10 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
     for (int s = sb0; s < sb1; ++s) {
         const int cur = (s - sb0) & 1;
         double2 loc[9];
+        // the block's special entries (at most one per thread, in thread
+        // order): each thread's entry by its pair, in LDS
+        const bool has_spec = e1 > e0;  // uniform over the block
+        if (has_spec && t < e1 - e0) s_ent[a.spec_loc[e0 + t]] = (short)t;
         const double xs = sg[cur][BMI_GEO_AREA + kl], ys = bg[BMI_GEO_AREA + ll];
         if (xs > 0.0 && ys > 0.0) {
             const double *xc = &sg[cur][BMI_GEO_CEN + 3 * kl];
@@ -1855,12 +1862,16 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
 #pragma unroll
             for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
         }
-        // special pairs of this block take their precomputed values (the
-        // block's entries are few; every thread scans them for its slot)
-        for (int e = e0; e < e1; ++e) {
-            if (a.spec_loc[e] == t) {
+        // special pairs of this block take their precomputed values; the
+        // slot is cleared for the next sub-batch's entries (written after
+        // the round barrier below)
+        if (has_spec) {
+            __syncthreads();
+            const int o = s_ent[t];
+            if (o >= 0) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) loc[v] = a.spec_val[9 * (long long)e + v];
+                for (int v = 0; v < 9; ++v) loc[v] = a.spec_val[9 * (long long)(e0 + o) + v];
+                s_ent[t] = -1;
             }
         }
         int tsl[3];
