@@ -1253,6 +1253,7 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     S.spec_K = P->d_spec_K.p;
     S.spec_L = P->d_spec_L.p;
     S.spec_cls = P->d_spec_cls.p;
+    S.spec_loc = P->d_spec_loc.p;
     S.n_special = P->d_total.p;
     S.blk_off = P->d_blk_off.p;
     S.blk_cnt = P->d_blk_cnt.p;

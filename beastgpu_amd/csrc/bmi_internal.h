@@ -141,6 +141,7 @@ struct SpecialArgs {
     DevSpace X, Y;
     const int *spec_K, *spec_L;
     const unsigned char *spec_cls;
+    const unsigned char *spec_loc;  // pair's thread in its block (kl * 16 + ll)
     const int *n_special;
     const int *blk_off, *blk_cnt;
     int npb;

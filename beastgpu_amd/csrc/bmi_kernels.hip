@@ -1479,14 +1479,23 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *a
         const int e = u - ebase[pi];
         if (e >= ns || !(a.spec_cls[e] & SPEC_NOT_OWNER)) return;
         const int K = a.spec_K[e], L = a.spec_L[e];
-        const long long b = (long long)a.X.home[K] * a.npb + a.Y.home[L];
+        const int hs = a.X.home[K];
+        const long long b = (long long)hs * a.npb + a.Y.home[L];
         const int o0 = a.blk_off[b], o1 = min(o0 + a.blk_cnt[b], a.capacity);
-        int o = -1;
-        for (int u = o0; u < o1; ++u)
-            if (a.spec_K[u] == K && a.spec_L[u] == L) {
-                o = u;
-                break;
-            }
+        // a block's entries are in the order of their pair's thread
+        // kl * 16 + ll; L sits in one patch, so ll is this entry's, and kl is
+        // K's place in its home sub-batch: binary search for that thread
+        int kl = 0;
+        for (int k = 0; k < BMI_PATCH; ++k)
+            if (a.X.patch_tri[(long long)hs * BMI_PATCH + k] == K) kl = k;
+        const int want = kl * BMI_PATCH + (a.spec_loc[e] & (BMI_PATCH - 1));
+        int lo = o0, hi = o1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a.spec_loc[mid] < want) lo = mid + 1;
+            else hi = mid;
+        }
+        const int o = lo < o1 && a.spec_K[lo] == K && a.spec_L[lo] == L ? lo : -1;
 #pragma unroll
         for (int v = 0; v < 9; ++v)
             a.spec_val[9 * (long long)e + v] =
