@@ -14,6 +14,7 @@
 #include "bmi_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 #define DTOL (2.220446049250313e-16 * 1.0e3)
 
@@ -1295,9 +1296,11 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             // uniform over the wave; the Helmholtz kernel takes both halves of
             // the table in one iteration (two independent evaluations; the
             // other operators' registers are better spent on one)
-            const int nreg = cls == CLS_CV ? 2 : (cls == CLS_CE ? 5 : 6);
+            // Each (class, region) loop is its own code, the region map
+            // folded in at compile time (no per-node dispatch).
             constexpr int U = OP == OPK_HH3D ? NH3 : 1;
-            for (int r = 0; r < nreg; ++r) {
+            auto region = [&](auto cls_c, auto r_c) {
+                constexpr int C = decltype(cls_c)::value, RR = decltype(r_c)::value;
 #pragma unroll 1
                 for (int i1 = 0; i1 < NG; ++i1) {
                     const double e1 = s_glx[0][i1], w1 = s_glw[0][i1];
@@ -1307,11 +1310,37 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                         for (int h2 = 0; h2 < U; ++h2) {
                             const double4 nd = s_node[lane + 64 * (h0 + h2)];
                             double u1, u2, v1, v2, m;
-                            ss_map(cls, r, e1, nd.x, nd.y, nd.z, w1 * nd.w, u1, u2, v1, v2, m);
+                            ss_map(C, RR, e1, nd.x, nd.y, nd.z, w1 * nd.w, u1, u2, v1, v2, m);
                             spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
                         }
                     }
                 }
+            };
+            using I0 = std::integral_constant<int, 0>;
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            using I3 = std::integral_constant<int, 3>;
+            using I4 = std::integral_constant<int, 4>;
+            using I5 = std::integral_constant<int, 5>;
+            using CCV = std::integral_constant<int, CLS_CV>;
+            using CCE = std::integral_constant<int, CLS_CE>;
+            using CCF = std::integral_constant<int, CLS_CF>;
+            if (cls == CLS_CV) {
+                region(CCV{}, I0{});
+                region(CCV{}, I1{});
+            } else if (cls == CLS_CE) {
+                region(CCE{}, I0{});
+                region(CCE{}, I1{});
+                region(CCE{}, I2{});
+                region(CCE{}, I3{});
+                region(CCE{}, I4{});
+            } else {
+                region(CCF{}, I0{});
+                region(CCF{}, I1{});
+                region(CCF{}, I2{});
+                region(CCF{}, I3{});
+                region(CCF{}, I4{});
+                region(CCF{}, I5{});
             }
         } else {
             const int g = cls - 1;
