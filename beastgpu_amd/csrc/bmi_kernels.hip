@@ -1938,6 +1938,9 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
         // all of its reads before its writes: one LDS wait per round instead
         // of one per entry; unused slots read entry 0 and write nothing.
         for (int ra = 0; ra < a.rounds_t; ++ra) {
+            bool tact[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tact[i] = (tsl[i] & 0xff) != 0xff && (tsl[i] >> 8) == ra;
             for (int rb = 0; rb < a.rounds_b; ++rb) {
                 if constexpr (MIXED && FAST) {
                     int ix[9];
@@ -1964,15 +1967,17 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
                             tile[ix[3 * i + j]] = v;
                         }
                 } else {
+                    // trial slot outermost: one rank test per column and
+                    // round (the test slots' tests do not depend on the round)
 #pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        if ((tsl[i] & 0xff) == 0xff || (tsl[i] >> 8) != ra) continue;
-                        const int row = tsl[i] & 0xff;
+                    for (int j = 0; j < 3; ++j) {
+                        if ((bsl[j] & 0xff) == 0xff || (bsl[j] >> 8) != rb) continue;
+                        const int col = bsl[j] & 0xff;
 #pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            if ((bsl[j] & 0xff) == 0xff || (bsl[j] >> 8) != rb) continue;
+                        for (int i = 0; i < 3; ++i) {
+                            if (!tact[i]) continue;
                             const double cc = tcf[i] * bcf[j];
-                            double2 &z = tile[tile_ix(row, bsl[j] & 0xff)];
+                            double2 &z = tile[tile_ix(tsl[i] & 0xff, col)];
                             z.x = fma(cc, loc[3 * i + j].x, z.x);
                             z.y = fma(cc, loc[3 * i + j].y, z.y);
                         }
