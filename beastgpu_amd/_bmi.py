@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbmi.so")
 
-OK, EINVAL, ENOMEM, EDEVICE, EBUSY, ECANCELED, EUNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
+OK, EINVAL, ENOMEM, EDEVICE, EBUSY, ECANCELED, EUNSUPPORTED, EINTERNAL = 0, -1, -2, -3, -4, -5, -6, -7
 SPACE_LAGRANGE_C0D1, SPACE_RWG = 1, 2
 OP_HH3D_SINGLELAYER, OP_MW_SINGLELAYER, OP_MW_DOUBLELAYER = 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1

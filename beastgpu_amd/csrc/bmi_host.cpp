@@ -14,6 +14,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -35,6 +36,26 @@ bmi_status fail(bmi_status s, const std::string &msg) {
 }  // namespace
 
 bmi_status set_error(bmi_status s, const std::string &msg) { return fail(s, msg); }
+
+// Status for an exception caught at the C boundary (called inside a catch
+// block): out of host memory or threads -> BMI_ENOMEM, else BMI_EINTERNAL.
+bmi_status fail_exception() noexcept {
+    try {
+        try {
+            throw;
+        } catch (const std::bad_alloc &) {
+            return fail(BMI_ENOMEM, "host memory allocation failed");
+        } catch (const std::system_error &e) {
+            return fail(BMI_ENOMEM, std::string("host resources: ") + e.what());
+        } catch (const std::exception &e) {
+            return fail(BMI_EINTERNAL, std::string("internal error: ") + e.what());
+        } catch (...) {
+            return fail(BMI_EINTERNAL, "internal error");
+        }
+    } catch (...) {
+        return BMI_ENOMEM;  // recording the message failed too
+    }
+}
 
 // Device memory comes from the device's stream-ordered pool, allocated and
 // freed on a library stream of its own per device (it carries nothing else):
@@ -952,7 +973,7 @@ void bmi_options_default(bmi_options *o) {
     o->near_factor = 2.0;
 }
 
-bmi_status bmi_context_create(int32_t device, bmi_context **out) {
+bmi_status bmi_context_create(int32_t device, bmi_context **out) try {
     if (!out) return fail(BMI_EINVAL, "out is NULL");
     int n = 0;
     BMI_HIP(hipGetDeviceCount(&n));
@@ -969,9 +990,11 @@ bmi_status bmi_context_create(int32_t device, bmi_context **out) {
     }
     *out = ctx.release();
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_device_memory(const bmi_context *ctx, int64_t *free_bytes, int64_t *total_bytes) {
+bmi_status bmi_device_memory(const bmi_context *ctx, int64_t *free_bytes, int64_t *total_bytes) try {
     if (!ctx || !free_bytes || !total_bytes) return fail(BMI_EINVAL, "NULL argument");
     BMI_HIP(hipSetDevice(ctx->device));
     size_t f = 0, t = 0;
@@ -980,9 +1003,11 @@ bmi_status bmi_device_memory(const bmi_context *ctx, int64_t *free_bytes, int64_
     *free_bytes = (int64_t)f;
     *total_bytes = (int64_t)t;
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_context_destroy(bmi_context *ctx) {
+bmi_status bmi_context_destroy(bmi_context *ctx) try {
     if (!ctx) return BMI_OK;
     (void)hipSetDevice(ctx->device);
     for (int i = 0; i < 2; ++i) {
@@ -993,10 +1018,12 @@ bmi_status bmi_context_destroy(bmi_context *ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
 bmi_status bmi_mesh_create(bmi_context *ctx, const double *xyz, int64_t nv, const int32_t *tri, int64_t nt,
-                           bmi_mesh **out) {
+                           bmi_mesh **out) try {
     if (!ctx || !xyz || !tri || !out) return fail(BMI_EINVAL, "NULL argument");
     if (nv < 3 || nt < 1 || nt > (1LL << 30)) return fail(BMI_EINVAL, "bad mesh size");
     BMI_HIP(hipSetDevice(ctx->device));
@@ -1041,18 +1068,22 @@ bmi_status bmi_mesh_create(bmi_context *ctx, const double *xyz, int64_t nv, cons
         return fail(BMI_EDEVICE, std::string("mesh upload: ") + hipGetErrorString(e));
     *out = m.release();
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_mesh_free(bmi_mesh *m) {
+bmi_status bmi_mesh_free(bmi_mesh *m) try {
     if (m) {
         (void)hipSetDevice(m->ctx->device);
         delete m;
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
 bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, const double *coeff, int64_t nfun,
-                            bmi_space **out) {
+                            bmi_space **out) try {
     if (!mesh || !gidx || !coeff || !out) return fail(BMI_EINVAL, "NULL argument");
     if (kind != BMI_SPACE_LAGRANGE_C0D1 && kind != BMI_SPACE_RWG) return fail(BMI_EUNSUPPORTED, "unknown space kind");
     if (nfun < 1 || nfun > (1LL << 31) - 1) return fail(BMI_EINVAL, "bad number of functions");
@@ -1073,17 +1104,21 @@ bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, c
     if (st != BMI_OK) return st;
     *out = s.release();
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_space_free(bmi_space *s) {
+bmi_status bmi_space_free(bmi_space *s) try {
     if (s) {
         (void)hipSetDevice(s->mesh->ctx->device);
         delete s;
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_matrix_create(bmi_context *ctx, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out) {
+bmi_status bmi_matrix_create(bmi_context *ctx, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out) try {
     if (!ctx || !out) return fail(BMI_EINVAL, "NULL argument");
     if (rows < 1 || cols < 1) return fail(BMI_EINVAL, "bad matrix size");
     if (dtype != BMI_C64 && dtype != BMI_C32) return fail(BMI_EINVAL, "bad dtype");
@@ -1100,26 +1135,32 @@ bmi_status bmi_matrix_create(bmi_context *ctx, int64_t rows, int64_t cols, int32
     if (e != hipSuccess) return fail(BMI_ENOMEM, std::string("matrix allocation: ") + hipGetErrorString(e));
     *out = m.release();
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_matrix_free(bmi_matrix *m) {
+bmi_status bmi_matrix_free(bmi_matrix *m) try {
     if (m) {
         (void)hipSetDevice(m->ctx->device);
         if (m->owned) dev_free(m->d);
         delete m;
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_matrix_device_ptr(const bmi_matrix *m, void **ptr, int64_t *ld) {
+bmi_status bmi_matrix_device_ptr(const bmi_matrix *m, void **ptr, int64_t *ld) try {
     if (!m) return fail(BMI_EINVAL, "NULL matrix");
     if (ptr) *ptr = m->d;
     if (ld) *ld = m->rows;
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
 bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_space *test, const bmi_space *trial,
-                           const bmi_options *opts, bmi_plan **out) {
+                           const bmi_options *opts, bmi_plan **out) try {
     if (!ctx || !op || !test || !trial || !out) return fail(BMI_EINVAL, "NULL argument");
     if (test->mesh->ctx != ctx || trial->mesh->ctx != ctx) return fail(BMI_EINVAL, "spaces belong to another context");
     bmi_options o;
@@ -1292,9 +1333,11 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     for (int i = 0; i < 4; ++i) BMI_HIP(hipEventCreate(&P->ev[i]));
     *out = P.release();
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_plan_free(bmi_plan *P) {
+bmi_status bmi_plan_free(bmi_plan *P) try {
     if (P) {
         (void)hipSetDevice(P->ctx->device);
         for (auto &e : P->ev)
@@ -1302,9 +1345,11 @@ bmi_status bmi_plan_free(bmi_plan *P) {
         delete P;
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
+bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) try {
     if (!P || !out) return fail(BMI_EINVAL, "NULL argument");
     bmi_status st = check_target(P, out);
     if (st != BMI_OK) return st;
@@ -1330,9 +1375,11 @@ bmi_status bmi_assemble(bmi_plan *P, bmi_matrix *out) {
     P->last.n_colours_test = 1;
     P->last.n_colours_trial = P->Y->ncolour;
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_plan_stats(const bmi_plan *Pc, bmi_stats *st) {
+bmi_status bmi_plan_stats(const bmi_plan *Pc, bmi_stats *st) try {
     if (!Pc || !st) return fail(BMI_EINVAL, "NULL argument");
     bmi_plan *P = const_cast<bmi_plan *>(Pc);
     if (!P->pairs_ready) {
@@ -1363,6 +1410,8 @@ bmi_status bmi_plan_stats(const bmi_plan *Pc, bmi_stats *st) {
     st->n_colours_test = 1;
     st->n_colours_trial = P->Y->ncolour;
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
 // Host copy of a source chunk into the destination, converting the element
@@ -1396,7 +1445,7 @@ static void consume_chunk(const void *src, int src32, void *dst, int dst32, size
     for (auto &x : th) x.join();
 }
 
-bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype) {
+bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype) try {
     if (!m || !dst) return fail(BMI_EINVAL, "NULL argument");
     if (dtype != BMI_C64 && dtype != BMI_C32) return fail(BMI_EINVAL, "bad dtype");
     bmi_context *ctx = m->ctx;
@@ -1429,14 +1478,18 @@ bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype) {
         consume_chunk(ctx->stage[k & 1], src32, dst, dtype == BMI_C32, o, c);
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_matrix_apply(const bmi_matrix *m, const double *x, double *y, int32_t nrhs) {
+bmi_status bmi_matrix_apply(const bmi_matrix *m, const double *x, double *y, int32_t nrhs) try {
     if (!m) return fail(BMI_EINVAL, "NULL argument");
     return matrix_apply_on(m->ctx, m, x, y, nrhs);
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double out[3]) {
+bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double out[3]) try {
     if (!a || !ref || !out) return fail(BMI_EINVAL, "NULL argument");
     if (a->rows != ref->rows || a->cols != ref->cols) return fail(BMI_EINVAL, "matrix shapes differ");
     if (a->ctx != ref->ctx) return fail(BMI_EINVAL, "matrices belong to different contexts");
@@ -1462,6 +1515,8 @@ bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double
     out[1] = std::sqrt(r2);
     out[2] = std::sqrt(dm);
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
 }  // extern "C"
@@ -1752,7 +1807,7 @@ void batch_destroy(bmi_batch *Bt) {
 extern "C" {
 
 bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix *const *matrices, int32_t n,
-                            bmi_batch **out) {
+                            bmi_batch **out) try {
     if (!ctx || !plans || !matrices || !out || n < 1) return fail(BMI_EINVAL, "bad argument");
     *out = nullptr;
     for (int i = 0; i < n; ++i) {
@@ -1819,9 +1874,11 @@ bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix
     if (st != BMI_OK) return st;
     *out = Bt.release();
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_batch_assemble(bmi_batch *Bt) {
+bmi_status bmi_batch_assemble(bmi_batch *Bt) try {
     if (!Bt) return fail(BMI_EINVAL, "NULL argument");
     BMI_HIP(hipSetDevice(Bt->ctx->device));
     hipStream_t s = Bt->ctx->stream;
@@ -1838,9 +1895,11 @@ bmi_status bmi_batch_assemble(bmi_batch *Bt) {
             if (st != BMI_OK) return st;
         }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_batch_stats_get(const bmi_batch *Bt, bmi_batch_stats *st) {
+bmi_status bmi_batch_stats_get(const bmi_batch *Bt, bmi_batch_stats *st) try {
     if (!Bt || !st) return fail(BMI_EINVAL, "NULL argument");
     *st = bmi_batch_stats{};
     st->last_ms = Bt->last_ms;
@@ -1854,14 +1913,18 @@ bmi_status bmi_batch_stats_get(const bmi_batch *Bt, bmi_batch_stats *st) {
         st->bucket_tuned_ms[b] = Bt->buckets[b].tuned_ms;
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_batch_free(bmi_batch *Bt) {
+bmi_status bmi_batch_free(bmi_batch *Bt) try {
     if (Bt) {
         (void)hipSetDevice(Bt->ctx->device);
         batch_destroy(Bt);
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
 }  // extern "C"

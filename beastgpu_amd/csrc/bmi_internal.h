@@ -204,5 +204,7 @@ hipError_t dev_alloc(void **p, size_t bytes);
 void dev_free(void *p);
 hipError_t dev_upload(void *dst, const void *src, size_t bytes);
 int32_t set_error(int32_t status, const std::string &msg);
+// status for an exception caught at the C boundary (call inside a catch block)
+int32_t fail_exception() noexcept;
 int32_t matrix_apply_on(bmi_context *run, const bmi_matrix *m, const double *x, double *y, int32_t nrhs);
 int32_t matrix_wrap(bmi_context *ctx, void *mem, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out);

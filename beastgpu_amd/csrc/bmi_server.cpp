@@ -238,7 +238,12 @@ void worker(bmi_server *S, Worker *W) {
         const int64_t cap0 = W->cap;
         lk.unlock();
         bmi_matrix *kept = nullptr;
-        const int32_t st = run(S, *W, *r, &kept);
+        int32_t st;
+        try {
+            st = run(S, *W, *r, &kept);
+        } catch (...) {
+            st = fail_exception();
+        }
         const std::string msg = st == BMI_OK ? std::string() : std::string(bmi_last_error());
         lk.lock();
         if (r->kind == BMI_REQ_APPLY) {
@@ -276,7 +281,7 @@ void worker(bmi_server *S, Worker *W) {
 
 extern "C" {
 
-bmi_status bmi_server_create(int32_t device, const bmi_server_options *opts, bmi_server **out) {
+bmi_status bmi_server_create(int32_t device, const bmi_server_options *opts, bmi_server **out) try {
     if (!out) return set_error(BMI_EINVAL, "out is NULL");
     *out = nullptr;
     const int slots = opts ? opts->slots : 4;
@@ -302,9 +307,11 @@ bmi_status bmi_server_create(int32_t device, const bmi_server_options *opts, bmi
     for (auto &W : S->w) S->workers.emplace_back(worker, S.get(), W.get());
     *out = S.release();
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_server_destroy(bmi_server *S) {
+bmi_status bmi_server_destroy(bmi_server *S) try {
     if (!S) return BMI_OK;
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -321,9 +328,11 @@ bmi_status bmi_server_destroy(bmi_server *S) {
     }
     delete S;
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_server_submit(bmi_server *S, const bmi_request *q, double *y, uint64_t *id) {
+bmi_status bmi_server_submit(bmi_server *S, const bmi_request *q, double *y, uint64_t *id) try {
     if (!S || !q || !id) return set_error(BMI_EINVAL, "NULL argument");
     auto r = std::make_shared<Request>();
     r->kind = q->kind;
@@ -376,9 +385,11 @@ bmi_status bmi_server_submit(bmi_server *S, const bmi_request *q, double *y, uin
     }
     S->cv_work.notify_all();
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_server_wait(bmi_server *S, uint64_t id, int32_t timeout_ms, bmi_result *res) {
+bmi_status bmi_server_wait(bmi_server *S, uint64_t id, int32_t timeout_ms, bmi_result *res) try {
     if (!S || !res) return set_error(BMI_EINVAL, "NULL argument");
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->pending.find(id);
@@ -391,9 +402,11 @@ bmi_status bmi_server_wait(bmi_server *S, uint64_t id, int32_t timeout_ms, bmi_r
     *res = r->res;
     S->pending.erase(id);
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_server_release(bmi_server *S, uint64_t id) {
+bmi_status bmi_server_release(bmi_server *S, uint64_t id) try {
     if (!S) return set_error(BMI_EINVAL, "NULL argument");
     std::lock_guard<std::mutex> lk(S->mu);
     auto m = S->resident_m.find(id);
@@ -407,14 +420,18 @@ bmi_status bmi_server_release(bmi_server *S, uint64_t id) {
         S->cv_work.notify_all();
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
-bmi_status bmi_server_memory(const bmi_server *S, int64_t *resident, int64_t *peak) {
+bmi_status bmi_server_memory(const bmi_server *S, int64_t *resident, int64_t *peak) try {
     if (!S || !resident || !peak) return set_error(BMI_EINVAL, "NULL argument");
     std::lock_guard<std::mutex> lk(const_cast<bmi_server *>(S)->mu);
     *resident = S->resident;
     *peak = S->peak;
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
 }
 
 }  // extern "C"

@@ -33,6 +33,7 @@ typedef int32_t bmi_status;
 #define BMI_EBUSY (-4)
 #define BMI_ECANCELED (-5)
 #define BMI_EUNSUPPORTED (-6)
+#define BMI_EINTERNAL (-7)  /* unexpected failure inside the library (bmi_last_error says which) */
 
 /* function spaces (BEAST lagrangec0d1 / raviartthomas) */
 #define BMI_SPACE_LAGRANGE_C0D1 1
