@@ -634,10 +634,13 @@ __device__ __forceinline__ int find_problem(const int *base, int n, int u) {
 // Work-group g takes sub-batch g - gbase[i] of problem i (one problem, or
 // every problem of a batch).
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyArgs *args, const int *gbase, int nprob) {
-    __shared__ double tv[BMI_PATCH][9], bv[BMI_PATCH][9];
-    __shared__ double tc[BMI_PATCH][3], bc[BMI_PATCH][3];
-    __shared__ double barea[BMI_PATCH];
-    __shared__ int tt[BMI_PATCH], bt[BMI_PATCH];
+    // candidate trial patches are taken CG at a time: their cells' data is
+    // loaded together, so a sub-batch waits for one round of loads per group
+    constexpr int CG = 4;
+    __shared__ double tv[BMI_PATCH][9], bv[CG][BMI_PATCH][9];
+    __shared__ double tc[BMI_PATCH][3], bc[CG][BMI_PATCH][3];
+    __shared__ double barea[CG][BMI_PATCH];
+    __shared__ int tt[BMI_PATCH], bt[CG][BMI_PATCH];
     __shared__ int nearq[BMI_BLOCK_THREADS];
     __shared__ int wsum[BMI_BLOCK_THREADS / 64];
     __shared__ int s_off;
@@ -686,24 +689,35 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
         if (cand) nearq[base + __popcll(m & ((1ull << lane) - 1ull))] = q;
         __syncthreads();
 
-        for (int u = 0; u < ncand; ++u) {
-            const int qq = nearq[u];
+        for (int u0 = 0; u0 < ncand; u0 += CG) {
+          const int ng = min(CG, ncand - u0);
+          if (t < CG * BMI_PATCH) {
+              const int g = t / BMI_PATCH, c = t - BMI_PATCH * g;
+              bt[g][c] = g < ng ? a.Y.patch_tri[nearq[u0 + g] * BMI_PATCH + c] : -1;
+          }
+          __syncthreads();
+          for (int i = t; i < CG * BMI_PATCH * 9; i += BMI_BLOCK_THREADS) {
+              const int g = i / (BMI_PATCH * 9), r = i - g * BMI_PATCH * 9;
+              const int c = r / 9, e = r - 9 * (r / 9);
+              const int L = bt[g][c];
+              bv[g][c][e] = L >= 0 ? a.Y.vtx[9 * L + e] : 0.0;
+          }
+          for (int i = t; i < CG * BMI_PATCH * 3; i += BMI_BLOCK_THREADS) {
+              const int g = i / (BMI_PATCH * 3), r = i - g * BMI_PATCH * 3;
+              const int c = r / 3, e = r - 3 * (r / 3);
+              const int L = bt[g][c];
+              bc[g][c][e] = L >= 0 ? a.Y.cen[3 * L + e] : 0.0;
+          }
+          if (t < CG * BMI_PATCH) {
+              const int g = t / BMI_PATCH, c = t - BMI_PATCH * g;
+              barea[g][c] = bt[g][c] >= 0 ? a.Y.area[bt[g][c]] : 1.0;
+          }
+          __syncthreads();
+          for (int g = 0; g < ng; ++g) {
+            const int qq = nearq[u0 + g];
             const long long b = (long long)p * a.npb + qq;
-            if (t < BMI_PATCH) bt[t] = a.Y.patch_tri[qq * BMI_PATCH + t];
-            __syncthreads();
-            for (int i = t; i < BMI_PATCH * 9; i += BMI_BLOCK_THREADS) {
-                const int c = i / 9, e = i - 9 * (i / 9);
-                bv[c][e] = bt[c] >= 0 ? a.Y.vtx[9 * bt[c] + e] : 0.0;
-            }
-            if (t < BMI_PATCH * 3) {
-                const int c = t / 3, e = t - 3 * (t / 3);
-                bc[c][e] = bt[c] >= 0 ? a.Y.cen[3 * bt[c] + e] : 0.0;
-            }
-            if (t < BMI_PATCH) barea[t] = bt[t] >= 0 ? a.Y.area[bt[t]] : 1.0;
-            __syncthreads();
-
             const int kl = t >> 4, ll = t & 15;
-            const int K = tt[kl], L = bt[ll];
+            const int K = tt[kl], L = bt[g][ll];
             int cls = -1;
             if (K >= 0 && L >= 0) {
                 int hits = 0;
@@ -712,22 +726,22 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
                 for (int i = 0; i < 3; ++i)
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
-                        const double dx = tv[kl][3 * i] - bv[ll][3 * j];
-                        const double dy = tv[kl][3 * i + 1] - bv[ll][3 * j + 1];
-                        const double dz = tv[kl][3 * i + 2] - bv[ll][3 * j + 2];
+                        const double dx = tv[kl][3 * i] - bv[g][ll][3 * j];
+                        const double dy = tv[kl][3 * i + 1] - bv[g][ll][3 * j + 1];
+                        const double dz = tv[kl][3 * i + 2] - bv[g][ll][3 * j + 2];
                         const double d2 = dx * dx + dy * dy + dz * dz;
                         dmin2 = fmin(dmin2, d2);
                         hits += d2 < dtol2;
                     }
                 if (hits > 0) {
                     cls = hits > 3 ? 3 : hits;
-                } else if (dmin2 * fmax(a.k2x16, 1.0 / barea[ll]) < 0.2 * 0.2 * 16) {
+                } else if (dmin2 * fmax(a.k2x16, 1.0 / barea[g][ll]) < 0.2 * 0.2 * 16) {
                     cls = CLS_NEAR;
                 } else {
                     cls = CLS_FAR;
                     if (a.near_d2 >= 0.0) {
-                        const double cx = tc[kl][0] - bc[ll][0], cy = tc[kl][1] - bc[ll][1],
-                                     cz = tc[kl][2] - bc[ll][2];
+                        const double cx = tc[kl][0] - bc[g][ll][0], cy = tc[kl][1] - bc[g][ll][1],
+                                     cz = tc[kl][2] - bc[g][ll][2];
                         if (cx * cx + cy * cy + cz * cz < a.near_d2) cls = CLS_FAR64;
                     }
                 }
@@ -763,7 +777,10 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
                     a.spec_loc[e] = (unsigned char)t;
                 }
             }
-            __syncthreads();  // bt, bv, wsum reused by the next block
+            __syncthreads();  // wsum, s_off reused by the next block
+          }
+          // bt, bv, bc, barea are refilled by the next group: the barrier
+          // above ends every thread's reads of this group
         }
     }
 }
