@@ -641,6 +641,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
     __shared__ double tc[BMI_PATCH][3], bc[CG][BMI_PATCH][3];
     __shared__ double barea[CG][BMI_PATCH];
     __shared__ int tt[BMI_PATCH], bt[CG][BMI_PATCH];
+    __shared__ double trad[BMI_PATCH];  // test cell: largest vertex distance from its centroid
     __shared__ int nearq[BMI_BLOCK_THREADS];
     __shared__ int wsum[BMI_BLOCK_THREADS / 64];
     __shared__ int s_off;
@@ -663,6 +664,17 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
     const double px = a.X.patch_sph[4 * p], py = a.X.patch_sph[4 * p + 1], pz = a.X.patch_sph[4 * p + 2];
     const double pr = a.X.patch_sph[4 * p + 3];
     const double dtol2 = DTOL * DTOL;
+    __syncthreads();
+    if (t < BMI_PATCH) {
+        double r2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double dx = tv[t][3 * i] - tc[t][0], dy = tv[t][3 * i + 1] - tc[t][1], dz = tv[t][3 * i + 2] - tc[t][2];
+            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+        }
+        trad[t] = sqrt(r2);
+    }
+    __syncthreads();
 
     for (int q0 = 0; q0 < a.npb; q0 += BMI_BLOCK_THREADS) {
         // cull this chunk of trial patches
@@ -671,10 +683,26 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
         if (q < a.npb) {
             const double dx = px - a.Y.patch_sph[4 * q], dy = py - a.Y.patch_sph[4 * q + 1],
                          dz = pz - a.Y.patch_sph[4 * q + 2];
-            const double D = sqrt(dx * dx + dy * dy + dz * dz) - pr - a.Y.patch_sph[4 * q + 3];
+            const double qr = a.Y.patch_sph[4 * q + 3];
+            const double D = sqrt(dx * dx + dy * dy + dz * dz) - pr - qr;
             double reach = fmax(1e-9, 0.8 * sqrt(a.Y.patch_amax[q]));
             if (a.near_d >= 0.0) reach = fmax(reach, a.near_d);
-            cand = D < 1.0001 * reach + 1e-12;
+            const double lim = 1.0001 * reach + 1e-12;
+            cand = D < lim;
+            if (cand) {
+                // then cell by cell: a pair can only be special if its test
+                // cell's bounding sphere comes within reach of the patch's
+                const double qx = a.Y.patch_sph[4 * q], qy = a.Y.patch_sph[4 * q + 1], qz = a.Y.patch_sph[4 * q + 2];
+                cand = 0;
+                for (int k = 0; k < BMI_PATCH; ++k) {
+                    if (tt[k] < 0) continue;
+                    const double ex = tc[k][0] - qx, ey = tc[k][1] - qy, ez = tc[k][2] - qz;
+                    if (sqrt(ex * ex + ey * ey + ez * ez) - trad[k] - qr < lim) {
+                        cand = 1;
+                        break;
+                    }
+                }
+            }
             if (!cand) a.blk_cnt[(long long)p * a.npb + q] = 0;
         }
         const unsigned long long m = __ballot(cand);
