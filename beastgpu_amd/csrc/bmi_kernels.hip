@@ -1829,11 +1829,11 @@ __device__ __forceinline__ int tile_ix(int r, int c) { return c * BMI_PANEL_ROWS
 
 // One (test panel P, trial patch q) block of the far-pair assembly with its
 // write-back; the caller has filled sh.s_sc (FP64 path) and passes the tile
-// of BMI_PANEL_ROWS x ncm entries.  Ends after the write-back stores are
+// of BMI_PANEL_ROWS x (widest patch) entries.  Ends after the write-back stores are
 // issued; a caller that reuses the LDS for another block synchronises first.
 //
 // Rank rounds: each thread adds its pair's 9 values straight into the LDS
-// tile of the work-group's 64 rows x ncm columns.  Two pairs add to the same
+// tile of the work-group's 64 rows x the patch's columns.  Two pairs add to the same
 // entry only if their test cells share the entry's row and their trial cells
 // share its column; giving every slot its rank among the slots of its group
 // with the same row (column), the additions go in rounds (rank_t, rank_b)
@@ -1841,8 +1841,7 @@ __device__ __forceinline__ int tile_ix(int r, int c) { return c * BMI_PANEL_ROWS
 // and the summation order is fixed.  RWG functions have at most two cells, so
 // there are at most 2 x 2 rounds.
 template <int OP, int MIXED, int FAST, typename OutT>
-__device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, int ncm, double2 *tile,
-                                            PanelShared &sh) {
+__device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, double2 *tile, PanelShared &sh) {
     auto &sg = sh.sg;
     auto &bg = sh.bg;
     auto &s_tslot = sh.s_tslot;
@@ -2066,25 +2065,25 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, in
 // One work-group per (test panel P, trial patch q of this launch's colour).
 template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_rr(PanelArgs a) {
-    extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x ncm
+    extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x a.ncm
     __shared__ PanelShared sh;
     if (!MIXED) fill_sincos_table(sh.s_sc);
     const int w = blockIdx.x;
     const int P = w / a.nbq;
     const int q = a.bq_list[w - P * a.nbq];
-    panel_block<OP, MIXED, FAST, OutT>(a, P, q, a.ncm, tile, sh);
+    panel_block<OP, MIXED, FAST, OutT>(a, P, q, tile, sh);
 }
 
 // Batched, one colour at a time: work-group i takes item i = (problem, panel,
 // patch, -) of the launch's colour, across the problems of a batch.
 template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_items(
-    const PanelArgs *args, const int4 *items, int ncm) {
-    extern __shared__ double2 tile[];
+    const PanelArgs *args, const int4 *items) {
+    extern __shared__ double2 tile[];  // BMI_PANEL_ROWS x (the batch's widest patch)
     __shared__ PanelShared sh;
     if (!MIXED) fill_sincos_table(sh.s_sc);
     const int4 it = items[blockIdx.x];
-    panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, ncm, tile, sh);
+    panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, tile, sh);
 }
 
 // Batched, persistent: the work-groups pop items (problem, panel, patch,
@@ -2097,7 +2096,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
 // all the same and a timeout sets *err instead of hanging.
 template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_queue(
-    const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm, int *head, int *done, int *err,
+    const PanelArgs *args, const int4 *items, const int *need, int nitems, int *head, int *done, int *err,
     int spin_limit) {
     extern __shared__ double2 tile[];
     __shared__ PanelShared sh;
@@ -2126,7 +2125,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
             __syncthreads();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, ncm, tile, sh);
+        panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, tile, sh);
         // publish this block's stores before counting it
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
@@ -2180,7 +2179,7 @@ template <int OP, int MIXED, int FAST, typename OutT>
 void items_t(const PanelArgs *args, const int4 *items, int nitems, int ncm, hipStream_t s) {
     const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * ncm;
     hipLaunchKernelGGL((k_panels_items<OP, MIXED, FAST, OutT>), dim3(nitems), dim3(BMI_BLOCK_THREADS), lds, s, args,
-                       items, ncm);
+                       items);
 }
 
 template <int OP, int MIXED, int FAST, typename OutT>
@@ -2188,7 +2187,7 @@ void queue_t(const PanelArgs *args, const int4 *items, const int *need, int nite
              int *err, int spin_limit, int grid, hipStream_t s) {
     const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * ncm;
     hipLaunchKernelGGL((k_panels_queue<OP, MIXED, FAST, OutT>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, args,
-                       items, need, nitems, ncm, head, done, err, spin_limit);
+                       items, need, nitems, head, done, err, spin_limit);
 }
 
 // f(op, mixed, fast, c32) -> the instantiation
@@ -2382,7 +2381,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_compare(const TA *a, cons
     s2[threadIdx.x] = dm;
     __syncthreads();
     for (int o = BMI_BLOCK_THREADS / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
+        if ((int)threadIdx.x < o) {
             s0[threadIdx.x] += s0[threadIdx.x + o];
             s1[threadIdx.x] += s1[threadIdx.x + o];
             s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + o]);
