@@ -294,7 +294,9 @@ struct bmi_space {
     // fixed-size records: test role (per sub-batch) and trial role (per patch)
     DevBuf<int> d_trs, d_bnc, d_bfn, d_bcs, d_tslot, d_bslot;
     DevBuf<double> d_tscf, d_bscf;
-    int rounds_t = 1, rounds_b = 1, ncmax = 1;
+    int rounds_t = 1, rounds_b = 1, levels_b = 1, ncmax = 1;
+    // rounds of each trial patch: npb by slot rank, then npb by slot level
+    DevBuf<unsigned char> d_brounds;
     DevBuf<unsigned char> d_tkl, d_bfirst, d_bkl;
     DevBuf<double> d_tcf, d_bcf;
 };
@@ -791,15 +793,24 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
             bcf[(size_t)p * BMI_MAX_ROWS + (u - base)] = s->slot_cf[u];
         }
     }
-    // rank-round records: column of each cell slot in its patch and its rank
+    // rank-round records: column of each cell slot in its patch (bits 0-7),
+    // its rank among the patch's slots of that column (bits 8-15) and its
+    // level, the rank among the slots of that column with the same local
+    // shape index (bits 16-23).  Rounds by rank touch an entry at most once
+    // per round; rounds by level may touch it once per local shape index, in
+    // shape order.  Lagrange functions span many cells of a patch with
+    // different local indices, so they need far fewer levels than ranks.
     std::vector<int> bslot((size_t)np * BMI_MAX_ROWS, 0xff);
     std::vector<double> bscf((size_t)np * BMI_MAX_ROWS, 0.0);
+    std::vector<unsigned char> brounds(2 * (size_t)np, 1);
     s->rounds_b = 1;
+    s->levels_b = 1;
     s->ncmax = 1;
     for (int p = 0; p < np; ++p) {
         const int r0 = s->row_ptr[p], nr = s->row_ptr[p + 1] - r0;
         s->ncmax = std::max(s->ncmax, nr);
-        int seen[BMI_MAX_ROWS] = {0};
+        int seen[BMI_MAX_ROWS] = {0}, seen_j[BMI_MAX_ROWS][3] = {};
+        int rk = 1, lv = 1;
         for (int l = 0; l < BMI_PATCH; ++l) {
             const int L = s->patch_tri[(size_t)p * BMI_PATCH + l];
             if (L < 0) continue;
@@ -808,14 +819,20 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
                 if (f < 0) continue;
                 const int c = (int)(std::lower_bound(s->row_fn.begin() + r0, s->row_fn.begin() + r0 + nr, (int)f) -
                                     (s->row_fn.begin() + r0));
-                const int rank = seen[c]++;
-                bslot[(size_t)p * BMI_MAX_ROWS + 3 * l + j] = c | (rank << 8);
+                const int rank = seen[c]++, level = seen_j[c][j]++;
+                bslot[(size_t)p * BMI_MAX_ROWS + 3 * l + j] = c | (rank << 8) | (level << 16);
                 bscf[(size_t)p * BMI_MAX_ROWS + 3 * l + j] = coeff[3 * (int64_t)L + j];
-                s->rounds_b = std::max(s->rounds_b, rank + 1);
+                rk = std::max(rk, rank + 1);
+                lv = std::max(lv, level + 1);
             }
         }
+        brounds[p] = (unsigned char)rk;
+        brounds[np + p] = (unsigned char)lv;
+        s->rounds_b = std::max(s->rounds_b, rk);
+        s->levels_b = std::max(s->levels_b, lv);
     }
-    if ((e = s->d_bslot.upload(bslot)) != hipSuccess || (e = s->d_bscf.upload(bscf)) != hipSuccess)
+    if ((e = s->d_bslot.upload(bslot)) != hipSuccess || (e = s->d_bscf.upload(bscf)) != hipSuccess ||
+        (e = s->d_brounds.upload(brounds)) != hipSuccess)
         return fail(BMI_EDEVICE, std::string("space upload: ") + hipGetErrorString(e));
     if ((e = s->d_bnc.upload(bnc)) != hipSuccess || (e = s->d_bfn.upload(bfn)) != hipSuccess ||
         (e = s->d_bcs.upload(bcs)) != hipSuccess || (e = s->d_bfirst.upload(bfirst)) != hipSuccess ||
@@ -926,9 +943,12 @@ PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) {
     B.bslot = P->Y->d_bslot.p;
     B.bscf = P->Y->d_bscf.p;
     B.rounds_t = P->X->rounds_t;
-    B.rounds_b = P->Y->rounds_b;
+    // the Helmholtz kernels run their trial rounds by level, the others by rank
+    const bool by_level = P->op_kind == OPK_HH3D;
+    B.rounds_b = by_level ? P->Y->levels_b : P->Y->rounds_b;
+    B.brounds = P->Y->d_brounds.p + (by_level ? P->Y->npatch : 0);
     B.ncm = P->Y->ncmax;
-    B.rr = P->X->rounds_t * P->Y->rounds_b <= 6;
+    B.rr = P->X->rounds_t * B.rounds_b <= 6;
     B.fast = P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
     B.npb = P->npb;
     B.blk_off = P->d_blk_off.p;

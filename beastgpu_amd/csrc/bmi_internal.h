@@ -100,12 +100,15 @@ struct PanelArgs {
     const double *bcf;           // npb x 48
     // rank-round records (one per cell slot, 48 per group): bits 0-7 row (test)
     // or column (trial) index inside the work-group's tile, 0xff = none;
-    // bits 8-15 rank of the slot among the group's slots of that row/column
+    // bits 8-15 rank of the slot among the group's slots of that row/column;
+    // trial side, bits 16-23: level of the slot, its rank among the patch's
+    // slots of that column with the same local shape index
     const int *tslot;            // nsb x 48
     const double *tscf;          // nsb x 48
     const int *bslot;            // npb x 48
     const double *bscf;          // npb x 48
-    int rounds_t, rounds_b;      // largest rank + 1 on each side
+    int rounds_t, rounds_b;      // largest test rank + 1; largest trial rank (level) + 1
+    const unsigned char *brounds;  // npb: trial rounds of each patch (by rank, or by level in the Helmholtz kernels)
     int ncm;                     // largest number of columns of a trial patch
     int rr;                      // 1: rank-round kernel, 0: pair buffer + gather
     int fast;                    // 1: default 3 x 4 point far rule and a lossless operator (compile-time rule)

@@ -1856,6 +1856,9 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     const int kl = t >> 4, ll = t & 15;
     const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
 
+    // trial rounds by level (bslot bits 16-23) or by rank (bits 8-15)
+    constexpr bool BY_LEVEL = OP == OPK_HH3D;
+    constexpr int RSH = BY_LEVEL ? 16 : 8;
     const int lane = t & 63, wv = t >> 6;
     for (int c = wv; c < nc; c += BMI_BLOCK_THREADS / 64) tile[c * BMI_PANEL_ROWS + lane] = make_double2(0.0, 0.0);
     short *s_ent = sh.s_ent;
@@ -1974,19 +1977,25 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
         // With one test rank (the round-robin sub-batches), different test
         // cells of a sub-batch own different rows, so two updates of one
         // entry come from lanes of the same 16-lane group, i.e. of one wave:
-        // the trial-rank rounds need no barrier, as one wave's LDS accesses
+        // the trial rounds need no barrier, as one wave's LDS accesses
         // are performed in program order.  More test ranks put colliding
         // rows in different waves: a barrier after each test round.
-        // Within a round a thread's entries are distinct and no other thread
-        // touches them, so the mixed kernel (which has the registers) issues
-        // all of its reads before its writes: one LDS wait per round instead
-        // of one per entry; unused slots read entry 0 and write nothing.
+        // Trial rounds by rank: within a round a thread's entries are
+        // distinct and no other thread touches them, so the mixed kernel
+        // (which has the registers) issues all of its reads before its
+        // writes: one LDS wait per round instead of one per entry; unused
+        // slots read entry 0 and write nothing.  Trial rounds by level (the
+        // Helmholtz kernels): two lanes may add to one entry in one round
+        // only through different local shapes j, and the updates go shape
+        // by shape in program order, each shape's reads after the previous
+        // shape's writes.
+        const int nrb = a.brounds[q];
         for (int ra = 0; ra < a.rounds_t; ++ra) {
             bool tact[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) tact[i] = (tsl[i] & 0xff) != 0xff && (tsl[i] >> 8) == ra;
-            for (int rb = 0; rb < a.rounds_b; ++rb) {
-                if constexpr (MIXED && FAST) {
+            for (int rb = 0; rb < nrb; ++rb) {
+                if constexpr (MIXED && FAST && !BY_LEVEL) {
                     int ix[9];
                     bool act[9];
                     double2 z[9];
@@ -1994,8 +2003,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                     for (int i = 0; i < 3; ++i)
 #pragma unroll
                         for (int j = 0; j < 3; ++j) {
-                            act[3 * i + j] = (tsl[i] & 0xff) != 0xff && (tsl[i] >> 8) == ra &&
-                                             (bsl[j] & 0xff) != 0xff && (bsl[j] >> 8) == rb;
+                            act[3 * i + j] = tact[i] && (bsl[j] & 0xff) != 0xff && ((bsl[j] >> 8) & 0xff) == rb;
                             ix[3 * i + j] = act[3 * i + j] ? tile_ix(tsl[i] & 0xff, bsl[j] & 0xff) : 0;
                             z[3 * i + j] = tile[ix[3 * i + j]];
                         }
@@ -2010,12 +2018,35 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                             v.y = fma(cc, loc[3 * i + j].y, v.y);
                             tile[ix[3 * i + j]] = v;
                         }
+                } else if constexpr (MIXED && FAST) {
+                    // by level: the three reads of one shape together
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const bool bj = (bsl[j] & 0xff) != 0xff && ((bsl[j] >> 16) & 0xff) == rb;
+                        int ix[3];
+                        bool act[3];
+                        double2 z[3];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            act[i] = tact[i] && bj;
+                            ix[i] = act[i] ? tile_ix(tsl[i] & 0xff, bsl[j] & 0xff) : 0;
+                            z[i] = tile[ix[i]];
+                        }
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            if (!act[i]) continue;
+                            const double cc = tcf[i] * bcf[j];
+                            z[i].x = fma(cc, loc[3 * i + j].x, z[i].x);
+                            z[i].y = fma(cc, loc[3 * i + j].y, z[i].y);
+                            tile[ix[i]] = z[i];
+                        }
+                    }
                 } else {
                     // trial slot outermost: one rank test per column and
                     // round (the test slots' tests do not depend on the round)
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
-                        if ((bsl[j] & 0xff) == 0xff || (bsl[j] >> 8) != rb) continue;
+                        if ((bsl[j] & 0xff) == 0xff || ((bsl[j] >> RSH) & 0xff) != rb) continue;
                         const int col = bsl[j] & 0xff;
 #pragma unroll
                         for (int i = 0; i < 3; ++i) {
