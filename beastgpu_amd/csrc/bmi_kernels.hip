@@ -901,6 +901,23 @@ struct SpecAcc {
 };
 
 
+// Entry (i, j) of the EFIE local matrix from the moments m[0..16) (M0, Mx,
+// My, Mxy as real/imaginary pairs in the order of the special-pair
+// accumulators) and a = v_i - c_K, b = v_j - c_L, the vertices opposite the
+// two shapes' edges relative to the centroids; as mw_epilogue.
+__device__ __forceinline__ double2 mw_entry(const OpConst &op, const double *m, const double *a, const double *b) {
+    const double ur0 = m[2] - a[0] * m[0], ur1 = m[3] - a[1] * m[0], ur2 = m[4] - a[2] * m[0];
+    const double ui0 = m[5] - a[0] * m[1], ui1 = m[6] - a[1] * m[1], ui2 = m[7] - a[2] * m[1];
+    const double ar = m[14] - (a[0] * m[8] + a[1] * m[9] + a[2] * m[10]);
+    const double ai = m[15] - (a[0] * m[11] + a[1] * m[12] + a[2] * m[13]);
+    const double vr = ar - (b[0] * ur0 + b[1] * ur1 + b[2] * ur2);
+    const double vi = ai - (b[0] * ui0 + b[1] * ui1 + b[2] * ui2);
+    double2 z;
+    z.x = op.alpha_re * vr - op.alpha_im * vi + 4.0 * (op.beta_re * m[0] - op.beta_im * m[1]);
+    z.y = op.alpha_re * vi + op.alpha_im * vr + 4.0 * (op.beta_re * m[1] + op.beta_im * m[0]);
+    return z;
+}
+
 struct CellG {
     double o[3];   // v3 - centroid
     double t1[3];  // v1 - v3
@@ -1419,85 +1436,70 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
         // butterfly over the 64 lanes, all accumulators at each step: six
         // dependent rounds of lane exchanges in all (one accumulator after
         // another would chain six exchanges per accumulator, each waiting
-        // for the last)
+        // for the last); the exchanges use the LDS crossbar, not the VALU.
+        // Every lane then holds the sums; lane l < 9 forms entry l of the
+        // chart-order 3 x 3 matrix.
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
 #pragma unroll
             for (int i = 0; i < spec_nacc(OP); ++i) acc.v[i] += __shfl_xor(acc.v[i], o);
         }
-        if (lane == 0) {
-            double2 loc[9];
-            if (OP == OPK_MW) {
-                MomMW mm;
-                mm.m0r = acc.v[0]; mm.m0i = acc.v[1];
+        const double *red = acc.v;
+        if (lane < 9) {
+            // chart vertices c (test) and c2 (trial) relative to the
+            // centroids (o + t1, o + t2, o); entry (c, c2) belongs to the
+            // local shapes (PT[c], PS[c2])
+            const int c = lane / 3, c2 = lane - 3 * (lane / 3);
+            double Xc[3], Yc[3];
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    mm.mxr[d] = acc.v[2 + d]; mm.mxi[d] = acc.v[5 + d];
-                    mm.myr[d] = acc.v[8 + d]; mm.myi[d] = acc.v[11 + d];
-                }
-                mm.mxyr = acc.v[14]; mm.mxyi = acc.v[15];
-                double xa[9], ya[9];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        xa[3 * i + d] = a.X.vtx[9 * K + 3 * i + d] - a.X.cen[3 * K + d];
-                        ya[3 * i + d] = a.Y.vtx[9 * L + 3 * i + d] - a.Y.cen[3 * L + d];
-                    }
-                mw_epilogue(a.op, mm, xa, ya, loc);
+            for (int d = 0; d < 3; ++d) {
+                Xc[d] = X.o[d] + (c == 0 ? X.t1[d] : (c == 1 ? X.t2[d] : 0.0));
+                Yc[d] = Y.o[d] + (c2 == 0 ? Y.t1[d] : (c2 == 1 ? Y.t2[d] : 0.0));
+            }
+            const int dst = 3 * (c == 0 ? PT[0] : (c == 1 ? PT[1] : PT[2])) + (c2 == 0 ? PS[0] : (c2 == 1 ? PS[1] : PS[2]));
+            double2 z;
+            if constexpr (OP == OPK_MW) {
+                // moments M0, Mx, My, Mxy (real, imaginary)
+                z = mw_entry(a.op, red, Xc, Yc);
             } else {
-                // entry (c, c') in chart order belongs to the local shapes
-                // (PT[c], PS[c'])
-                double pr[9], pi[9];
+                double pr, pi;
                 if constexpr (OP == OPK_DL) {
-                    // chart vertices relative to the centroids
-                    double Xc[3][3], Yc[3][3];
+                    // M2 - X.M3 - Y.M4 + (X.Y) M1 - M5 + X.M6 + (n.Y)(M7 - X.M8)
+                    double tr = red[2] - red[4], ti = red[3] - red[5];
+                    double qr = red[6], qi = red[7];
+                    double ur[3], ui[3];
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        Xc[0][d] = X.o[d] + X.t1[d];
-                        Xc[1][d] = X.o[d] + X.t2[d];
-                        Xc[2][d] = X.o[d];
-                        Yc[0][d] = Y.o[d] + Y.t1[d];
-                        Yc[1][d] = Y.o[d] + Y.t2[d];
-                        Yc[2][d] = Y.o[d];
+                        tr += Xc[d] * (red[20 + d] - red[8 + d]);
+                        ti += Xc[d] * (red[23 + d] - red[11 + d]);
+                        ur[d] = red[14 + d] - Xc[d] * red[0];
+                        ui[d] = red[17 + d] - Xc[d] * red[1];
+                        qr -= Xc[d] * red[26 + d];
+                        qi -= Xc[d] * red[29 + d];
                     }
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        // terms that depend on c only
-                        double tr = acc.v[2] - acc.v[4], ti = acc.v[3] - acc.v[5];
-                        double ur[3], ui[3], qr = acc.v[6], qi = acc.v[7];
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            tr += Xc[c][d] * (acc.v[20 + d] - acc.v[8 + d]);
-                            ti += Xc[c][d] * (acc.v[23 + d] - acc.v[11 + d]);
-                            ur[d] = acc.v[14 + d] - Xc[c][d] * acc.v[0];
-                            ui[d] = acc.v[17 + d] - Xc[c][d] * acc.v[1];
-                            qr -= Xc[c][d] * acc.v[26 + d];
-                            qi -= Xc[c][d] * acc.v[29 + d];
-                        }
-#pragma unroll
-                        for (int c2 = 0; c2 < 3; ++c2) {
-                            const double nY = nrm[0] * Yc[c2][0] + nrm[1] * Yc[c2][1] + nrm[2] * Yc[c2][2];
-                            // M2 - X.M3 - Y.M4 + (X.Y) M1 - M5 + X.M6 + (n.Y)(M7 - X.M8)
-                            pr[3 * PT[c] + PS[c2]] = tr - (Yc[c2][0] * ur[0] + Yc[c2][1] * ur[1] + Yc[c2][2] * ur[2]) + nY * qr;
-                            pi[3 * PT[c] + PS[c2]] = ti - (Yc[c2][0] * ui[0] + Yc[c2][1] * ui[1] + Yc[c2][2] * ui[2]) + nY * qi;
-                        }
-                    }
+                    const double nY = nrm[0] * Yc[0] + nrm[1] * Yc[1] + nrm[2] * Yc[2];
+                    pr = tr - (Yc[0] * ur[0] + Yc[1] * ur[1] + Yc[2] * ur[2]) + nY * qr;
+                    pi = ti - (Yc[0] * ui[0] + Yc[1] * ui[1] + Yc[2] * ui[2]) + nY * qi;
                 } else {
+                    // red[2 lane], red[2 lane + 1] by selects (a register
+                    // array indexed by the lane would go to scratch memory)
+                    pr = red[0];
+                    pi = red[1];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c)
-#pragma unroll
-                        for (int c2 = 0; c2 < 3; ++c2) {
-                            pr[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2)];
-                            pi[3 * PT[c] + PS[c2]] = acc.v[2 * (3 * c + c2) + 1];
+                    for (int k = 1; k < 9; ++k)
+                        if (lane == k) {
+                            pr = red[2 * k];
+                            pi = red[2 * k + 1];
                         }
                 }
                 // Helmholtz: Jacobian 4 A_K A_L; double layer: it cancels the
                 // 1/(2A) of the two RWG shapes
-                hh_epilogue(a.op, pr, pi, OP == OPK_DL ? 1.0 : 4.0 * a.X.area[K] * a.Y.area[L], loc);
+                const double scale = OP == OPK_DL ? 1.0 : 4.0 * a.X.area[K] * a.Y.area[L];
+                const double sr = a.op.alpha_re * scale, si = a.op.alpha_im * scale;
+                z.x = sr * pr - si * pi;
+                z.y = sr * pi + si * pr;
             }
-#pragma unroll
-            for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
+            a.spec_val[9 * (long long)e + dst] = z;
         }
     }
     }
