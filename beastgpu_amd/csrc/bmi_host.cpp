@@ -943,13 +943,15 @@ PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) {
     B.bslot = P->Y->d_bslot.p;
     B.bscf = P->Y->d_bscf.p;
     B.rounds_t = P->X->rounds_t;
-    // the Helmholtz kernels run their trial rounds by level, the others by rank
-    const bool by_level = P->op_kind == OPK_HH3D;
+    B.fast = P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
+    // trial rounds by level: the Helmholtz kernels and the FP64 EFIE kernel
+    // with the compile-time rule; the others by rank (must match panel_block)
+    const bool by_level = P->op_kind == OPK_HH3D ||
+                          (B.fast && P->op_kind == OPK_MW && P->opts.precision != BMI_PREC_MIXED);
     B.rounds_b = by_level ? P->Y->levels_b : P->Y->rounds_b;
     B.brounds = P->Y->d_brounds.p + (by_level ? P->Y->npatch : 0);
     B.ncm = P->Y->ncmax;
     B.rr = P->X->rounds_t * B.rounds_b <= 6;
-    B.fast = P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
     B.npb = P->npb;
     B.blk_off = P->d_blk_off.p;
     B.blk_cnt = P->d_blk_cnt.p;

@@ -1859,7 +1859,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
 
     // trial rounds by level (bslot bits 16-23) or by rank (bits 8-15)
-    constexpr bool BY_LEVEL = OP == OPK_HH3D;
+    constexpr bool BY_LEVEL = OP == OPK_HH3D || (FAST && !MIXED && OP == OPK_MW);
     constexpr int RSH = BY_LEVEL ? 16 : 8;
     const int lane = t & 63, wv = t >> 6;
     for (int c = wv; c < nc; c += BMI_BLOCK_THREADS / 64) tile[c * BMI_PANEL_ROWS + lane] = make_double2(0.0, 0.0);
@@ -2020,7 +2020,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                             v.y = fma(cc, loc[3 * i + j].y, v.y);
                             tile[ix[3 * i + j]] = v;
                         }
-                } else if constexpr (MIXED && FAST) {
+                } else if constexpr (FAST && BY_LEVEL) {
                     // by level: the three reads of one shape together
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
