@@ -270,21 +270,25 @@ struct MomMW {
 
 __device__ __forceinline__ void mw_epilogue(const OpConst &op, const MomMW &m, const double *xa, const double *ya,
                                             double2 loc[9]) {
+    // divergence term 4 beta M0
+    const double dr = 4.0 * (op.beta_re * m.m0r - op.beta_im * m.m0i);
+    const double di = 4.0 * (op.beta_re * m.m0i + op.beta_im * m.m0r);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const double a0 = xa[3 * i], a1 = xa[3 * i + 1], a2 = xa[3 * i + 2];
         const double ur0 = m.mxr[0] - a0 * m.m0r, ur1 = m.mxr[1] - a1 * m.m0r, ur2 = m.mxr[2] - a2 * m.m0r;
         const double ui0 = m.mxi[0] - a0 * m.m0i, ui1 = m.mxi[1] - a1 * m.m0i, ui2 = m.mxi[2] - a2 * m.m0i;
-        const double ar = m.mxyr - (a0 * m.myr[0] + a1 * m.myr[1] + a2 * m.myr[2]);
-        const double ai = m.mxyi - (a0 * m.myi[0] + a1 * m.myi[1] + a2 * m.myi[2]);
+        // chains of fused multiply-adds, one instruction per term
+        const double ar = fma(-a2, m.myr[2], fma(-a1, m.myr[1], fma(-a0, m.myr[0], m.mxyr)));
+        const double ai = fma(-a2, m.myi[2], fma(-a1, m.myi[1], fma(-a0, m.myi[0], m.mxyi)));
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const double b0 = ya[3 * j], b1 = ya[3 * j + 1], b2 = ya[3 * j + 2];
-            const double vr = ar - (b0 * ur0 + b1 * ur1 + b2 * ur2);
-            const double vi = ai - (b0 * ui0 + b1 * ui1 + b2 * ui2);
+            const double vr = fma(-b2, ur2, fma(-b1, ur1, fma(-b0, ur0, ar)));
+            const double vi = fma(-b2, ui2, fma(-b1, ui1, fma(-b0, ui0, ai)));
             double2 z;
-            z.x = op.alpha_re * vr - op.alpha_im * vi + 4.0 * (op.beta_re * m.m0r - op.beta_im * m.m0i);
-            z.y = op.alpha_re * vi + op.alpha_im * vr + 4.0 * (op.beta_re * m.m0i + op.beta_im * m.m0r);
+            z.x = fma(op.alpha_re, vr, fma(-op.alpha_im, vi, dr));
+            z.y = fma(op.alpha_re, vi, fma(op.alpha_im, vr, di));
             loc[3 * i + j] = z;
         }
     }
@@ -393,18 +397,20 @@ __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const S a0 = (S)xa[3 * i], a1 = (S)xa[3 * i + 1], a2 = (S)xa[3 * i + 2];
-        // terms that depend on i only
-        const S ir = Ar - (a0 * Cr[0] + a1 * Cr[1] + a2 * Cr[2]) - Er + (a0 * Fr[0] + a1 * Fr[1] + a2 * Fr[2]);
-        const S ii = Ai - (a0 * Ci[0] + a1 * Ci[1] + a2 * Ci[2]) - Ei + (a0 * Fi[0] + a1 * Fi[1] + a2 * Fi[2]);
-        const S aHr = a0 * Hr[0] + a1 * Hr[1] + a2 * Hr[2], aHi = a0 * Hi[0] + a1 * Hi[1] + a2 * Hi[2];
+        // terms that depend on i only (chains of fused multiply-adds, one
+        // instruction per term)
+        const S ir = fma(a2, Fr[2] - Cr[2], fma(a1, Fr[1] - Cr[1], fma(a0, Fr[0] - Cr[0], Ar - Er)));
+        const S ii = fma(a2, Fi[2] - Ci[2], fma(a1, Fi[1] - Ci[1], fma(a0, Fi[0] - Ci[0], Ai - Ei)));
+        const S gr = Gr - fma(a2, Hr[2], fma(a1, Hr[1], a0 * Hr[0]));
+        const S gi = Gi - fma(a2, Hi[2], fma(a1, Hi[1], a0 * Hi[0]));
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const S b0 = (S)ya[3 * j], b1 = (S)ya[3 * j + 1], b2 = (S)ya[3 * j + 2];
-            const S ab = a0 * b0 + a1 * b1 + a2 * b2, nb = n0 * b0 + n1 * b1 + n2 * b2;
-            const S vr = ir - (b0 * Br[0] + b1 * Br[1] + b2 * Br[2]) + ab * Dr + nb * (Gr - aHr);
-            const S vi = ii - (b0 * Bi[0] + b1 * Bi[1] + b2 * Bi[2]) + ab * Di + nb * (Gi - aHi);
-            loc[3 * i + j].x = (double)(alr * vr - ali * vi);
-            loc[3 * i + j].y = (double)(alr * vi + ali * vr);
+            const S ab = fma(a2, b2, fma(a1, b1, a0 * b0)), nb = fma(n2, b2, fma(n1, b1, n0 * b0));
+            const S vr = fma(nb, gr, fma(ab, Dr, fma(-b2, Br[2], fma(-b1, Br[1], fma(-b0, Br[0], ir)))));
+            const S vi = fma(nb, gi, fma(ab, Di, fma(-b2, Bi[2], fma(-b1, Bi[1], fma(-b0, Bi[0], ii)))));
+            loc[3 * i + j].x = (double)fma(alr, vr, -ali * vi);
+            loc[3 * i + j].y = (double)fma(alr, vi, ali * vr);
         }
     }
 }
@@ -451,8 +457,8 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
             m.mxi[0] = fma(x0, si, m.mxi[0]); m.mxi[1] = fma(x1, si, m.mxi[1]); m.mxi[2] = fma(x2, si, m.mxi[2]);
             m.myr[0] += tr0; m.myr[1] += tr1; m.myr[2] += tr2;
             m.myi[0] += ti0; m.myi[1] += ti1; m.myi[2] += ti2;
-            m.mxyr += x0 * tr0 + x1 * tr1 + x2 * tr2;
-            m.mxyi += x0 * ti0 + x1 * ti1 + x2 * ti2;
+            m.mxyr = fma(x2, tr2, fma(x1, tr1, fma(x0, tr0, m.mxyr)));
+            m.mxyi = fma(x2, ti2, fma(x1, ti1, fma(x0, ti0, m.mxyi)));
         }
         mw_epilogue(op, m, xa, ya, loc);
     } else {
@@ -544,8 +550,8 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
             mxi[0] = fmaf(x0, si, mxi[0]); mxi[1] = fmaf(x1, si, mxi[1]); mxi[2] = fmaf(x2, si, mxi[2]);
             myr[0] += tr0; myr[1] += tr1; myr[2] += tr2;
             myi[0] += ti0; myi[1] += ti1; myi[2] += ti2;
-            mxyr += x0 * tr0 + x1 * tr1 + x2 * tr2;
-            mxyi += x0 * ti0 + x1 * ti1 + x2 * ti2;
+            mxyr = fmaf(x2, tr2, fmaf(x1, tr1, fmaf(x0, tr0, mxyr)));
+            mxyi = fmaf(x2, ti2, fmaf(x1, ti1, fmaf(x0, ti0, mxyi)));
         }
         const float alr = (float)op.alpha_re, ali = (float)op.alpha_im;
         const float b4r = 4.0f * (float)op.beta_re, b4i = 4.0f * (float)op.beta_im;
@@ -555,15 +561,17 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
             const float a0 = (float)xa[3 * i], a1 = (float)xa[3 * i + 1], a2 = (float)xa[3 * i + 2];
             const float ur0 = mxr[0] - a0 * m0r, ur1 = mxr[1] - a1 * m0r, ur2 = mxr[2] - a2 * m0r;
             const float ui0 = mxi[0] - a0 * m0i, ui1 = mxi[1] - a1 * m0i, ui2 = mxi[2] - a2 * m0i;
-            const float ar = mxyr - (a0 * myr[0] + a1 * myr[1] + a2 * myr[2]);
-            const float ai = mxyi - (a0 * myi[0] + a1 * myi[1] + a2 * myi[2]);
+            // written as chains of fused multiply-adds (one instruction per
+            // term; the compiler may not reassociate the sums itself)
+            const float ar = fmaf(-a2, myr[2], fmaf(-a1, myr[1], fmaf(-a0, myr[0], mxyr)));
+            const float ai = fmaf(-a2, myi[2], fmaf(-a1, myi[1], fmaf(-a0, myi[0], mxyi)));
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const float b0 = (float)ya[3 * j], b1 = (float)ya[3 * j + 1], b2 = (float)ya[3 * j + 2];
-                const float vr = ar - (b0 * ur0 + b1 * ur1 + b2 * ur2);
-                const float vi = ai - (b0 * ui0 + b1 * ui1 + b2 * ui2);
-                loc[3 * i + j].x = (double)(alr * vr - ali * vi + dr);
-                loc[3 * i + j].y = (double)(alr * vi + ali * vr + di);
+                const float vr = fmaf(-b2, ur2, fmaf(-b1, ur1, fmaf(-b0, ur0, ar)));
+                const float vi = fmaf(-b2, ui2, fmaf(-b1, ui1, fmaf(-b0, ui0, ai)));
+                loc[3 * i + j].x = (double)fmaf(alr, vr, fmaf(-ali, vi, dr));
+                loc[3 * i + j].y = (double)fmaf(alr, vi, fmaf(ali, vr, di));
             }
         }
     } else {
@@ -908,13 +916,15 @@ struct SpecAcc {
 __device__ __forceinline__ double2 mw_entry(const OpConst &op, const double *m, const double *a, const double *b) {
     const double ur0 = m[2] - a[0] * m[0], ur1 = m[3] - a[1] * m[0], ur2 = m[4] - a[2] * m[0];
     const double ui0 = m[5] - a[0] * m[1], ui1 = m[6] - a[1] * m[1], ui2 = m[7] - a[2] * m[1];
-    const double ar = m[14] - (a[0] * m[8] + a[1] * m[9] + a[2] * m[10]);
-    const double ai = m[15] - (a[0] * m[11] + a[1] * m[12] + a[2] * m[13]);
-    const double vr = ar - (b[0] * ur0 + b[1] * ur1 + b[2] * ur2);
-    const double vi = ai - (b[0] * ui0 + b[1] * ui1 + b[2] * ui2);
+    const double ar = fma(-a[2], m[10], fma(-a[1], m[9], fma(-a[0], m[8], m[14])));
+    const double ai = fma(-a[2], m[13], fma(-a[1], m[12], fma(-a[0], m[11], m[15])));
+    const double vr = fma(-b[2], ur2, fma(-b[1], ur1, fma(-b[0], ur0, ar)));
+    const double vi = fma(-b[2], ui2, fma(-b[1], ui1, fma(-b[0], ui0, ai)));
+    const double dr = 4.0 * (op.beta_re * m[0] - op.beta_im * m[1]);
+    const double di = 4.0 * (op.beta_re * m[1] + op.beta_im * m[0]);
     double2 z;
-    z.x = op.alpha_re * vr - op.alpha_im * vi + 4.0 * (op.beta_re * m[0] - op.beta_im * m[1]);
-    z.y = op.alpha_re * vi + op.alpha_im * vr + 4.0 * (op.beta_re * m[1] + op.beta_im * m[0]);
+    z.x = fma(op.alpha_re, vr, fma(-op.alpha_im, vi, dr));
+    z.y = fma(op.alpha_re, vi, fma(op.alpha_im, vr, di));
     return z;
 }
 
