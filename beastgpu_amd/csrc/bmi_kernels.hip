@@ -1955,10 +1955,10 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                 const double *ya = &bg[BMI_GEO_VREL + 9 * ll];
                 pair_fp64<OP, FAST, true>(a.op, a.rule, xp, xa, xc, xs, yp, ya, yc, ys, loc, s_sc);
             }
-        } else {
-#pragma unroll
-            for (int v = 0; v < 9; ++v) loc[v] = make_double2(0.0, 0.0);
         }
+        // a lane without a pair (padding, or a cell of zero area, which
+        // contributes nothing) leaves loc unset and takes part in no round
+        bool live = xs > 0.0 && ys > 0.0;
         // special pairs of this block take their precomputed values; the
         // slot is cleared for the next sub-batch's entries (written after
         // the round barrier below)
@@ -1969,6 +1969,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
 #pragma unroll
                 for (int v = 0; v < 9; ++v) loc[v] = a.spec_val[9 * (long long)(e0 + o) + v];
                 s_ent[t] = -1;
+                live = true;
             }
         }
         int tsl[3];
@@ -2005,7 +2006,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
         for (int ra = 0; ra < a.rounds_t; ++ra) {
             bool tact[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) tact[i] = (tsl[i] & 0xff) != 0xff && (tsl[i] >> 8) == ra;
+            for (int i = 0; i < 3; ++i) tact[i] = live && (tsl[i] & 0xff) != 0xff && (tsl[i] >> 8) == ra;
             for (int rb = 0; rb < nrb; ++rb) {
                 if constexpr (MIXED && FAST && !BY_LEVEL) {
                     int ix[9];
