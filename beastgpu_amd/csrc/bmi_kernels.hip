@@ -900,9 +900,9 @@ hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob
 // ------------------------------------------------------- special pairs
 
 // Accumulator of one lane for a special pair: moment form for RWG (16 of
-// the 18 for the single layer, 32 for the double layer), 3x3 shape products
+// the 18 for the single layer, 24 for the double layer), 3x3 shape products
 // for P1.
-constexpr int spec_nacc(int op) { return op == OPK_DL ? 32 : 18; }
+constexpr int spec_nacc(int op) { return op == OPK_DL ? 24 : 18; }
 template <int OP>
 struct SpecAcc {
     double v[spec_nacc(OP)];
@@ -949,37 +949,36 @@ __device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const 
     if constexpr (OP == OPK_DL) {
         // (n x a_c).(grad G x b_c') = q [(n.r)(a_c.b_c') - (n.b_c')(a_c.r)] for the
         // chart vertices c, c' (a_c = x - X_c, b_c' = y - Y_c', relative to the
-        // centroids); summed over the nodes it is a combination of eight
-        // moments of h = w q (dl_epilogue):
-        //   M1 = h n.r, M2 = h (n.r)(x.y), M3 = h (n.r) y, M4 = h (n.r) x,
-        //   M5 = h (n.y)(x.r), M6 = h (n.y) r, M7 = h x.r, M8 = h r
+        // centroids); summed over the nodes it is a combination of moments of
+        // h = w q, twelve complex accumulators:
+        //   M1 = h n.r, M25 = h [(n.r)(x.y) - (n.y)(x.r)], M7 = h x.r,
+        //   M63 = h [(n.y) r - (n.r) y], M4 = h (n.r) x, M8 = h r
+        // entry (c, c') = M25 + X.M63 - Y.(M4 - X M1) + (n.Y)(M7 - X.M8)
         const double r0 = dc[0] + x[0] - y[0], r1 = dc[1] + x[1] - y[1], r2 = dc[2] + x[2] - y[2];
         double hr, hi;
         green_grad64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), r0, r1, r2, hr, hi, tab);
-        const double nr = nrm[0] * r0 + nrm[1] * r1 + nrm[2] * r2;
-        const double ny = nrm[0] * y[0] + nrm[1] * y[1] + nrm[2] * y[2];
-        const double xy = x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
-        const double xr = x[0] * r0 + x[1] * r1 + x[2] * r2;
+        const double nr = fma(nrm[2], r2, fma(nrm[1], r1, nrm[0] * r0));
+        const double ny = fma(nrm[2], y[2], fma(nrm[1], y[1], nrm[0] * y[0]));
+        const double xy = fma(x[2], y[2], fma(x[1], y[1], x[0] * y[0]));
+        const double xr = fma(x[2], r2, fma(x[1], r1, x[0] * r0));
         const double r[3] = {r0, r1, r2};
-        const double hnr = hr * nr, hni = hi * nr, hyr = hr * ny, hyi = hi * ny;
+        const double hnr = hr * nr, hni = hi * nr;
+        const double t25 = fma(nr, xy, -ny * xr);
         acc.v[0] += hnr;
         acc.v[1] += hni;
-        acc.v[2] = fma(hnr, xy, acc.v[2]);
-        acc.v[3] = fma(hni, xy, acc.v[3]);
-        acc.v[4] = fma(hyr, xr, acc.v[4]);
-        acc.v[5] = fma(hyi, xr, acc.v[5]);
-        acc.v[6] = fma(hr, xr, acc.v[6]);
-        acc.v[7] = fma(hi, xr, acc.v[7]);
+        acc.v[2] = fma(hr, t25, acc.v[2]);
+        acc.v[3] = fma(hi, t25, acc.v[3]);
+        acc.v[4] = fma(hr, xr, acc.v[4]);
+        acc.v[5] = fma(hi, xr, acc.v[5]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            acc.v[8 + d] = fma(hnr, y[d], acc.v[8 + d]);
-            acc.v[11 + d] = fma(hni, y[d], acc.v[11 + d]);
-            acc.v[14 + d] = fma(hnr, x[d], acc.v[14 + d]);
-            acc.v[17 + d] = fma(hni, x[d], acc.v[17 + d]);
-            acc.v[20 + d] = fma(hyr, r[d], acc.v[20 + d]);
-            acc.v[23 + d] = fma(hyi, r[d], acc.v[23 + d]);
-            acc.v[26 + d] = fma(hr, r[d], acc.v[26 + d]);
-            acc.v[29 + d] = fma(hi, r[d], acc.v[29 + d]);
+            const double t63 = fma(ny, r[d], -nr * y[d]);
+            acc.v[6 + d] = fma(hr, t63, acc.v[6 + d]);
+            acc.v[9 + d] = fma(hi, t63, acc.v[9 + d]);
+            acc.v[12 + d] = fma(hnr, x[d], acc.v[12 + d]);
+            acc.v[15 + d] = fma(hni, x[d], acc.v[15 + d]);
+            acc.v[18 + d] = fma(hr, r[d], acc.v[18 + d]);
+            acc.v[21 + d] = fma(hi, r[d], acc.v[21 + d]);
         }
         return;
     }
@@ -1474,18 +1473,18 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             } else {
                 double pr, pi;
                 if constexpr (OP == OPK_DL) {
-                    // M2 - X.M3 - Y.M4 + (X.Y) M1 - M5 + X.M6 + (n.Y)(M7 - X.M8)
-                    double tr = red[2] - red[4], ti = red[3] - red[5];
-                    double qr = red[6], qi = red[7];
+                    // M25 + X.M63 - Y.(M4 - X M1) + (n.Y)(M7 - X.M8)
+                    double tr = red[2], ti = red[3];
+                    double qr = red[4], qi = red[5];
                     double ur[3], ui[3];
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        tr += Xc[d] * (red[20 + d] - red[8 + d]);
-                        ti += Xc[d] * (red[23 + d] - red[11 + d]);
-                        ur[d] = red[14 + d] - Xc[d] * red[0];
-                        ui[d] = red[17 + d] - Xc[d] * red[1];
-                        qr -= Xc[d] * red[26 + d];
-                        qi -= Xc[d] * red[29 + d];
+                        tr = fma(Xc[d], red[6 + d], tr);
+                        ti = fma(Xc[d], red[9 + d], ti);
+                        ur[d] = fma(-Xc[d], red[0], red[12 + d]);
+                        ui[d] = fma(-Xc[d], red[1], red[15 + d]);
+                        qr = fma(-Xc[d], red[18 + d], qr);
+                        qi = fma(-Xc[d], red[21 + d], qi);
                     }
                     const double nY = nrm[0] * Yc[0] + nrm[1] * Yc[1] + nrm[2] * Yc[2];
                     pr = tr - (Yc[0] * ur[0] + Yc[1] * ur[1] + Yc[2] * ur[2]) + nY * qr;
