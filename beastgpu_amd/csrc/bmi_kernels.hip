@@ -312,7 +312,8 @@ __device__ __forceinline__ void hh_epilogue(const OpConst &op, const double *pr,
 //   (n x a_i).(grad G x b_j) = q [p (a_i.b_j) - (n.b_j)(a_i.r)]
 // = q [p x'.y' - beta_j.(p x') - alpha_i.(p y') + (alpha_i.beta_j) p
 //      - t x'.r + alpha_i.(t r) + (n.beta_j) x'.r - (n.beta_j) alpha_i.r],
-// so eight moments of h = w w' q carry the pair; the 1/(2A) of each RWG
+// so moments of h = w w' q carry the pair (eight, of which two pairs enter
+// only as differences and are accumulated so); the 1/(2A) of each RWG
 // shape cancels the Jacobian 2A of the point weights.  S is the arithmetic
 // type (double, or float for the mixed path).
 template <typename S, int FAST, bool TAB, typename G = double>
@@ -337,15 +338,18 @@ __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule
     }
     const S n0 = (S)nx, n1 = (S)ny, n2 = (S)nz;
     const S nd = n0 * dcx + n1 * dcy + n2 * dcz;
-    S Ar = 0, Ai = 0, Dr = 0, Di = 0, Er = 0, Ei = 0, Gr = 0, Gi = 0;
-    S Br[3] = {}, Bi[3] = {}, Cr[3] = {}, Ci[3] = {}, Fr[3] = {}, Fi[3] = {}, Hr[3] = {}, Hi[3] = {};
+    // Of the eight moments only A - E and F - C enter the entries, and in both
+    // the sums of h t y' cancel: AE = sum nd (x'.h y') - (x'.u) h t and
+    // FC = sum u h t - nd h y' (u = d + x')
+    S AEr = 0, AEi = 0, Dr = 0, Di = 0, Gr = 0, Gi = 0;
+    S Br[3] = {}, Bi[3] = {}, FCr[3] = {}, FCi[3] = {}, Hr[3] = {}, Hi[3] = {};
 #pragma unroll
     for (int I = 0; I < BMI_MAX_RULE; ++I) {
         if (I >= rule.nt()) break;
         const S x0 = (S)xp[3 * I], x1 = (S)xp[3 * I + 1], x2 = (S)xp[3 * I + 2];
         const S u0 = dcx + x0, u1 = dcy + x1, u2 = dcz + x2;
         S shr = 0, shi = 0, str = 0, sti = 0;
-        S syr[3] = {}, syi[3] = {}, styr[3] = {}, styi[3] = {};
+        S syr[3] = {}, syi[3] = {};
 #pragma unroll
         for (int J = 0; J < BMI_MAX_RULE; ++J) {
             if (J >= rule.nb()) break;
@@ -356,41 +360,34 @@ __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule
             } else {
                 green_grad32(op, lossy, (float)rule.w4pi(I, J), u0 - y0, u1 - y1, u2 - y2, hr, hi);
             }
-            const S t = n0 * y0 + n1 * y1 + n2 * y2;
-            const S htr = hr * t, hti = hi * t;
+            const S t = fma(n2, y2, fma(n1, y1, n0 * y0));
             shr += hr;
             shi += hi;
-            str += htr;
-            sti += hti;
+            str = fma(hr, t, str);
+            sti = fma(hi, t, sti);
             syr[0] = fma(hr, y0, syr[0]); syr[1] = fma(hr, y1, syr[1]); syr[2] = fma(hr, y2, syr[2]);
             syi[0] = fma(hi, y0, syi[0]); syi[1] = fma(hi, y1, syi[1]); syi[2] = fma(hi, y2, syi[2]);
-            styr[0] = fma(htr, y0, styr[0]); styr[1] = fma(htr, y1, styr[1]); styr[2] = fma(htr, y2, styr[2]);
-            styi[0] = fma(hti, y0, styi[0]); styi[1] = fma(hti, y1, styi[1]); styi[2] = fma(hti, y2, styi[2]);
         }
-        // sums over J of h p and h p y' (p = n.d - t)
-        const S pr = nd * shr - str, pi = nd * shi - sti;
-        const S pyr[3] = {nd * syr[0] - styr[0], nd * syr[1] - styr[1], nd * syr[2] - styr[2]};
-        const S pyi[3] = {nd * syi[0] - styi[0], nd * syi[1] - styi[1], nd * syi[2] - styi[2]};
-        const S xu = x0 * u0 + x1 * u1 + x2 * u2;  // x'.(d + x')
+        // sums over J of h p (p = n.d - t)
+        const S pr = fma(nd, shr, -str), pi = fma(nd, shi, -sti);
+        const S xu = fma(x2, u2, fma(x1, u1, x0 * u0));  // x'.(d + x')
+        const S xsr = fma(x2, syr[2], fma(x1, syr[1], x0 * syr[0]));
+        const S xsi = fma(x2, syi[2], fma(x1, syi[1], x0 * syi[0]));
         const S x[3] = {x0, x1, x2}, u[3] = {u0, u1, u2};
         Dr += pr;
         Di += pi;
-        Ar += x0 * pyr[0] + x1 * pyr[1] + x2 * pyr[2];
-        Ai += x0 * pyi[0] + x1 * pyi[1] + x2 * pyi[2];
-        Er += xu * str - (x0 * styr[0] + x1 * styr[1] + x2 * styr[2]);
-        Ei += xu * sti - (x0 * styi[0] + x1 * styi[1] + x2 * styi[2]);
-        Gr += xu * shr - (x0 * syr[0] + x1 * syr[1] + x2 * syr[2]);
-        Gi += xu * shi - (x0 * syi[0] + x1 * syi[1] + x2 * syi[2]);
+        AEr = fma(nd, xsr, fma(-xu, str, AEr));
+        AEi = fma(nd, xsi, fma(-xu, sti, AEi));
+        Gr = fma(xu, shr, Gr - xsr);
+        Gi = fma(xu, shi, Gi - xsi);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             Br[k] = fma(x[k], pr, Br[k]);
             Bi[k] = fma(x[k], pi, Bi[k]);
-            Cr[k] += pyr[k];
-            Ci[k] += pyi[k];
-            Fr[k] += u[k] * str - styr[k];
-            Fi[k] += u[k] * sti - styi[k];
-            Hr[k] += u[k] * shr - syr[k];
-            Hi[k] += u[k] * shi - syi[k];
+            FCr[k] = fma(u[k], str, fma(-nd, syr[k], FCr[k]));
+            FCi[k] = fma(u[k], sti, fma(-nd, syi[k], FCi[k]));
+            Hr[k] = fma(u[k], shr, Hr[k] - syr[k]);
+            Hi[k] = fma(u[k], shi, Hi[k] - syi[k]);
         }
     }
     const S alr = (S)op.alpha_re, ali = (S)op.alpha_im;
@@ -399,8 +396,8 @@ __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule
         const S a0 = (S)xa[3 * i], a1 = (S)xa[3 * i + 1], a2 = (S)xa[3 * i + 2];
         // terms that depend on i only (chains of fused multiply-adds, one
         // instruction per term)
-        const S ir = fma(a2, Fr[2] - Cr[2], fma(a1, Fr[1] - Cr[1], fma(a0, Fr[0] - Cr[0], Ar - Er)));
-        const S ii = fma(a2, Fi[2] - Ci[2], fma(a1, Fi[1] - Ci[1], fma(a0, Fi[0] - Ci[0], Ai - Ei)));
+        const S ir = fma(a2, FCr[2], fma(a1, FCr[1], fma(a0, FCr[0], AEr)));
+        const S ii = fma(a2, FCi[2], fma(a1, FCi[1], fma(a0, FCi[0], AEi)));
         const S gr = Gr - fma(a2, Hr[2], fma(a1, Hr[1], a0 * Hr[0]));
         const S gi = Gi - fma(a2, Hi[2], fma(a1, Hi[1], a0 * Hi[0]));
 #pragma unroll
