@@ -1848,7 +1848,10 @@ __device__ __forceinline__ int tile_ix(int r, int c) { return c * BMI_PANEL_ROWS
 // with a barrier between rounds, so no entry is touched twice inside a round
 // and the summation order is fixed.  RWG functions have at most two cells, so
 // there are at most 2 x 2 rounds.
-template <int OP, int MIXED, int FAST, typename OutT>
+// REFETCH: recompute the lane offsets of the per-sub-batch record loads
+// (the single-problem kernels; measured faster there, slower in the batched
+// ones).
+template <int OP, int MIXED, int FAST, typename OutT, bool REFETCH = false>
 __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, double2 *tile, PanelShared &sh) {
     auto &sg = sh.sg;
     auto &bg = sh.bg;
@@ -1904,8 +1907,14 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
         pg0 = g[t];
         if (t < BMI_GEO - BMI_BLOCK_THREADS) pg1 = g[BMI_BLOCK_THREADS + t];
         if (t < BMI_MAX_ROWS) {
-            psl = a.tslot[(long long)s * BMI_MAX_ROWS + t];
-            pcf = a.tscf[(long long)s * BMI_MAX_ROWS + t];
+            // REFETCH: the lane offset is recomputed here (an empty asm hides
+            // that it is loop-invariant); a per-lane 64-bit pointer kept
+            // across the sub-batch loop costs two registers the FP64 kernels
+            // lack (the compiler spilled it)
+            int tt = t;
+            if constexpr (REFETCH) asm volatile("" : "+v"(tt));
+            psl = a.tslot[(long long)s * BMI_MAX_ROWS + tt];
+            pcf = a.tscf[(long long)s * BMI_MAX_ROWS + tt];
         }
         const long long b = (long long)s * a.npb + q;
         pe0 = a.blk_off[b];
@@ -2111,7 +2120,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
     const int w = blockIdx.x;
     const int P = w / a.nbq;
     const int q = a.bq_list[w - P * a.nbq];
-    panel_block<OP, MIXED, FAST, OutT>(a, P, q, tile, sh);
+    panel_block<OP, MIXED, FAST, OutT, true>(a, P, q, tile, sh);
 }
 
 // Batched, one colour at a time: work-group i takes item i = (problem, panel,
