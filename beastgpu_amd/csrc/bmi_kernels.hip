@@ -458,6 +458,40 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
             m.mxyi = fma(x2, ti2, fma(x1, ti1, fma(x0, ti0, m.mxyi)));
         }
         mw_epilogue(op, m, xa, ya, loc);
+    } else if constexpr (FAST) {
+        // default rules: trial point J > 0 has shape value 3/5 at vertex J - 1
+        // and 1/5 at the others, J = 0 is the centroid; test point I has 2/3
+        // at vertex I and 1/6 at the others.  So per test point
+        //   v_j = h_0 / 3 + (h_1 + h_2 + h_3) / 5 + (2/5) h_(j+1)
+        // and 6 sum_I lt(I, i) v_Ij = 3 v_ij + sum_I v_Ij: no product per shape
+        double vr[3][3], vi[3][3], tr[3] = {}, ti[3] = {};
+#pragma unroll
+        for (int I = 0; I < 3; ++I) {
+            const double ux = dcx + xp[3 * I], uy = dcy + xp[3 * I + 1], uz = dcz + xp[3 * I + 2];
+            double hr[4], hi[4];
+#pragma unroll
+            for (int J = 0; J < 4; ++J)
+                green64<TAB>(op, false, rule.w4pi(I, J), ux - yp[3 * J], uy - yp[3 * J + 1], uz - yp[3 * J + 2],
+                             hr[J], hi[J], tab);
+            const double br = fma(0.2, hr[1] + hr[2] + hr[3], hr[0] * (1.0 / 3.0));
+            const double bi = fma(0.2, hi[1] + hi[2] + hi[3], hi[0] * (1.0 / 3.0));
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                vr[I][j] = fma(0.4, hr[j + 1], br);
+                vi[I][j] = fma(0.4, hi[j + 1], bi);
+                tr[j] += vr[I][j];
+                ti[j] += vi[I][j];
+            }
+        }
+        double pr[9], pi[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                pr[3 * i + j] = fma(3.0, vr[i][j], tr[j]);
+                pi[3 * i + j] = fma(3.0, vi[i][j], ti[j]);
+            }
+        hh_epilogue(op, pr, pi, 4.0 * xs * ys * (1.0 / 6.0), loc);
     } else {
         double pr[9] = {}, pi[9] = {};
 #pragma unroll
@@ -571,6 +605,37 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
                 loc[3 * i + j].y = (double)fmaf(alr, vi, fmaf(ali, vr, di));
             }
         }
+    } else if constexpr (FAST) {
+        // default rules, as in pair_fp64
+        float vr[3][3], vi[3][3], tr[3] = {}, ti[3] = {};
+#pragma unroll
+        for (int I = 0; I < 3; ++I) {
+            const float ux = dcx + (float)xp[3 * I], uy = dcy + (float)xp[3 * I + 1], uz = dcz + (float)xp[3 * I + 2];
+            float hr[4], hi[4];
+#pragma unroll
+            for (int J = 0; J < 4; ++J)
+                green32(op, false, (float)rule.w4pi(I, J), ux - ypf[3 * J], uy - ypf[3 * J + 1], uz - ypf[3 * J + 2],
+                        hr[J], hi[J]);
+            const float br = fmaf(0.2f, hr[1] + hr[2] + hr[3], hr[0] * (1.0f / 3.0f));
+            const float bi = fmaf(0.2f, hi[1] + hi[2] + hi[3], hi[0] * (1.0f / 3.0f));
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                vr[I][j] = fmaf(0.4f, hr[j + 1], br);
+                vi[I][j] = fmaf(0.4f, hi[j + 1], bi);
+                tr[j] += vr[I][j];
+                ti[j] += vi[I][j];
+            }
+        }
+        const float sc = 4.0f * (float)(xs * ys * (1.0 / 6.0));
+        const float ar = (float)op.alpha_re * sc, ai = (float)op.alpha_im * sc;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const float pr = fmaf(3.0f, vr[i][j], tr[j]), pi = fmaf(3.0f, vi[i][j], ti[j]);
+                loc[3 * i + j].x = (double)fmaf(ar, pr, -ai * pi);
+                loc[3 * i + j].y = (double)fmaf(ar, pi, ai * pr);
+            }
     } else {
         float pr[9] = {}, pi[9] = {};
 #pragma unroll
