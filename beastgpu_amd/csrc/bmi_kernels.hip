@@ -1912,7 +1912,9 @@ __device__ __forceinline__ int tile_ix(int r, int c) { return c * BMI_PANEL_ROWS
 // with the same row (column), the additions go in rounds (rank_t, rank_b)
 // with a barrier between rounds, so no entry is touched twice inside a round
 // and the summation order is fixed.  RWG functions have at most two cells, so
-// there are at most 2 x 2 rounds.
+// there are at most 2 x 2 rounds.  A trial patch runs only the rounds it
+// needs; the Helmholtz kernels and the FP64 EFIE kernel with the fixed rule
+// round by level instead of rank (below).
 // REFETCH: recompute the lane offsets of the per-sub-batch record loads
 // (the single-problem kernels; measured faster there, slower in the batched
 // ones).
@@ -2068,7 +2070,8 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
         // (which has the registers) issues all of its reads before its
         // writes: one LDS wait per round instead of one per entry; unused
         // slots read entry 0 and write nothing.  Trial rounds by level (the
-        // Helmholtz kernels): two lanes may add to one entry in one round
+        // Helmholtz kernels, the FP64 EFIE kernel with the fixed rule): two
+        // lanes may add to one entry in one round
         // only through different local shapes j, and the updates go shape
         // by shape in program order, each shape's reads after the previous
         // shape's writes.
@@ -2251,7 +2254,8 @@ template <int OP, int MIXED, int FAST>
 static void launch_panels_t(const PanelArgs &a, int c32, hipStream_t s) {
     const unsigned grid = (unsigned)((long long)a.npanel * a.nbq);
     if (a.rr) {
-        // few rounds (RWG: at most 2): rank rounds into an LDS tile
+        // few rounds (RWG: at most 2; Lagrange by level: 2-4): rounds into an
+        // LDS tile
         const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * a.ncm;
         if (c32)
             hipLaunchKernelGGL((k_panels_rr<OP, MIXED, FAST, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
