@@ -5,6 +5,7 @@
 #include "bmi_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -284,7 +285,10 @@ struct bmi_space {
     DevBuf<double> d_patch_sph, d_patch_amax;
     DevBuf<int> d_home_patch, d_home_sb;  // cell -> its patch / its first sub-batch (-1: none)
     std::vector<int> colour_off;  // offsets of each colour's list in d_by_colour
-    // test role: panels of BMI_PANEL_ROWS functions in sub-batches of BMI_PATCH cells
+    // test role: panels of prows functions in sub-batches of BMI_PATCH cells;
+    // trial role: patches of at most maxcols functions (Lagrange: wide panels,
+    // narrow patches)
+    int prows = BMI_PANEL_ROWS, maxcols = BMI_MAX_COLS;
     int npanel = 0, nsb = 0;
     std::vector<int> sb_off, sb_cells, sb_row_ptr;
     std::vector<unsigned char> sb_slot_kl;
@@ -427,7 +431,7 @@ void parallel_for(int n, F f) {
 bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, const std::vector<int> &morton_rank) {
     const bmi_mesh *m = s->mesh;
     const int64_t nt = m->nt;
-    const int R = BMI_PANEL_ROWS;
+    const int R = s->prows;
     const int np = (int)((s->nfun + R - 1) / R);
     s->npanel = np;
     std::vector<int> cnt(np + 1, 0);
@@ -477,25 +481,30 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
         // such conflict (one more sub-batch if none has).  So every row has at
         // most one cell per sub-batch (its rank in the sub-batch is 0).
         const int ncell = cnt[P + 1] - cnt[P];
+        // rows of the panel as a 128-bit mask (two words)
         std::vector<std::vector<int>> bins;
-        std::vector<uint64_t> used;
+        std::vector<std::array<uint64_t, 2>> used;
         for (int nb = std::max(1, (ncell + BMI_PATCH - 1) / BMI_PATCH);; ++nb) {
             bins.assign(nb, {});
-            used.assign(nb, 0);
+            used.assign(nb, {0, 0});
             bool ok = true;
             for (int i = 0; i < ncell && ok; ++i) {
                 const int K = cells[cnt[P] + i];
-                uint64_t rows = 0;
+                std::array<uint64_t, 2> rows = {0, 0};
                 for (int j = 0; j < 3; ++j) {
                     const int64_t f = gidx[3 * (int64_t)K + j];
-                    if (f >= 0 && f / R == P) rows |= 1ull << (f - (int64_t)P * R);
+                    if (f >= 0 && f / R == P) {
+                        const int r = (int)(f - (int64_t)P * R);
+                        rows[r >> 6] |= 1ull << (r & 63);
+                    }
                 }
                 ok = false;
                 for (int d = 0; d < nb; ++d) {
                     const int b = (i + d) % nb;
-                    if ((int)bins[b].size() < BMI_PATCH && !(used[b] & rows)) {
+                    if ((int)bins[b].size() < BMI_PATCH && !(used[b][0] & rows[0]) && !(used[b][1] & rows[1])) {
                         bins[b].push_back(K);
-                        used[b] |= rows;
+                        used[b][0] |= rows[0];
+                        used[b][1] |= rows[1];
                         ok = true;
                         break;
                     }
@@ -572,7 +581,7 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
     s->rounds_t = 1;
     for (int P = 0; P < np; ++P)
         for (int b = s->sb_off[P]; b < s->sb_off[P + 1]; ++b) {
-            int seen[BMI_PANEL_ROWS] = {0};
+            int seen[BMI_PANEL_ROWS_WIDE] = {0};
             for (int k = 0; k < BMI_PATCH; ++k) {
                 const int K = s->sb_cells[(size_t)b * BMI_PATCH + k];
                 if (K < 0) continue;
@@ -639,7 +648,7 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
     for (int64_t i = 0; i < nt; ++i) morton_rank[order[i]] = (int)i;
 
     // runs of at most BMI_PATCH consecutive cells carrying at most
-    // BMI_MAX_COLS functions (bounds the LDS tile of the far kernel)
+    // s->maxcols functions (bounds the LDS tile of the far kernel)
     s->patch_tri.clear();
     {
         std::vector<int64_t> fs;
@@ -658,7 +667,7 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
                         std::find(add, add + na, f) == add + na)
                         add[na++] = f;
                 }
-                if (n > 0 && (int)fs.size() + na > BMI_MAX_COLS) break;
+                if (n > 0 && (int)fs.size() + na > s->maxcols) break;
                 fs.insert(fs.end(), add, add + na);
                 s->patch_tri.push_back(K);
                 ++n;
@@ -951,7 +960,9 @@ PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) {
     B.rounds_b = by_level ? P->Y->levels_b : P->Y->rounds_b;
     B.brounds = P->Y->d_brounds.p + (by_level ? P->Y->npatch : 0);
     B.ncm = P->Y->ncmax;
-    B.rr = P->X->rounds_t * B.rounds_b <= 6;
+    // Lagrange test spaces have wide panels, which only the rank-round
+    // kernels handle
+    B.rr = P->X->prows != BMI_PANEL_ROWS || P->X->rounds_t * B.rounds_b <= 6;
     B.npb = P->npb;
     B.blk_off = P->d_blk_off.p;
     B.blk_cnt = P->d_blk_cnt.p;
@@ -1119,6 +1130,10 @@ bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, c
     s->mesh = mesh;
     s->kind = kind;
     s->nfun = nfun;
+    if (kind == BMI_SPACE_LAGRANGE_C0D1) {
+        s->prows = BMI_PANEL_ROWS_WIDE;
+        s->maxcols = BMI_MAX_COLS_WIDE;
+    }
     std::vector<int> rank;
     bmi_status st = build_patches(s.get(), gidx, coeff, rank);
     if (st != BMI_OK) return st;

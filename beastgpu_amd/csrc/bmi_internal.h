@@ -16,7 +16,13 @@
 // output rows owned by one test panel (consecutive functions); a column of
 // a panel is one contiguous 1 KB (ComplexF64) segment of the matrix
 #define BMI_PANEL_ROWS 64
-static_assert(BMI_PANEL_ROWS <= 64, "a panel's rows are tracked in a 64-bit mask");
+// Lagrange spaces: test panels of 128 rows (a vertex function has about 6
+// cells, so 64 consecutive vertices are a thin strip whose border cells repeat
+// in the neighbouring panels), trial patches of at most 20 columns; the tile
+// has the same size
+#define BMI_PANEL_ROWS_WIDE 128
+#define BMI_MAX_COLS_WIDE 20
+static_assert(BMI_PANEL_ROWS_WIDE * BMI_MAX_COLS_WIDE <= BMI_PANEL_ROWS * BMI_MAX_COLS, "wide panels need a larger tile");
 // accumulators per thread: 64 rows x 48 columns / 256 threads
 #define BMI_PANEL_ACC (BMI_PANEL_ROWS * BMI_MAX_ROWS / BMI_BLOCK_THREADS)
 #define BMI_MAX_RULE 4      // far-rule points per cell
@@ -76,7 +82,7 @@ static_assert((BMI_GEOF_VREL + 9 * BMI_PATCH) * sizeof(float) <= BMI_GEO_CEN * s
               "FP32 geometry overlaps the centroids");
 
 // Far-pair assembly.  Test side: panels of BMI_PANEL_ROWS consecutive
-// functions, each with the cells that carry them (cells of several panels are
+// functions (BMI_PANEL_ROWS_WIDE for Lagrange spaces), each with the cells that carry them (cells of several panels are
 // repeated), cut into sub-batches of BMI_PATCH cells; per sub-batch a slot
 // table: for each of the 64 rows the slots (local cell * 3 + shape,
 // coefficient) of that row's function.  Trial side: Morton patches of
@@ -88,7 +94,7 @@ struct PanelArgs {
     int nrows;                   // functions of the test space
     const int *sb_off;           // npanel + 1: sub-batches of each panel
     const double *tgeo;          // nsb x BMI_GEO
-    const int *trs;              // nsb x (BMI_PANEL_ROWS + 1): slot range of each row (0..48)
+    const int *trs;              // nsb x (BMI_PANEL_ROWS + 1): slot range of each row (0..48); 64-row panels only
     const unsigned char *tkl;    // nsb x 48
     const double *tcf;           // nsb x 48
     const double *bgeo;          // npb x BMI_GEO

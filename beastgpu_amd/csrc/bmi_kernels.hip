@@ -1898,11 +1898,18 @@ struct PanelShared {
 // column, the row index XOR-ed with c mod 16, so that the 16 lanes of a rank
 // round (one row, up to 16 columns) fall into different LDS banks while each
 // column stays one contiguous 1 KB segment (one wave's direct-to-LDS load).
-__device__ __forceinline__ int tile_ix(int r, int c) { return c * BMI_PANEL_ROWS + (r ^ (c & 15)); }
+template <int PR>
+__device__ __forceinline__ int tile_ix(int r, int c) { return c * PR + (r ^ (c & 15)); }
+
+// Rows of a test panel and the widest trial patch of the operator's spaces:
+// the Helmholtz operator works on Lagrange spaces (wide panels, narrow
+// patches), the Maxwell operators on RWG.
+__host__ __device__ constexpr int panel_rows(int op) { return op == OPK_HH3D ? BMI_PANEL_ROWS_WIDE : BMI_PANEL_ROWS; }
+__host__ __device__ constexpr int panel_cols(int op) { return op == OPK_HH3D ? BMI_MAX_COLS_WIDE : BMI_MAX_COLS; }
 
 // One (test panel P, trial patch q) block of the far-pair assembly with its
 // write-back; the caller has filled sh.s_sc (FP64 path) and passes the tile
-// of BMI_PANEL_ROWS x (widest patch) entries.  Ends after the write-back stores are
+// of panel_rows(OP) x (widest patch) entries.  Ends after the write-back stores are
 // issued; a caller that reuses the LDS for another block synchronises first.
 //
 // Rank rounds: each thread adds its pair's 9 values straight into the LDS
@@ -1928,8 +1935,9 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     auto &s_cfirst = sh.s_cfirst;
     const double2 *s_sc = sh.s_sc;
     const int t = threadIdx.x;
-    const int r0 = P * BMI_PANEL_ROWS;
-    const int nr = min(BMI_PANEL_ROWS, a.nrows - r0);
+    constexpr int PR = panel_rows(OP);
+    const int r0 = P * PR;
+    const int nr = min(PR, a.nrows - r0);
     const int nc = a.bnc[q];
     const int kl = t >> 4, ll = t & 15;
     const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
@@ -1938,7 +1946,9 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     constexpr bool BY_LEVEL = OP == OPK_HH3D || (FAST && !MIXED && OP == OPK_MW);
     constexpr int RSH = BY_LEVEL ? 16 : 8;
     const int lane = t & 63, wv = t >> 6;
-    for (int c = wv; c < nc; c += BMI_BLOCK_THREADS / 64) tile[c * BMI_PANEL_ROWS + lane] = make_double2(0.0, 0.0);
+    for (int c = wv; c < nc; c += BMI_BLOCK_THREADS / 64)
+#pragma unroll
+        for (int h = 0; h < PR / 64; ++h) tile[c * PR + 64 * h + lane] = make_double2(0.0, 0.0);
     short *s_ent = sh.s_ent;
     s_ent[t] = -1;
     // MIXED: the rule points and vertices of a geometry record are kept in
@@ -2090,7 +2100,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
 #pragma unroll
                         for (int j = 0; j < 3; ++j) {
                             act[3 * i + j] = tact[i] && (bsl[j] & 0xff) != 0xff && ((bsl[j] >> 8) & 0xff) == rb;
-                            ix[3 * i + j] = act[3 * i + j] ? tile_ix(tsl[i] & 0xff, bsl[j] & 0xff) : 0;
+                            ix[3 * i + j] = act[3 * i + j] ? tile_ix<PR>(tsl[i] & 0xff, bsl[j] & 0xff) : 0;
                             z[3 * i + j] = tile[ix[3 * i + j]];
                         }
 #pragma unroll
@@ -2115,7 +2125,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
 #pragma unroll
                         for (int i = 0; i < 3; ++i) {
                             act[i] = tact[i] && bj;
-                            ix[i] = act[i] ? tile_ix(tsl[i] & 0xff, bsl[j] & 0xff) : 0;
+                            ix[i] = act[i] ? tile_ix<PR>(tsl[i] & 0xff, bsl[j] & 0xff) : 0;
                             z[i] = tile[ix[i]];
                         }
 #pragma unroll
@@ -2138,7 +2148,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                         for (int i = 0; i < 3; ++i) {
                             if (!tact[i]) continue;
                             const double cc = tcf[i] * bcf[j];
-                            double2 &z = tile[tile_ix(tsl[i] & 0xff, col)];
+                            double2 &z = tile[tile_ix<PR>(tsl[i] & 0xff, col)];
                             z.x = fma(cc, loc[3 * i + j].x, z.x);
                             z.y = fma(cc, loc[3 * i + j].y, z.y);
                         }
@@ -2155,34 +2165,40 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     // the column's segment (row l ^ (c mod 16)).  The stored values of all of
     // the wave's read-modify-write columns are loaded first, so that the
     // loads are in flight together, then added and stored.
+    // (wide panels: each column in PR / 64 segments of 64 rows)
     OutT *Z = (OutT *)a.Z;
-    constexpr int NCW = (BMI_MAX_COLS + BMI_BLOCK_THREADS / 64 - 1) / (BMI_BLOCK_THREADS / 64);
-    OutT old[NCW];
+    constexpr int NH = PR / 64;
+    constexpr int NCW = (panel_cols(OP) + BMI_BLOCK_THREADS / 64 - 1) / (BMI_BLOCK_THREADS / 64);
+    OutT old[NCW * NH];
 #pragma unroll
-    for (int k = 0; k < NCW; ++k) {
-        const int c = wv + k * (BMI_BLOCK_THREADS / 64);
-        const int r = lane ^ (c & 15);
-        old[k] = OutT{};
-        if (c < nc && r < nr && !s_cfirst[c]) old[k] = Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld];
-    }
+    for (int k = 0; k < NCW; ++k)
 #pragma unroll
-    for (int k = 0; k < NCW; ++k) {
-        const int c = wv + k * (BMI_BLOCK_THREADS / 64);
-        const int r = lane ^ (c & 15);
-        if (c < nc && r < nr) {
-            const double2 v = tile[c * BMI_PANEL_ROWS + lane];
-            OutT o;
-            o.x = (decltype(o.x))((double)old[k].x + v.x);
-            o.y = (decltype(o.y))((double)old[k].y + v.y);
-            Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld] = o;
+        for (int h = 0; h < NH; ++h) {
+            const int c = wv + k * (BMI_BLOCK_THREADS / 64);
+            const int r = 64 * h + (lane ^ (c & 15));
+            old[NH * k + h] = OutT{};
+            if (c < nc && r < nr && !s_cfirst[c]) old[NH * k + h] = Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld];
         }
-    }
+#pragma unroll
+    for (int k = 0; k < NCW; ++k)
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            const int c = wv + k * (BMI_BLOCK_THREADS / 64);
+            const int r = 64 * h + (lane ^ (c & 15));
+            if (c < nc && r < nr) {
+                const double2 v = tile[c * PR + 64 * h + lane];
+                OutT o;
+                o.x = (decltype(o.x))((double)old[NH * k + h].x + v.x);
+                o.y = (decltype(o.y))((double)old[NH * k + h].y + v.y);
+                Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld] = o;
+            }
+        }
 }
 
 // One work-group per (test panel P, trial patch q of this launch's colour).
 template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_rr(PanelArgs a) {
-    extern __shared__ double2 tile[];             // BMI_PANEL_ROWS x a.ncm
+    extern __shared__ double2 tile[];             // panel_rows(OP) x a.ncm
     __shared__ PanelShared sh;
     if (!MIXED) fill_sincos_table(sh.s_sc);
     const int w = blockIdx.x;
@@ -2196,7 +2212,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
 template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_items(
     const PanelArgs *args, const int4 *items) {
-    extern __shared__ double2 tile[];  // BMI_PANEL_ROWS x (the batch's widest patch)
+    extern __shared__ double2 tile[];  // panel_rows(OP) x (the batch's widest patch)
     __shared__ PanelShared sh;
     if (!MIXED) fill_sincos_table(sh.s_sc);
     const int4 it = items[blockIdx.x];
@@ -2253,10 +2269,11 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
 template <int OP, int MIXED, int FAST>
 static void launch_panels_t(const PanelArgs &a, int c32, hipStream_t s) {
     const unsigned grid = (unsigned)((long long)a.npanel * a.nbq);
-    if (a.rr) {
+    // wide panels (the Helmholtz operator) only in the rank-round kernel
+    if (a.rr || panel_rows(OP) != BMI_PANEL_ROWS) {
         // few rounds (RWG: at most 2; Lagrange by level: 2-4): rounds into an
         // LDS tile
-        const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * a.ncm;
+        const size_t lds = sizeof(double2) * panel_rows(OP) * a.ncm;
         if (c32)
             hipLaunchKernelGGL((k_panels_rr<OP, MIXED, FAST, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
         else
@@ -2295,7 +2312,7 @@ hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hi
 namespace {
 template <int OP, int MIXED, int FAST, typename OutT>
 void items_t(const PanelArgs *args, const int4 *items, int nitems, int ncm, hipStream_t s) {
-    const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * ncm;
+    const size_t lds = sizeof(double2) * panel_rows(OP) * ncm;
     hipLaunchKernelGGL((k_panels_items<OP, MIXED, FAST, OutT>), dim3(nitems), dim3(BMI_BLOCK_THREADS), lds, s, args,
                        items);
 }
@@ -2303,7 +2320,7 @@ void items_t(const PanelArgs *args, const int4 *items, int nitems, int ncm, hipS
 template <int OP, int MIXED, int FAST, typename OutT>
 void queue_t(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm, int *head, int *done,
              int *err, int spin_limit, int grid, hipStream_t s) {
-    const size_t lds = sizeof(double2) * BMI_PANEL_ROWS * ncm;
+    const size_t lds = sizeof(double2) * panel_rows(OP) * ncm;
     hipLaunchKernelGGL((k_panels_queue<OP, MIXED, FAST, OutT>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, args,
                        items, need, nitems, head, done, err, spin_limit);
 }
