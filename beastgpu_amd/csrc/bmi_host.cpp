@@ -907,6 +907,12 @@ bmi_status check_target(const bmi_plan *P, const bmi_matrix *out) {
     return BMI_OK;
 }
 
+// 1 if the far kernels' compile-time rule applies: default 3 x 4 point
+// rule, lossless operator.
+int fast_rule(const bmi_plan *P) {
+    return P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
+}
+
 // 1 if the plan's Sauter-Schwab rules all have 5 points per dimension.
 int gl5(const bmi_plan *P) {
     return P->opts.ss_order[0] == 5 && P->opts.ss_order[1] == 5 && P->opts.ss_order[2] == 5;
@@ -922,7 +928,7 @@ bmi_status enqueue_prologue(bmi_plan *P, bmi_matrix *out, hipStream_t s, hipEven
     if (ev_mid) BMI_HIP(hipEventRecord(ev_mid, s));
     const int mixed = P->opts.precision == BMI_PREC_MIXED;
     BMI_HIP(launch_special(P->d_sargs.p, P->d_cbase.p, P->d_ebase.p, 1, P->nchunks, P->capacity, P->cs, mixed,
-                           P->op_kind, gl5(P), s));
+                           P->op_kind, gl5(P), fast_rule(P), s));
     // functions without local shapes keep zero rows/columns (a fill kernel:
     // memset nodes misbehave in replayed graphs)
     if (P->X->uncovered || P->Y->uncovered)
@@ -952,7 +958,7 @@ PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) {
     B.bslot = P->Y->d_bslot.p;
     B.bscf = P->Y->d_bscf.p;
     B.rounds_t = P->X->rounds_t;
-    B.fast = P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
+    B.fast = fast_rule(P);
     // trial rounds by level: the Helmholtz kernels and the FP64 EFIE kernel
     // with the compile-time rule; the others by rank (must match panel_block)
     const bool by_level = P->op_kind == OPK_HH3D ||
@@ -1586,7 +1592,7 @@ std::vector<std::pair<std::string, std::pair<int, int>>> g_tune_cache;
 // Problems of a batch with one operator kind: one launch of the special-pair
 // kernels over all of them.
 struct BatchOpGroup {
-    int op_kind = 0, mixed = 0, ng5 = 1;
+    int op_kind = 0, mixed = 0, ng5 = 1, fast = 1;
     std::vector<int> problems;
     int total_chunks = 0, total_entries = 0, cs = 256;
     DevBuf<SpecialArgs> d_sargs;
@@ -1739,7 +1745,7 @@ bmi_status enqueue_batch_prologue(bmi_batch *Bt, hipStream_t s) {
     for (const bmi_plan *P : Bt->plans) P->filled = true;
     for (auto &G : Bt->opgroups)
         BMI_HIP(launch_special(G.d_sargs.p, G.d_cbase.p, G.d_ebase.p, (int)G.problems.size(), G.total_chunks,
-                               G.total_entries, G.cs, G.mixed, G.op_kind, G.ng5, s));
+                               G.total_entries, G.cs, G.mixed, G.op_kind, G.ng5, G.fast, s));
     for (int i = 0; i < n; ++i) {
         const bmi_plan *P = Bt->plans[i];
         const bmi_matrix *M = Bt->mats[i];
@@ -1778,6 +1784,7 @@ bmi_status build_prologue_tables(bmi_batch *Bt) {
         G->problems.push_back(i);
         G->mixed |= P->opts.precision == BMI_PREC_MIXED;
         G->ng5 &= gl5(P);
+        G->fast &= fast_rule(P);
     }
     for (auto &G : Bt->opgroups) {
         int cap = 0;

@@ -175,9 +175,10 @@ hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob
 hipError_t launch_zero_each(int *const *ptrs, int n, hipStream_t s);
 // special pairs, one problem or a batch of one operator: cbase / ebase are
 // prefixes of the problems' chunk counts (chunks of cs entries) and capacities
-// ng5: every problem uses 5 Gauss-Legendre points per dimension for CV, CE, CF
+// ng5: every problem uses 5 Gauss-Legendre points per dimension for CV, CE, CF;
+// fast: every problem has the default 3 x 4 far rule and a lossless operator
 hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *ebase, int nprob, int total_chunks,
-                          int total_entries, int cs, int mixed, int op_kind, int ng5, hipStream_t s);
+                          int total_entries, int cs, int mixed, int op_kind, int ng5, int fast, hipStream_t s);
 int special_chunk(int capacity);
 hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hipStream_t s);
 // batched far-pair launches over items (problem, panel, patch, global panel)

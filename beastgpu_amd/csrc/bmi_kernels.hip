@@ -1578,8 +1578,12 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
 
 // One thread per far pair that the mixed path keeps in FP64 (class 5), owner
 // entries only; thread u takes entry u - ebase[i] of problem i.
-template <int OP>
+// FAST: the default 3 x 4 point rule of a lossless operator at compile time.
+template <int OP, int FAST>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *args, const int *ebase, int nprob) {
+    __shared__ double2 s_sc[64];
+    fill_sincos_table(s_sc);
+    __syncthreads();
     const int u = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
     if (u >= ebase[nprob]) return;
     const int pi = find_problem(ebase, nprob, u);
@@ -1606,7 +1610,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *
             ya[v] = a.Y.vtx[9 * L + v] - yc[v % 3];
         }
         double2 loc[9];
-        pair_fp64<OP, 0, false>(a.op, a.rule, xp, xa, xc, a.X.area[K], yp, ya, yc, a.Y.area[L], loc, nullptr);
+        pair_fp64<OP, FAST, true>(a.op, a.rule, xp, xa, xc, a.X.area[K], yp, ya, yc, a.Y.area[L], loc, s_sc);
 #pragma unroll
         for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
     }
@@ -1652,7 +1656,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *a
 }
 
 hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *ebase, int nprob, int total_chunks,
-                          int total_entries, int cs, int mixed, int op_kind, int ng5, hipStream_t s) {
+                          int total_entries, int cs, int mixed, int op_kind, int ng5, int fast, hipStream_t s) {
     if (total_entries <= 0) return hipSuccess;
     const dim3 b(BMI_BLOCK_THREADS);
     const dim3 g(total_chunks);
@@ -1669,10 +1673,14 @@ hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *
     else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
     else hipLaunchKernelGGL((k_special<OPK_HH3D, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
     const int g2 = (total_entries + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
-    if (mixed) {
-        if (op_kind == OPK_MW) hipLaunchKernelGGL(k_far64<OPK_MW>, dim3(g2), b, 0, s, args, ebase, nprob);
-        else if (op_kind == OPK_DL) hipLaunchKernelGGL(k_far64<OPK_DL>, dim3(g2), b, 0, s, args, ebase, nprob);
-        else hipLaunchKernelGGL(k_far64<OPK_HH3D>, dim3(g2), b, 0, s, args, ebase, nprob);
+    if (mixed && fast) {
+        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_far64<OPK_MW, 1>), dim3(g2), b, 0, s, args, ebase, nprob);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_far64<OPK_DL, 1>), dim3(g2), b, 0, s, args, ebase, nprob);
+        else hipLaunchKernelGGL((k_far64<OPK_HH3D, 1>), dim3(g2), b, 0, s, args, ebase, nprob);
+    } else if (mixed) {
+        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_far64<OPK_MW, 0>), dim3(g2), b, 0, s, args, ebase, nprob);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_far64<OPK_DL, 0>), dim3(g2), b, 0, s, args, ebase, nprob);
+        else hipLaunchKernelGGL((k_far64<OPK_HH3D, 0>), dim3(g2), b, 0, s, args, ebase, nprob);
     }
     hipLaunchKernelGGL(k_link, dim3(g2), b, 0, s, args, ebase, nprob);
     return hipGetLastError();
