@@ -1611,6 +1611,11 @@ struct bmi_batch {
     std::vector<int> singles;
     std::deque<BatchBucket> buckets;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // graph branches: one stream per operator group (default priority), the
+    // fork event and one join event per branch
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int cus = 0;
@@ -1736,22 +1741,34 @@ bmi_status tune_bucket(bmi_batch *Bt, BatchBucket &B) {
     return BMI_OK;
 }
 
-// Classification of every problem (one launch), the special pairs per
-// operator group, zero fill of uncovered rows: on stream s.
-bmi_status enqueue_batch_prologue(bmi_batch *Bt, hipStream_t s) {
+// Classification of every problem (one launch) and zero fill of uncovered
+// rows: on stream s.
+bmi_status enqueue_batch_classify(bmi_batch *Bt, hipStream_t s) {
     const int n = (int)Bt->plans.size();
     BMI_HIP(launch_zero_each(Bt->d_counters.p, n, s));
     BMI_HIP(launch_classify(Bt->d_cargs.p, Bt->d_gbase.p, n, Bt->total_groups, s));
     for (const bmi_plan *P : Bt->plans) P->filled = true;
-    for (auto &G : Bt->opgroups)
-        BMI_HIP(launch_special(G.d_sargs.p, G.d_cbase.p, G.d_ebase.p, (int)G.problems.size(), G.total_chunks,
-                               G.total_entries, G.cs, G.mixed, G.op_kind, G.ng5, G.fast, s));
     for (int i = 0; i < n; ++i) {
         const bmi_plan *P = Bt->plans[i];
         const bmi_matrix *M = Bt->mats[i];
         if (P->X->uncovered || P->Y->uncovered)
             BMI_HIP(launch_fill_zero(M->d, (size_t)M->rows * M->cols * (M->dtype == BMI_C64 ? 16 : 8), s));
     }
+    return BMI_OK;
+}
+
+// The special pairs of one operator group, on stream s.
+hipError_t launch_group_special(const BatchOpGroup &G, hipStream_t s) {
+    return launch_special(G.d_sargs.p, G.d_cbase.p, G.d_ebase.p, (int)G.problems.size(), G.total_chunks,
+                          G.total_entries, G.cs, G.mixed, G.op_kind, G.ng5, G.fast, s);
+}
+
+// Classification, zero fills and the special pairs of every operator group,
+// all on stream s (the batch before the far phase, outside a graph).
+bmi_status enqueue_batch_prologue(bmi_batch *Bt, hipStream_t s) {
+    bmi_status st = enqueue_batch_classify(Bt, s);
+    if (st != BMI_OK) return st;
+    for (auto &G : Bt->opgroups) BMI_HIP(launch_group_special(G, s));
     return BMI_OK;
 }
 
@@ -1807,16 +1824,41 @@ bmi_status build_prologue_tables(bmi_batch *Bt) {
     return BMI_OK;
 }
 
+// The whole batch as one graph: the classification of every problem (and
+// zero fills) on the context stream, then one branch per operator group, its
+// special pairs followed by the far buckets of that operator (the groups'
+// problems are disjoint, so the branches overlap each other's launch tails),
+// joined back before the problems outside the buckets.
 bmi_status capture_batch(bmi_batch *Bt) {
     hipStream_t s = Bt->ctx->stream;
+    const size_t ng = Bt->opgroups.size();
+    while (Bt->branch.size() < ng) {
+        hipStream_t b = nullptr;
+        hipEvent_t e = nullptr;
+        BMI_HIP(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        Bt->branch.push_back(b);
+        BMI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        Bt->join.push_back(e);
+    }
+    if (!Bt->fork) BMI_HIP(hipEventCreateWithFlags(&Bt->fork, hipEventDisableTiming));
     BMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     bmi_status st = BMI_OK;
     auto body = [&]() -> bmi_status {
-        bmi_status r0 = enqueue_batch_prologue(Bt, s);
+        bmi_status r0 = enqueue_batch_classify(Bt, s);
         if (r0 != BMI_OK) return r0;
-        for (auto &B : Bt->buckets) {
-            bmi_status r = enqueue_bucket(B, B.mode, B.wpc, Bt->cus, s);
-            if (r != BMI_OK) return r;
+        BMI_HIP(hipEventRecord(Bt->fork, s));
+        for (size_t gi = 0; gi < ng; ++gi) {
+            const BatchOpGroup &G = Bt->opgroups[gi];
+            hipStream_t b = Bt->branch[gi];
+            BMI_HIP(hipStreamWaitEvent(b, Bt->fork, 0));
+            BMI_HIP(launch_group_special(G, b));
+            for (auto &B : Bt->buckets) {
+                if (B.op_kind != G.op_kind) continue;
+                bmi_status r = enqueue_bucket(B, B.mode, B.wpc, Bt->cus, b);
+                if (r != BMI_OK) return r;
+            }
+            BMI_HIP(hipEventRecord(Bt->join[gi], b));
+            BMI_HIP(hipStreamWaitEvent(s, Bt->join[gi], 0));
         }
         for (int p : Bt->singles) {
             int launches = 0;
@@ -1843,6 +1885,9 @@ void batch_destroy(bmi_batch *Bt) {
     if (Bt->graph) (void)hipGraphDestroy(Bt->graph);
     if (Bt->ev0) (void)hipEventDestroy(Bt->ev0);
     if (Bt->ev1) (void)hipEventDestroy(Bt->ev1);
+    if (Bt->fork) (void)hipEventDestroy(Bt->fork);
+    for (hipEvent_t e : Bt->join) (void)hipEventDestroy(e);
+    for (hipStream_t b : Bt->branch) (void)hipStreamDestroy(b);
     delete Bt;
 }
 
