@@ -1311,26 +1311,34 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
         __syncthreads();
     }
     // one chunk of cs (<= 256) entries: the threads pick out the touching (or
-    // near) entries the problem owns, then each wave integrates every fourth
+    // near) entries the problem owns, listed by decreasing cost (common face:
+    // 6 regions, common edge: 5, common vertex: 2), then each wave integrates
+    // every fourth, so the waves of the work-group finish close together
     {
         const int c0 = ((int)blockIdx.x - cbase[pi]) * cs;
         const int ec = c0 + t;
-        int take = 0;
+        int take = 0, cl = 0;
         if (t < cs && ec < ns) {
-            const int cl = a.spec_cls[ec];
+            cl = a.spec_cls[ec];
             take = !(cl & SPEC_NOT_OWNER) && (NEARK ? cl == CLS_NEAR : (cl >= CLS_CV && cl <= CLS_CF));
         }
-        const unsigned long long m = __ballot(take);
-        __syncthreads();  // s_items, s_wsum of the previous chunk are consumed
-        if (lane == 0) s_wsum[wv] = __popcll(m);
-        __syncthreads();
-        int base = 0, nitem = 0;
+        int nitem = 0;
 #pragma unroll
-        for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
-            base += i < wv ? s_wsum[i] : 0;
-            nitem += s_wsum[i];
+        for (int pass = 0; pass < (NEARK ? 1 : 3); ++pass) {
+            const bool mine = take && (NEARK || cl == CLS_CF - pass);
+            const unsigned long long m = __ballot(mine);
+            __syncthreads();  // s_items, s_wsum of the previous chunk (pass) are consumed
+            if (lane == 0) s_wsum[wv] = __popcll(m);
+            __syncthreads();
+            int base = 0, n = 0;
+#pragma unroll
+            for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
+                base += i < wv ? s_wsum[i] : 0;
+                n += s_wsum[i];
+            }
+            if (mine) s_items[nitem + base + __popcll(m & ((1ull << lane) - 1ull))] = ec;
+            nitem += n;
         }
-        if (take) s_items[base + __popcll(m & ((1ull << lane) - 1ull))] = ec;
         __syncthreads();
     for (int w = wv; w < nitem; w += BMI_BLOCK_THREADS / 64) {
         const int e = s_items[w];
