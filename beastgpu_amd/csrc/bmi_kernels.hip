@@ -1695,10 +1695,11 @@ hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *
 }
 
 // Entries per chunk of k_special: enough chunks to fill the device
-// (>= 2048 work-groups) down to 16 entries per chunk.
+// (>= 8192 work-groups, so the last round of work-groups is a small part)
+// down to 16 entries per chunk.
 int special_chunk(int capacity) {
     int cs = 256;
-    while (cs > 16 && (capacity + cs - 1) / cs < 2048) cs >>= 1;
+    while (cs > 16 && (capacity + cs - 1) / cs < 8192) cs >>= 1;
     return cs;
 }
 
