@@ -703,7 +703,10 @@ __device__ __forceinline__ int find_problem(const int *base, int n, int u) {
 // block); culled blocks record a count of zero.
 // Work-group g takes sub-batch g - gbase[i] of problem i (one problem, or
 // every problem of a batch).
-__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyArgs *args, const int *gbase, int nprob) {
+// split: work-groups per sub-batch, each taking a contiguous share of the
+// trial patches (small problems still fill the device).
+__global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyArgs *args, const int *gbase, int nprob,
+                                                                int split) {
     // candidate trial patches are taken CG at a time: their cells' data is
     // loaded together, so a sub-batch waits for one round of loads per group
     constexpr int CG = 4;
@@ -716,9 +719,12 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
     __shared__ int wsum[BMI_BLOCK_THREADS / 64];
     __shared__ int s_off;
 
-    const int pi = find_problem(gbase, nprob, (int)blockIdx.x);
+    const int grp = (int)blockIdx.x / split, part = (int)blockIdx.x - grp * split;
+    const int pi = find_problem(gbase, nprob, grp);
     const ClassifyArgs a = args[pi];  // a copy: the kernel's stores cannot alias it
-    const int p = (int)blockIdx.x - gbase[pi];
+    const int p = grp - gbase[pi];
+    const int qshare = (a.npb + split - 1) / split;
+    const int qlo = part * qshare, qhi = min(a.npb, qlo + qshare);
     const int t = threadIdx.x;
     const int lane = t & 63, wv = t >> 6;
     if (t < BMI_PATCH) tt[t] = a.X.patch_tri[p * BMI_PATCH + t];
@@ -746,11 +752,11 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
     }
     __syncthreads();
 
-    for (int q0 = 0; q0 < a.npb; q0 += BMI_BLOCK_THREADS) {
+    for (int q0 = qlo; q0 < qhi; q0 += BMI_BLOCK_THREADS) {
         // cull this chunk of trial patches
         const int q = q0 + t;
         int cand = 0;
-        if (q < a.npb) {
+        if (q < qhi) {
             const double dx = px - a.Y.patch_sph[4 * q], dy = py - a.Y.patch_sph[4 * q + 1],
                          dz = pz - a.Y.patch_sph[4 * q + 2];
             const double qr = a.Y.patch_sph[4 * q + 3];
@@ -955,7 +961,10 @@ hipError_t launch_geo_records(const double *pts, const double *vtx, const double
 
 hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, hipStream_t s) {
     if (total_groups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_classify, dim3((unsigned)total_groups), dim3(BMI_BLOCK_THREADS), 0, s, args, gbase, nprob);
+    // up to 8 work-groups per sub-batch, for at least 8192 work-groups
+    const int split = std::max(1, std::min(8, 8192 / total_groups));
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)total_groups * (unsigned)split), dim3(BMI_BLOCK_THREADS), 0, s, args,
+                       gbase, nprob, split);
     return hipGetLastError();
 }
 
