@@ -61,30 +61,6 @@ __device__ __forceinline__ void sincos_turns64(double t, double &s, double &c) {
     c = ((qi + 1) & 2) ? -b : b;
 }
 
-// sin and cos of 2*pi*f for |f| <= 1/2, FP32; Taylor to x^9 / x^10 on |x| <= pi/4.
-__device__ __forceinline__ void sincos_turns32(float f, float &s, float &c) {
-    const float q = rintf(4.0f * f);
-    const float g = fmaf(q, -0.25f, f);
-    const float x = g * 6.28318530717958647692f;
-    const float x2 = x * x;
-    float ps = 1.0f / 362880.0f;
-    ps = fmaf(ps, -x2, 1.0f / 5040.0f);
-    ps = fmaf(ps, -x2, 1.0f / 120.0f);
-    ps = fmaf(ps, -x2, 1.0f / 6.0f);
-    const float sp = fmaf(-x * x2, ps, x);
-    float pc = 1.0f / 3628800.0f;
-    pc = fmaf(pc, -x2, 1.0f / 40320.0f);
-    pc = fmaf(pc, -x2, 1.0f / 720.0f);
-    pc = fmaf(pc, -x2, 1.0f / 24.0f);
-    pc = fmaf(pc, -x2, 0.5f);
-    const float cp = fmaf(-x2, pc, 1.0f);
-    const int qi = ((int)q) & 3;
-    const float a = (qi & 1) ? cp : sp;
-    const float b = (qi & 1) ? sp : cp;
-    s = (qi & 2) ? -a : a;
-    c = ((qi + 1) & 2) ? -b : b;
-}
-
 // sin and cos of 2*pi*u/64 (u in 64ths of a turn), FP64, with a table of the
 // 64 angles 2*pi*m/64: u = m + g exactly (m = rint(u), |g| <= 1/2), the angle
 // of g is x = (pi/32) g, |x| <= pi/64; sine: Taylor polynomial to x^7
