@@ -84,7 +84,11 @@ __device__ __forceinline__ void sincos_64ths_tab(double u, const double2 *tab, d
     double pc = fma(g2, C6, C4);
     pc = fma(g2, pc, C2);
     const double cx = fma(g2, pc, 1.0);
-    const double2 sc = tab[((int)m) & 63];
+    // m + 1.5 * 2^48 is exact (|m| < 2^47) with 1/16 as the unit in the last
+    // place, so its low word is 16 m mod 2^32: masked, the byte offset of
+    // tab[m mod 64]
+    const int off = (int)__builtin_bit_cast(unsigned long long, m + 0x1.8p48) & (63 * (int)sizeof(double2));
+    const double2 sc = *(const double2 *)((const char *)tab + off);
     s = fma(sc.x, cx, sc.y * sx);
     c = fma(sc.y, cx, -(sc.x * sx));
 }
