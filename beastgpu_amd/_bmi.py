@@ -27,10 +27,10 @@ EXPORTS = (
     "bmi_space_create", "bmi_space_free",
     "bmi_plan_create", "bmi_plan_free",
     "bmi_matrix_create", "bmi_matrix_free", "bmi_matrix_device_ptr",
-    "bmi_assemble", "bmi_plan_stats", "bmi_matrix_to_host",
+    "bmi_assemble", "bmi_plan_stats", "bmi_matrix_to_host", "bmi_assemble_to_host",
     "bmi_matrix_apply", "bmi_matrix_compare", "bmi_device_memory",
     "bmi_server_create", "bmi_server_destroy", "bmi_server_submit", "bmi_server_wait",
-    "bmi_server_release", "bmi_server_memory",
+    "bmi_server_release", "bmi_server_memory", "bmi_request_bytes",
     "bmi_batch_create", "bmi_batch_assemble", "bmi_batch_stats_get", "bmi_batch_free",
 )
 BATCH_MAX_BUCKETS = 8
@@ -73,10 +73,12 @@ class Stats(ctypes.Structure):
         ("far_launches", ctypes.c_int32),
         ("n_colours_test", ctypes.c_int32),
         ("n_colours_trial", ctypes.c_int32),
+        ("plan_bytes", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
         return {
+            "plan_bytes": int(self.plan_bytes),
             "classify_ms": self.classify_ms,
             "special_ms": self.special_ms,
             "far_ms": self.far_ms,
@@ -182,6 +184,7 @@ def lib():
     L.bmi_assemble.argtypes = [vp, vp]
     L.bmi_plan_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.bmi_matrix_to_host.argtypes = [vp, vp, i32]
+    L.bmi_assemble_to_host.argtypes = [vp, vp, i32, i64]
     L.bmi_matrix_apply.argtypes = [vp, vp, vp, i32]
     L.bmi_matrix_compare.argtypes = [vp, vp, vp]
     L.bmi_device_memory.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
@@ -191,6 +194,7 @@ def lib():
     L.bmi_server_wait.argtypes = [vp, ctypes.c_uint64, i32, ctypes.POINTER(Result)]
     L.bmi_server_release.argtypes = [vp, ctypes.c_uint64]
     L.bmi_server_memory.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.bmi_request_bytes.argtypes = [ctypes.POINTER(Request), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.bmi_batch_create.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i32, pp]
     L.bmi_batch_assemble.argtypes = [vp]
     L.bmi_batch_stats_get.argtypes = [vp, ctypes.POINTER(BatchStats)]

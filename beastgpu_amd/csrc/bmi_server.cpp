@@ -100,13 +100,93 @@ namespace {
 
 int64_t matrix_bytes(int64_t n, int32_t dtype) { return n * n * (dtype == BMI_C32 ? 8 : 16); }
 
-// Device bytes an assembly holds besides its matrix: geometry records, slot
-// tables, classifier block tables and special-pair values; a bound that grows
-// with the cell count (the block table with the square of it).
-int64_t plan_bytes(int64_t nt) {
-    const int64_t groups = nt / 8 + 64;
-    return (64LL << 20) + nt * 8192 + groups * groups * 8;
+}  // namespace
+
+// Device bytes an assembly's plan holds besides its matrix, bounded from the
+// mesh: geometry and slot records per cell, the classifier's block tables,
+// and 154 bytes per special-pair entry (nine complex values, two cell ids,
+// class, slot) for at most 3 entries per pair (a test cell sits in at most 3
+// panels).  The pairs that can be special: the touching ones (at most the sum
+// over vertices of degree^2), and those whose centroids are within the reach
+// of the near test (dmin < 0.8 sqrt(A) of the trial cell, so within twice the
+// largest cell radius plus 0.8 sqrt of the largest area) or, for the mixed
+// path, within near_factor times the longest edge -- counted on a grid of
+// that spacing.
+int64_t plan_bytes_bound(const double *xyz, int64_t nv, const int32_t *tri, int64_t nt, const bmi_options &o) {
+    std::vector<int64_t> deg(nv, 0);
+    std::vector<double> cen(3 * nt);
+    double rad = 0.0, amax = 0.0, hmax = 0.0;
+    for (int64_t k = 0; k < nt; ++k) {
+        const double *v[3];
+        for (int i = 0; i < 3; ++i) {
+            const int32_t vi = tri[3 * k + i];
+            if (vi < 0 || vi >= nv) return -1;
+            ++deg[vi];
+            v[i] = xyz + 3 * (int64_t)vi;
+        }
+        double e1[3], e2[3];
+        for (int d = 0; d < 3; ++d) {
+            cen[3 * k + d] = (v[0][d] + v[1][d] + v[2][d]) / 3.0;
+            e1[d] = v[0][d] - v[2][d];
+            e2[d] = v[1][d] - v[2][d];
+        }
+        const double n0 = e1[1] * e2[2] - e1[2] * e2[1], n1 = e1[2] * e2[0] - e1[0] * e2[2],
+                     n2 = e1[0] * e2[1] - e1[1] * e2[0];
+        amax = std::max(amax, 0.5 * std::sqrt(n0 * n0 + n1 * n1 + n2 * n2));
+        for (int i = 0; i < 3; ++i) {
+            double r2 = 0.0, h2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const double x = v[i][d] - cen[3 * k + d], y = v[i][d] - v[(i + 1) % 3][d];
+                r2 += x * x;
+                h2 += y * y;
+            }
+            rad = std::max(rad, std::sqrt(r2));
+            hmax = std::max(hmax, std::sqrt(h2));
+        }
+    }
+    double touching = 0.0;
+    for (int64_t d : deg) touching += (double)d * (double)d;
+    double reach = 2.0 * rad + 0.8 * std::sqrt(amax);
+    if (o.precision == BMI_PREC_MIXED) reach = std::max(reach, o.near_factor * hmax);
+    reach = std::max(reach * (1.0 + 1e-9), 1e-300);
+    // pairs with centroids closer than reach, on a grid of spacing reach
+    double lo[3] = {1e300, 1e300, 1e300};
+    for (int64_t k = 0; k < nt; ++k)
+        for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], cen[3 * k + d]);
+    std::vector<std::pair<uint64_t, int>> keys(nt);
+    auto cell_of = [&](int64_t k, int d) { return (int64_t)std::floor((cen[3 * k + d] - lo[d]) / reach); };
+    auto pack = [](int64_t a, int64_t b, int64_t c) {
+        return ((uint64_t)(a & 0x1fffff) << 42) | ((uint64_t)(b & 0x1fffff) << 21) | (uint64_t)(c & 0x1fffff);
+    };
+    for (int64_t k = 0; k < nt; ++k) keys[k] = {pack(cell_of(k, 0), cell_of(k, 1), cell_of(k, 2)), (int)k};
+    std::sort(keys.begin(), keys.end());
+    double close = 0.0;
+    const double r2 = reach * reach;
+    for (int64_t k = 0; k < nt; ++k) {
+        const int64_t c0 = cell_of(k, 0), c1 = cell_of(k, 1), c2 = cell_of(k, 2);
+        for (int64_t a = c0 - 1; a <= c0 + 1; ++a)
+            for (int64_t b = c1 - 1; b <= c1 + 1; ++b)
+                for (int64_t c = c2 - 1; c <= c2 + 1; ++c) {
+                    if (a < 0 || b < 0 || c < 0) continue;
+                    const uint64_t key = pack(a, b, c);
+                    auto it = std::lower_bound(keys.begin(), keys.end(), std::make_pair(key, -1));
+                    for (; it != keys.end() && it->first == key; ++it) {
+                        const int64_t L = it->second;
+                        double d2 = 0.0;
+                        for (int d = 0; d < 3; ++d) {
+                            const double x = cen[3 * k + d] - cen[3 * L + d];
+                            d2 += x * x;
+                        }
+                        close += d2 < r2;
+                    }
+                }
+    }
+    const double pairs = std::min((double)nt * (double)nt, touching + close);
+    const double groups = (double)nt / 8.0 + 64.0;
+    return (int64_t)((64 << 20) + (double)nt * 8192.0 + groups * groups * 8.0 + 3.0 * pairs * 154.0);
 }
+
+namespace {
 
 void finish(bmi_server *S, const std::shared_ptr<Request> &r, int32_t status, const char *msg) {
     r->res.status = status;
@@ -367,7 +447,8 @@ bmi_status bmi_server_submit(bmi_server *S, const bmi_request *q, double *y, uin
         r->opts = q->opts;
         r->dtype = q->dtype;
         r->mbytes = matrix_bytes(q->nfun, q->dtype);
-        r->pbytes = plan_bytes(q->nt);
+        r->pbytes = plan_bytes_bound(q->xyz, q->nv, q->tri, q->nt, q->opts);
+        if (r->pbytes < 0) return set_error(BMI_EINVAL, "vertex index out of range");
         if (r->mbytes + r->pbytes > S->budget)
             return set_error(BMI_ENOMEM, "request needs more device memory than the budget");
     }
@@ -419,6 +500,24 @@ bmi_status bmi_server_release(bmi_server *S, uint64_t id) try {
         S->resident_m.erase(m);
         S->cv_work.notify_all();
     }
+    return BMI_OK;
+} catch (...) {
+    return fail_exception();
+}
+
+bmi_status bmi_request_bytes(const bmi_request *q, int64_t *matrix_bytes_out, int64_t *plan_bytes_out) try {
+    if (!q || !matrix_bytes_out || !plan_bytes_out) return set_error(BMI_EINVAL, "NULL argument");
+    if (q->kind == BMI_REQ_APPLY) {
+        *matrix_bytes_out = *plan_bytes_out = 0;
+        return BMI_OK;
+    }
+    if (!q->xyz || !q->tri || q->nv < 3 || q->nt < 1 || q->nfun < 1)
+        return set_error(BMI_EINVAL, "assembly requests need a mesh and a space");
+    if (q->dtype != BMI_C64 && q->dtype != BMI_C32) return set_error(BMI_EINVAL, "bad dtype");
+    const int64_t pb = plan_bytes_bound(q->xyz, q->nv, q->tri, q->nt, q->opts);
+    if (pb < 0) return set_error(BMI_EINVAL, "vertex index out of range");
+    *matrix_bytes_out = matrix_bytes(q->nfun, q->dtype);
+    *plan_bytes_out = pb;
     return BMI_OK;
 } catch (...) {
     return fail_exception();

@@ -11,7 +11,8 @@ Also reported on the same mesh: the mixed-precision path (FP32 far pairs,
 ComplexF32 and ComplexF64 outputs) with its relative Frobenius and Z.x errors
 against the FP64 path, the roofline of the far-pair kernel, and the CPU oracle
 timed on a bounded sample of the same workload; then the 64-problem batched
-EFIE/MFIE workload and a 400-request sample of the serving stream.  Other
+EFIE/MFIE workload with each problem's mixed-vs-FP64 error, and the
+2,000-request serving stream.  Other
 workloads (--workload): efie-L3, efie-L3-k4, ellipsoid-60k, hh3d-cubeD,
 batched, serving.
 
@@ -30,7 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from beastgpu_amd import Helmholtz3D, Maxwell3D, meshes, spaces  # noqa: E402
+from beastgpu_amd import Helmholtz3D, Maxwell3D, meshes, spaces, workloads  # noqa: E402
 from beastgpu_amd.api import Assembler, Context, GPUConfiguration  # noqa: E402
 
 METRIC = "matrix entries/s, mixed vs FP64 GPU (rel err ≤1e-6); serving p50/p99 latency"
@@ -101,10 +102,13 @@ def time_steps(asm, steps, warmup):
     return dt / steps, far_ms / steps, special_ms / steps, classify_ms / steps, asm.stats()
 
 
-def cpu_baseline(w, seconds=15.0):
-    """The oracle (C, OpenMP over all host cores) on a sample of test cells:
-    the rows of a compact run of test functions.  Full-assembly time is
-    extrapolated by the number of cell pairs the sample integrated."""
+def cpu_baseline(w, seconds=15.0, full_limit=45.0):
+    """The oracle (oracle/oracle.c: the project's plain-C FP64 restatement of
+    the reference algorithm, OpenMP over all host cores; not BEAST itself,
+    which needs Julia) timed on the same workload: the whole matrix when a
+    probe projects it to finish within ``full_limit`` seconds, else the rows
+    of a compact run of test functions (about ``seconds`` of work), with the
+    full-assembly time extrapolated by the cell pairs the sample integrated."""
     from oracle import oracle as O
 
     X, op = w["X"], w["op"]
@@ -117,23 +121,39 @@ def cpu_baseline(w, seconds=15.0):
     dt = time.perf_counter() - t0
     pairs = int(counts.sum())
     rate = pairs / dt
+    full_pairs = nt * nt
+    entries = X.nfun * X.nfun
+    label = "project CPU restatement (oracle/oracle.c, C + OpenMP), not BEAST"
+    if full_pairs / rate <= full_limit:
+        t0 = time.perf_counter()
+        Z, counts = O.assemble(op, X, X, nthreads=cores)
+        t_full = time.perf_counter() - t0
+        del Z
+        return {
+            "value": entries / t_full,
+            "unit": "matrix entries/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"{label}: whole matrix ({X.nfun} x {X.nfun}, {full_pairs} cell pairs) in {t_full:.1f} s",
+            "pairs_per_s": full_pairs / t_full,
+            "full_assembly_s": t_full,
+        }
     want = int(min(max(rate * seconds / (2 * nt), 4), X.nfun))
     rows = np.arange(0, want)
     t0 = time.perf_counter()
     Z, counts = O.assemble(op, X, X, rows=rows, nthreads=cores)
     dt = time.perf_counter() - t0
     pairs = int(counts.sum())
-    full_pairs = nt * nt
     t_full = dt * full_pairs / pairs
-    entries = X.nfun * X.nfun
     return {
         "value": entries / t_full,
         "unit": "matrix entries/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"oracle rows 0..{want - 1} of {X.nfun} ({pairs} cell pairs of {full_pairs}, {dt:.1f} s); "
+        "sample": f"{label}: rows 0..{want - 1} of {X.nfun} ({pairs} cell pairs of {full_pairs}, {dt:.1f} s); "
                   f"full assembly extrapolated to {t_full:.0f} s by cell pairs",
         "pairs_per_s": pairs / dt,
+        "full_assembly_s": t_full,
     }
 
 
@@ -161,34 +181,18 @@ def serving_run(n_req: int, rate: float = 20.0, seed: int = 5, slots: int = 4, p
     import threading
 
     from beastgpu_amd.serving import Server
-    from beastgpu_amd.spaces import Space
 
-    rng = np.random.default_rng(seed)
-    levels = rng.integers(2, 6, size=n_req)
-    ks = rng.uniform(0.5, 6.0, size=n_req)
-    amps = rng.uniform(0.0, 0.05, size=n_req)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
-    base = {L: meshes.icosphere(L) for L in range(2, 6)}
-    bsp = {L: spaces.raviartthomas(base[L]) for L in base}
-    cache = {L: {} for L in base}
-    reqs = []
-    for i in range(n_req):
-        L = int(levels[i])
-        m = meshes.perturb_radial(base[L], seed=10000 + i, amplitude=float(amps[i]), cache=cache[L])
-        X = Space(m, bsp[L].kind, bsp[L].idx, bsp[L].coeff, bsp[L].nfun)
-        reqs.append((Maxwell3D.singlelayer(wavenumber=float(ks[i])), X))
+    reqs = workloads.poisson_stream(n_req, rate=rate, seed=seed)
+    levels = np.array([r.level for r in reqs])
     cfg = GPUConfiguration(precision=precision, output=output)
-
-    def rhs(i, n):
-        r = np.random.default_rng([55, i])
-        return np.sqrt(r.uniform(size=(n, 4))) * np.exp(2j * np.pi * r.uniform(size=(n, 4)))
+    base = {L: spaces.raviartthomas(meshes.icosphere(L)) for L in range(2, 6)}
 
     srv = Server(slots=slots)
     solo = {}
     for rep in range(2):  # warm-up, then one request per level alone (untimed stream)
-        for L in base:
-            res, _ = srv.wait(srv.submit_assemble_apply(Maxwell3D.singlelayer(wavenumber=1.0), bsp[L],
-                                                        rhs(0, bsp[L].nfun), cfg))
+        for L, X in base.items():
+            x0 = workloads.StreamRequest(0.0, L, None, X).rhs(0)
+            res, _ = srv.wait(srv.submit_assemble_apply(Maxwell3D.singlelayer(wavenumber=1.0), X, x0, cfg))
             if rep == 1:
                 solo[f"L{L}"] = {k: res[k] for k in ("total_ms", "prep_ms", "device_ms")}
     ids = [None] * n_req
@@ -196,13 +200,13 @@ def serving_run(n_req: int, rate: float = 20.0, seed: int = 5, slots: int = 4, p
     t0 = time.perf_counter()
 
     def submitter():
-        for i, (op, X) in enumerate(reqs):
-            x = rhs(i, X.nfun)
-            dt = t0 + arrivals[i] - time.perf_counter()
+        for i, r in enumerate(reqs):
+            x = r.rhs(i)
+            dt = t0 + r.arrival_s - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
-            lag[i] = time.perf_counter() - (t0 + arrivals[i])
-            ids[i] = srv.submit_assemble_apply(op, X, x, cfg)
+            lag[i] = time.perf_counter() - (t0 + r.arrival_s)
+            ids[i] = srv.submit_assemble_apply(r.op, r.X, x, cfg)
 
     th = threading.Thread(target=submitter)
     th.start()
@@ -252,30 +256,22 @@ def serving_run(n_req: int, rate: float = 20.0, seed: int = 5, slots: int = 4, p
     }
 
 
-def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4) -> dict:
+def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4, errors: bool = True) -> dict:
     """64 independent EFIE and MFIE (n x K) assemblies, mixed precision,
     ComplexF64 output: perturbed icospheres of level 2..4 (amplitude
     U[0, 5%]), k ~ U[0.5, 8] capped so that the longest edge is at most a
     wavelength / 8.  Times the batch (one replayed graph) against the same
-    plans assembled one after another."""
-    from beastgpu_amd.api import Batch
-    from beastgpu_amd.spaces import Space
+    plans assembled one after another; checks every batched matrix bitwise
+    against its own single assembly and reports each problem's error against
+    an FP64 assembly (relative Frobenius and relative Z x, x uniform in the
+    complex unit disk, seed 33)."""
+    from beastgpu_amd.api import Batch, DeviceMatrix
 
-    rng = np.random.default_rng(seed)
-    base = {L: meshes.icosphere(L) for L in (2, 3, 4)}
-    bsp = {L: spaces.raviartthomas(base[L]) for L in base}
-    cache = {L: {} for L in base}
+    from beastgpu_amd import _bmi as _b
+
+    probs = workloads.batched_efie_mfie(n_prob, seed)
     ctx = Context(0)
-    asms, desc = [], []
-    for i in range(n_prob):
-        L = int(rng.integers(2, 5))
-        amp = float(rng.uniform(0.0, 0.05))
-        m = meshes.perturb_radial(base[L], seed=400 + i, amplitude=amp, cache=cache[L])
-        k = min(float(rng.uniform(0.5, 8.0)), 2 * np.pi / (8 * m.max_edge()))
-        X = Space(m, bsp[L].kind, bsp[L].idx, bsp[L].coeff, bsp[L].nfun)
-        op = Maxwell3D.singlelayer(wavenumber=k) if i % 2 == 0 else Maxwell3D.doublelayer(wavenumber=k)
-        asms.append(Assembler(op, X, X, GPUConfiguration(precision="mixed"), ctx=ctx))
-        desc.append((L, "efie" if i % 2 == 0 else "mfie"))
+    asms = [Assembler(p.op, p.X, p.X, GPUConfiguration(precision="mixed"), ctx=ctx) for p in probs]
     entries = sum(a.matrix.rows * a.matrix.cols for a in asms)
     for _ in range(warmup):
         for a in asms:
@@ -285,11 +281,12 @@ def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4) -> dic
         for a in asms:
             a.assemble()
     t_seq = (time.perf_counter() - t0) / steps
-    # reference copies of one small and one large problem of each operator
-    pick = {}
-    for i, d in enumerate(desc):
-        pick.setdefault(d, i)
-    refs = {i: asms[i].matrix.to_host() for i in pick.values()}
+    # each plan's own assembly, kept for the bitwise check
+    singles = []
+    for a in asms:
+        s = DeviceMatrix(ctx, a.matrix.rows, a.matrix.cols)
+        _b.check(_b.lib().bmi_assemble(a.plan.h, s.h))
+        singles.append(s)
     t0 = time.perf_counter()
     b = Batch(asms)
     t_create = time.perf_counter() - t0
@@ -301,12 +298,25 @@ def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4) -> dic
         b.assemble()
         dev += b.stats()["last_ms"]
     t_b = (time.perf_counter() - t0) / steps
-    same = all(np.array_equal(asms[i].matrix.to_host(), r) for i, r in refs.items())
+    same = all(a.matrix.compare(s)["diff_fro"] == 0.0 for a, s in zip(asms, singles))
+    for s in singles:
+        s.close()
     st = b.stats()
+    per = []
+    if errors:
+        for p, a in zip(probs, asms):
+            f = Assembler(p.op, p.X, p.X, GPUConfiguration(), ctx=ctx)
+            M64 = f.assemble()
+            cmp = a.matrix.compare(M64)
+            x = workloads.unit_disk_vector(p.X.nfun, 33)
+            y64 = M64.apply(x)
+            zx = float(np.linalg.norm(a.matrix.apply(x) - y64) / np.linalg.norm(y64))
+            f.close()
+            per.append([p.level, p.kind, round(p.op.wavenumber, 4), cmp["rel_fro"], zx])
     b.close()
     for a in asms:
         a.close()
-    return {
+    out = {
         "problems": n_prob,
         "entries": int(entries),
         "batched_ms": t_b * 1e3,
@@ -316,10 +326,18 @@ def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4) -> dic
         "entries_per_s": entries / t_b,
         "create_and_tune_s": t_create,
         "bitwise_equal_to_single_assemblies": bool(same),
-        "checked_problems": len(refs),
+        "checked_problems": n_prob,
         "schedule": st,
-        "levels": {f"L{L}": sum(1 for d in desc if d[0] == L) for L in base},
+        "levels": {f"L{L}": sum(1 for p in probs if p.level == L) for L in (2, 3, 4)},
     }
+    if per:
+        out["mixed_vs_fp64"] = {
+            "max_rel_fro": max(r[3] for r in per),
+            "max_rel_zx": max(r[4] for r in per),
+            "per_problem": per,
+            "columns": ["level", "operator", "k", "rel_fro", "rel_zx"],
+        }
+    return out
 
 
 def main():
@@ -329,11 +347,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="efie-L5")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--to-host", action="store_true", help="also time assembly plus the copy into a host matrix")
+    ap.add_argument("--no-to-host", dest="to_host", action="store_false",
+                    help="skip timing plan + assembly + the copy into a host matrix")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--serving-requests", type=int, default=400,
-                    help="requests of the serving stream in the default run (0: skip); --workload serving runs 2,000")
+    ap.add_argument("--serving-requests", type=int, default=2000,
+                    help="requests of the serving stream in the default run (0: skip)")
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--no-batched", dest="batched", action="store_false",
                     help="skip the 64-problem batched sample (--workload batched runs it alone)")
@@ -408,7 +427,7 @@ def main():
         "phases_ms": {"classify": classify_ms, "special": special_ms, "far_blocks": far_ms},
         "pair_counts": pairs,
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "pipe": "VALU FP64 (no MFMA: sqrt/sincos evaluation); peak = MI355X FP64 78.6 TFLOP/s",
             "kernel": FAR_KERNEL[kind],
             "achieved": achieved,
@@ -424,20 +443,32 @@ def main():
         },
     }
 
+    # end to end as a caller sees it: mesh, space, plan and matrix set up,
+    # then one assembly (plan + assembly); then also the copy of the whole
+    # matrix into a host ComplexF64 array, as the reference's store-on-GPU
+    # path returns it (pages of the host array touched beforehand)
+    t0 = time.perf_counter()
+    a2 = Assembler(op, X, X, GPUConfiguration(), ctx=ctx)
+    a2.assemble()
+    t_plan_asm = time.perf_counter() - t0
+    a2.close()
+    e2e = {"plan_and_assembly_ms": t_plan_asm * 1e3}
     if args.to_host:
-        # end to end as the reference's store-on-GPU path: assembly plus the
-        # copy of the whole matrix into a host ComplexF64 array
         host = np.empty((n, n), dtype=np.complex128, order="F")
         from beastgpu_amd import _bmi as _b
         _b.check(_b.lib().bmi_matrix_to_host(asm.matrix.h, host.ctypes.data, _b.C64))  # touch the pages
         t0 = time.perf_counter()
-        asm.assemble()
-        _b.check(_b.lib().bmi_matrix_to_host(asm.matrix.h, host.ctypes.data, _b.C64))
+        a2 = Assembler(op, X, X, GPUConfiguration(), ctx=ctx)
+        a2.assemble()
+        _b.check(_b.lib().bmi_matrix_to_host(a2.matrix.h, host.ctypes.data, _b.C64))
         t_e2e = time.perf_counter() - t0
-        result["end_to_end_with_host_copy_s"] = t_e2e
-        result["host_copy_gbs"] = entries * 16 / (t_e2e - t_step) / 1e9
-        print(f"[bench] assembly + copy to a host ComplexF64 matrix: {t_e2e:.3f} s", file=sys.stderr, flush=True)
+        a2.close()
+        e2e["plan_assembly_and_host_copy_ms"] = t_e2e * 1e3
+        e2e["host_copy_gbs"] = entries * 16 / (t_e2e - t_plan_asm) / 1e9
+        print(f"[bench] plan + assembly + copy to a host ComplexF64 matrix: {t_e2e:.3f} s", file=sys.stderr,
+              flush=True)
         del host
+    result["end_to_end"] = e2e
     if not args.no_mixed:
         rng = np.random.default_rng(33)
         x = np.sqrt(rng.uniform(size=n)) * np.exp(2j * np.pi * rng.uniform(size=n))

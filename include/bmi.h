@@ -105,6 +105,7 @@ typedef struct {
     int32_t far_launches;
     int32_t n_colours_test;
     int32_t n_colours_trial;
+    int64_t plan_bytes;  /* device memory held by the plan (its spaces' and meshes' not included) */
 } bmi_stats;
 
 /* Library version string. */
@@ -161,6 +162,21 @@ bmi_status bmi_plan_stats(const bmi_plan *plan, bmi_stats *stats);
 /* Copy a device matrix to caller-owned host memory as C64 or C32.
  * Replaces copy_to_CPU (utils/copy_to_CPU.jl:5-61). */
 bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype);
+
+/* Assemble the whole matrix straight into caller-owned host memory dst
+ * (column-major rows x cols, dtype BMI_C64 or BMI_C32, pageable or pinned),
+ * for matrices that need not -- or do not -- fit in device memory: row blocks
+ * of whole test panels are assembled into two device staging blocks that
+ * together hold at most device_budget bytes, and each block is copied out
+ * while the next is being assembled (at least 8 blocks when the test space
+ * has 8 panels).  The values are bitwise those of bmi_assemble into a
+ * matrix of element type dtype followed by bmi_matrix_to_host.  Blocking;
+ * BMI_ENOMEM if the budget holds less than two blocks of one test panel.
+ * Replaces the storeCPU path of assemble_gpu (assamble_gpu.jl:8-23 with
+ * allocatestorage :195-226): the memory-budgeted tiles and producer tasks of
+ * schedule_kernel! / producer (GPU_scheduler.jl:134-288) and the host scatter
+ * write_to_compact_matrix (CustomDataStructs/GpuWriteBack.jl:60-132). */
+bmi_status bmi_assemble_to_host(bmi_plan *plan, void *dst, int32_t dtype, int64_t device_budget);
 
 /* y = Z x for nrhs (1..8) right-hand sides on the device-resident matrix: x is
  * cols x nrhs, y is rows x nrhs, both caller-owned host ComplexF64 arrays in
@@ -283,6 +299,11 @@ bmi_status bmi_server_wait(bmi_server *server, uint64_t id, int32_t timeout_ms, 
 bmi_status bmi_server_release(bmi_server *server, uint64_t id);
 /* Device memory held by requests now and at most so far (bytes). */
 bmi_status bmi_server_memory(const bmi_server *server, int64_t *resident, int64_t *peak);
+/* Device memory the server admits an assembly request against: its matrix
+ * and a bound on its plan's buffers (geometry and slot records, block
+ * tables, special-pair entries of touching, near and -- mixed path -- FP64
+ * far pairs), computed from the request's mesh.  Apply requests: 0. */
+bmi_status bmi_request_bytes(const bmi_request *req, int64_t *matrix_bytes, int64_t *plan_bytes);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,53 @@ def test_mixed_within_tolerance(gpu_ctx, kind):
     assert err <= MIXED_TOL
 
 
+@pytest.mark.parametrize("output", ["c32", "c64"])
+def test_mixed_icosphere_l3_k4(gpu_ctx, output):
+    """The unperturbed level-3 icosphere (1,920 RWG) at k = 4 on the mixed
+    path: relative Frobenius error and relative Z x error (x uniform in the
+    complex unit disk, seed 33) against the FP64 oracle."""
+    m = meshes.icosphere(3)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.singlelayer(wavenumber=4.0)
+    print(f"case mixed L3 k=4 {output}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(precision="mixed", output=output), ctx=gpu_ctx)
+    M = a.assemble()
+    Z = M.to_host().astype(np.complex128)
+    rng = np.random.default_rng(33)
+    x = np.sqrt(rng.uniform(size=X.nfun)) * np.exp(2j * np.pi * rng.uniform(size=X.nfun))
+    y = M.apply(x)
+    a.close()
+    Zref, _ = O.assemble(op, X, X)
+    yref = Zref @ x
+    err = relfro(Z, Zref)
+    zx = np.linalg.norm(y - yref) / np.linalg.norm(yref)
+    print("mixed L3 k=4 rel err", err, "zx", zx)
+    assert err <= MIXED_TOL and zx <= MIXED_TOL
+
+
+@pytest.mark.parametrize("d,precision", [(4, "fp64"), (6, "fp64"), (6, "mixed")])
+def test_meshcuboid_hh3d_multi_panel(gpu_ctx, d, precision):
+    """The reference's own workload shape, meshcuboid(1,1,1,0.5/d) with the
+    Helmholtz single layer on P1, at sizes with several 128-row panels (d=4:
+    386 functions, d=6: 866): class counts equal to the oracle's, whole matrix
+    within 1e-12 (FP64) or 1e-6 (mixed)."""
+    m = meshes.meshcuboid(1.0, 1.0, 1.0, 0.5 / d)
+    X = spaces.lagrangec0d1(m)
+    assert X.nfun > 2 * 128
+    op = Helmholtz3D.singlelayer(wavenumber=1.0)
+    print(f"case meshcuboid d={d} {precision} nfun={X.nfun}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(precision=precision), ctx=gpu_ctx)
+    Z = a.assemble().to_host()
+    st = a.stats()
+    a.close()
+    Zref, counts = O.assemble(op, X, X)
+    err = relfro(Z, Zref)
+    print("meshcuboid rel err", err, st)
+    assert [st["pairs"][n] for n in ("far", "common_vertex", "common_edge", "common_face", "near")] == \
+        [int(counts[0]) - (st["pairs"]["far_fp64"] if precision == "mixed" else 0)] + list(counts[1:])
+    assert err <= (MIXED_TOL if precision == "mixed" else FP64_TOL)
+
+
 def test_apply_and_compare_match_host(gpu_ctx):
     op, X = _case("efie", 3, 1.5, 0.02, seed=3)
     a = Assembler(op, X, X, GPUConfiguration(), ctx=gpu_ctx)

@@ -86,3 +86,41 @@ def test_small_budget_serialises_but_completes():
     srv.close()
     for (op, X, cfg, x), (res, y) in zip(reqs, out):
         assert y.shape == (X.nfun, 4) and np.all(np.isfinite(y))
+
+
+@pytest.mark.parametrize("which,precision", [("ellipsoid", "mixed"), ("ellipsoid", "fp64"), ("shells", "mixed"),
+                                             ("icosphere4", "fp64")])
+def test_admission_bound_covers_the_plan(gpu_ctx, which, precision):
+    """The server admits an assembly against bmi_request_bytes: its plan
+    bound is at least what the plan allocates (jittered ellipsoid with many
+    FP64-kept far pairs, two concentric shells with near pairs)."""
+    import ctypes
+
+    from beastgpu_amd import _bmi
+    from beastgpu_amd.api import _c_operator
+
+    if which == "ellipsoid":
+        m = meshes.ellipsoid(cells=(40, 28, 28))
+    elif which == "shells":
+        s = meshes.icosphere(3)
+        m = meshes.Mesh(np.vstack([s.vertices, 1.03 * s.vertices]), np.vstack([s.faces, s.faces + s.nv]))
+    else:
+        m = meshes.icosphere(4)
+    X = spaces.raviartthomas(m)
+    op = Maxwell3D.singlelayer(wavenumber=3.0)
+    cfg = GPUConfiguration(precision=precision)
+    print(f"case admission bound {which} {precision} nt={m.nt}", flush=True)
+    a = Assembler(op, X, X, cfg, ctx=gpu_ctx)
+    st = a.stats()
+    a.close()
+    keep = [np.ascontiguousarray(m.vertices), np.ascontiguousarray(m.faces, dtype=np.int32),
+            np.ascontiguousarray(X.idx), np.ascontiguousarray(X.coeff)]
+    r = _bmi.Request()
+    r.kind = _bmi.REQ_ASSEMBLE_APPLY
+    r.xyz, r.nv, r.tri, r.nt = keep[0].ctypes.data, m.nv, keep[1].ctypes.data, m.nt
+    r.space_kind, r.gidx, r.coeff, r.nfun = _bmi.SPACE_RWG, keep[2].ctypes.data, keep[3].ctypes.data, X.nfun
+    r.op, r.opts, r.dtype = _c_operator(op), cfg.options(), cfg.dtype()
+    mb, pb = ctypes.c_int64(), ctypes.c_int64()
+    _bmi.check(_bmi.lib().bmi_request_bytes(ctypes.byref(r), ctypes.byref(mb), ctypes.byref(pb)))
+    print("plan bytes", st["plan_bytes"], "bound", pb.value, st["pairs"], flush=True)
+    assert 0 < st["plan_bytes"] <= pb.value

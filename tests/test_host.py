@@ -120,7 +120,7 @@ def test_options_default_and_version():
 def test_struct_layouts_match_header():
     assert ctypes.sizeof(_bmi.Operator) == 4 + 4 + 6 * 8  # int32 + padding + 3 complex
     assert ctypes.sizeof(_bmi.Options) == 6 * 4 + 4 + 4 + 8
-    assert ctypes.sizeof(_bmi.Stats) == 4 * 8 + 6 * 8 + 3 * 4 + 4
+    assert ctypes.sizeof(_bmi.Stats) == 4 * 8 + 6 * 8 + 3 * 4 + 4 + 8
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
@@ -179,3 +179,55 @@ def test_server_without_device_fails_loudly():
 
     with pytest.raises(_bmi.BMIError):
         Server(slots=2)
+
+
+def _assembly_request(mesh, X, precision=_bmi.PREC_FP64, near_factor=2.0):
+    keep = [np.ascontiguousarray(mesh.vertices, dtype=np.float64), np.ascontiguousarray(mesh.faces, dtype=np.int32),
+            np.ascontiguousarray(X.idx, dtype=np.int64), np.ascontiguousarray(X.coeff, dtype=np.float64)]
+    r = _bmi.Request()
+    r.kind = _bmi.REQ_ASSEMBLE_APPLY
+    r.xyz, r.nv = keep[0].ctypes.data, mesh.nv
+    r.tri, r.nt = keep[1].ctypes.data, mesh.nt
+    r.space_kind = _bmi.SPACE_RWG
+    r.gidx, r.coeff, r.nfun = keep[2].ctypes.data, keep[3].ctypes.data, X.nfun
+    r.opts = _bmi.default_options()
+    r.opts.precision = precision
+    r.opts.near_factor = near_factor
+    r.dtype = _bmi.C64
+    return r, keep
+
+
+def _special_candidates(mesh, reach_mixed):
+    """Brute force: cell pairs that can be special -- touching (a shared
+    vertex), near (min vertex distance below 0.8 sqrt(largest area)) or, for
+    the mixed path, with centroids closer than reach_mixed."""
+    v = mesh.vertices[mesh.faces]
+    c = v.mean(axis=1)
+    a = mesh.areas()
+    dmin = np.full((mesh.nt, mesh.nt), np.inf)
+    for i in range(3):
+        for j in range(3):
+            d = np.linalg.norm(v[:, None, i, :] - v[None, :, j, :], axis=2)
+            dmin = np.minimum(dmin, d)
+    cd = np.linalg.norm(c[:, None, :] - c[None, :, :], axis=2)
+    cand = (dmin < 1e-12) | (dmin < 0.8 * np.sqrt(a.max())) | (cd < reach_mixed)
+    return int(cand.sum())
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_request_bytes_bound_covers_special_pairs(mixed):
+    """The admission bound covers 3 entries of 154 bytes for every cell pair
+    that can be special, counted by brute force (jittered ellipsoid: thin
+    cells, many FP64-kept far pairs on the mixed path), and the matrix."""
+    m = meshes.ellipsoid(cells=(16, 10, 10))
+    X = spaces.raviartthomas(m)
+    r, keep = _assembly_request(m, X, _bmi.PREC_MIXED if mixed else _bmi.PREC_FP64)
+    mb, pb = ctypes.c_int64(), ctypes.c_int64()
+    _bmi.check(_bmi.lib().bmi_request_bytes(ctypes.byref(r), ctypes.byref(mb), ctypes.byref(pb)))
+    assert mb.value == X.nfun ** 2 * 16
+    n = _special_candidates(m, 2.0 * m.max_edge() if mixed else 0.0)
+    assert pb.value >= 3 * 154 * n
+    assert pb.value <= (64 << 20) + m.nt * 8192 + (m.nt / 8 + 64) ** 2 * 8 + 3 * 154 * m.nt ** 2
+    bad = np.full(3 * m.nt, m.nv + 5, dtype=np.int32)
+    r.tri = bad.ctypes.data
+    assert _bmi.lib().bmi_request_bytes(ctypes.byref(r), ctypes.byref(mb), ctypes.byref(pb)) == _bmi.EINVAL
