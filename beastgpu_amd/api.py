@@ -27,15 +27,21 @@ class GPUConfiguration:
     (utils/configuration.jl:11-26) for the fields that change what is computed
     or where the result lives.
 
-    writeBackStrategy: "gpu" keeps the matrix on the device and copies it to the
-        host at the end (GpuWriteBackTrueInstance); "cpu" is accepted for
-        compatibility and behaves the same (the device write-back needs no
-        host scatter).
+    writeBackStrategy: "gpu" assembles the whole matrix on the device and
+        copies it to the host at the end (GpuWriteBackTrueInstance); "cpu"
+        streams it into host memory (GpuWriteBackFalseInstance): row blocks
+        of test panels are assembled into device staging blocks of at most
+        GPU_budget_pipeline_result bytes together and copied out while the
+        next block is assembled, so the matrix need not fit in device memory.
+        Both give the same values bitwise.
+    GPU_budget_pipeline_result: device bytes of the staging blocks of the "cpu"
+        strategy (the reference's field of that name, 24 GiB by default).
     floatType / precision: "fp64" (plain) or "mixed" (FP32 far pairs).
     output: "c64" (ComplexF64) or "c32" (ComplexF32) matrix.
     """
 
     writeBackStrategy: str = "gpu"
+    GPU_budget_pipeline_result: int = 24 << 30
     precision: str = "fp64"
     output: str = "c64"
     device: int = 0
@@ -184,10 +190,12 @@ class DeviceMatrix(_Handle):
 
 
 class Assembler:
-    """Resident assembly of one (operator, test space, trial space)."""
+    """Resident assembly of one (operator, test space, trial space).  With
+    ``resident=False`` no device matrix is allocated: only assemble_to_host
+    is available (matrices larger than device memory)."""
 
     def __init__(self, op: Operator, X: Space, Y: Space, config: GPUConfiguration | None = None,
-                 ctx: Context | None = None):
+                 ctx: Context | None = None, resident: bool = True):
         self.config = config or GPUConfiguration()
         self.ctx = ctx or Context(self.config.device)
         self.dmx = DeviceMesh(self.ctx, X.mesh)
@@ -198,18 +206,36 @@ class Assembler:
             self.dmy = self.dmx if Y.mesh is X.mesh else DeviceMesh(self.ctx, Y.mesh)
             self.dY = DeviceSpace(self.dmy, Y)
         self.plan = Plan(self.ctx, op, self.dX, self.dY, self.config)
-        self.matrix = DeviceMatrix(self.ctx, X.nfun, Y.nfun, self.config.dtype())
+        self.shape = (X.nfun, Y.nfun)
+        self.matrix = DeviceMatrix(self.ctx, X.nfun, Y.nfun, self.config.dtype()) if resident else None
 
     def assemble(self) -> DeviceMatrix:
+        if self.matrix is None:
+            raise ValueError("this assembler keeps no device matrix: use assemble_to_host")
         _bmi.check(_bmi.lib().bmi_assemble(self.plan.h, self.matrix.h))
         return self.matrix
+
+    def assemble_to_host(self, budget: int | None = None, out: np.ndarray | None = None) -> np.ndarray:
+        """Assemble straight into a host column-major matrix through device
+        staging blocks of at most ``budget`` bytes together (default: the
+        configuration's GPU_budget_pipeline_result)."""
+        budget = int(self.config.GPU_budget_pipeline_result if budget is None else budget)
+        dt = self.config.dtype()
+        want = np.complex128 if dt == _bmi.C64 else np.complex64
+        if out is None:
+            out = np.empty(self.shape, dtype=want, order="F")
+        elif out.shape != self.shape or out.dtype != want or not out.flags.f_contiguous:
+            raise ValueError(f"out must be a Fortran-ordered {self.shape} {np.dtype(want).name} array")
+        _bmi.check(_bmi.lib().bmi_assemble_to_host(self.plan.h, out.ctypes.data, dt, budget))
+        return out
 
     def stats(self) -> dict:
         return self.plan.stats()
 
     def close(self):
         for o in (self.matrix, self.plan, self.dY, self.dX, self.dmy, self.dmx):
-            o.close()
+            if o is not None:
+                o.close()
 
 
 class Batch(_Handle):
@@ -242,14 +268,16 @@ class Batch(_Handle):
 def assemble_gpu(operator: Operator, test_functions: Space, trial_functions: Space,
                  config: GPUConfiguration | None = None, writeBackStrategy: str | None = None) -> np.ndarray:
     """Dense Galerkin matrix of ``operator`` on (test, trial) as a host
-    column-major complex array (ComplexF64 unless config.output == "c32")."""
+    column-major complex array (ComplexF64 unless config.output == "c32");
+    writeBackStrategy "gpu" (default) or "cpu" as in GPUConfiguration."""
     config = dataclasses.replace(config or GPUConfiguration())
     if writeBackStrategy is not None:
         if writeBackStrategy not in ("gpu", "cpu"):
             raise ValueError("writeBackStrategy must be 'gpu' or 'cpu'")
         config.writeBackStrategy = writeBackStrategy
-    a = Assembler(operator, test_functions, trial_functions, config)
+    stream = config.writeBackStrategy == "cpu"
+    a = Assembler(operator, test_functions, trial_functions, config, resident=not stream)
     try:
-        return a.assemble().to_host()
+        return a.assemble_to_host() if stream else a.assemble().to_host()
     finally:
         a.close()

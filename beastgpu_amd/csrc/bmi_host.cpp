@@ -176,6 +176,11 @@ struct DevBuf {
     }
 };
 
+template <class T>
+size_t buf_bytes(const DevBuf<T> &b) {
+    return b.p ? sizeof(T) * (b.n ? b.n : 1) : 0;
+}
+
 constexpr double kPi = 3.14159265358979323846;
 
 // Symmetric triangle rules on the reference simplex (weights sum to 1/2):
@@ -259,6 +264,10 @@ struct bmi_context {
     // pinned staging ring for device-to-host copies of matrices
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // streamed assembly into host memory: copies run on their own stream
+    // (default priority) beside the next block's kernels
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t blk_ev[2] = {nullptr, nullptr};
 };
 
 struct bmi_mesh {
@@ -930,16 +939,20 @@ bmi_status enqueue_prologue(bmi_plan *P, bmi_matrix *out, hipStream_t s, hipEven
     BMI_HIP(launch_special(P->d_sargs.p, P->d_cbase.p, P->d_ebase.p, 1, P->nchunks, P->capacity, P->cs, mixed,
                            P->op_kind, gl5(P), fast_rule(P), s));
     // functions without local shapes keep zero rows/columns (a fill kernel:
-    // memset nodes misbehave in replayed graphs)
-    if (P->X->uncovered || P->Y->uncovered)
+    // memset nodes misbehave in replayed graphs); out == nullptr: the caller
+    // fills its own target
+    if (out && (P->X->uncovered || P->Y->uncovered))
         BMI_HIP(launch_fill_zero(out->d, (size_t)out->rows * out->cols * (out->dtype == BMI_C64 ? 16 : 8), s));
     return BMI_OK;
 }
 
-// Arguments of the far-pair kernels of plan P writing into out.
-PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) {
+// Arguments of the far-pair kernels of plan P writing into the column-major
+// matrix Z with leading dimension ld (global row r of column c at
+// Z[r + c ld]).
+PanelArgs panel_args(const bmi_plan *P, void *Z, long long ld) {
     PanelArgs B{};
     B.npanel = P->X->npanel;
+    B.panel0 = 0;
     B.nrows = (int)P->X->nfun;
     B.sb_off = P->X->d_sb_off.p;
     B.tgeo = P->d_tgeo.p;
@@ -975,25 +988,37 @@ PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) {
     B.spec_loc = P->d_spec_loc.p;
     B.spec_val = P->d_spec_val.p;
     B.capacity = P->capacity;
-    B.Z = out->d;
-    B.ld = out->rows;
+    B.Z = Z;
+    B.ld = ld;
     B.op = P->opc;
     B.rule = P->rule;
     return B;
 }
 
-// The far-pair launches of one assembly, one per trial colour, on stream s.
-bmi_status enqueue_far(bmi_plan *P, bmi_matrix *out, hipStream_t s, int *launches) {
-    PanelArgs B = panel_args(P, out);
+PanelArgs panel_args(const bmi_plan *P, const bmi_matrix *out) { return panel_args(P, out->d, out->rows); }
+
+// The far-pair launches of test panels [p0, p1) of one assembly, one per
+// trial colour, on stream s, into Z (element type dtype, leading dimension
+// ld, global rows).
+bmi_status enqueue_far_range(bmi_plan *P, void *Z, long long ld, int dtype, int p0, int p1, hipStream_t s,
+                             int *launches) {
+    PanelArgs B = panel_args(P, Z, ld);
+    B.panel0 = p0;
+    B.npanel = p1 - p0;
     const int mixed = P->opts.precision == BMI_PREC_MIXED;
-    *launches = 0;
     for (int cb = 0; cb < P->Y->ncolour; ++cb) {
         B.bq_list = P->Y->d_by_colour.p + P->Y->colour_off[cb];
         B.nbq = P->Y->colour_off[cb + 1] - P->Y->colour_off[cb];
-        BMI_HIP(launch_panels(B, P->op_kind, mixed, out->dtype == BMI_C32, s));
+        BMI_HIP(launch_panels(B, P->op_kind, mixed, dtype == BMI_C32, s));
         ++*launches;
     }
     return BMI_OK;
+}
+
+// The far-pair launches of one assembly, one per trial colour, on stream s.
+bmi_status enqueue_far(bmi_plan *P, bmi_matrix *out, hipStream_t s, int *launches) {
+    *launches = 0;
+    return enqueue_far_range(P, out->d, out->rows, out->dtype, 0, P->X->npanel, s, launches);
 }
 
 extern "C" {
@@ -1052,7 +1077,9 @@ bmi_status bmi_context_destroy(bmi_context *ctx) try {
     for (int i = 0; i < 2; ++i) {
         if (ctx->stage[i]) (void)hipHostFree(ctx->stage[i]);
         if (ctx->stage_ev[i]) (void)hipEventDestroy(ctx->stage_ev[i]);
+        if (ctx->blk_ev[i]) (void)hipEventDestroy(ctx->blk_ev[i]);
     }
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     dev_free(ctx->ws);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1452,6 +1479,12 @@ bmi_status bmi_plan_stats(const bmi_plan *Pc, bmi_stats *st) try {
     for (int i = 0; i < 6; ++i) st->pairs[i] = P->pairs[i];
     st->n_colours_test = 1;
     st->n_colours_trial = P->Y->ncolour;
+    st->plan_bytes = (int64_t)(buf_bytes(P->d_xpts) + buf_bytes(P->d_ypts) + buf_bytes(P->d_tgeo) +
+                               buf_bytes(P->d_bgeo) + buf_bytes(P->d_blk_cnt) + buf_bytes(P->d_blk_off) +
+                               buf_bytes(P->d_total) + buf_bytes(P->d_spec_K) + buf_bytes(P->d_spec_L) +
+                               buf_bytes(P->d_spec_cls) + buf_bytes(P->d_spec_loc) + buf_bytes(P->d_spec_val) +
+                               buf_bytes(P->d_cargs) + buf_bytes(P->d_sargs) + buf_bytes(P->d_gbase) +
+                               buf_bytes(P->d_cbase) + buf_bytes(P->d_ebase));
     return BMI_OK;
 } catch (...) {
     return fail_exception();
@@ -1488,6 +1521,151 @@ static void consume_chunk(const void *src, int src32, void *dst, int dst32, size
     for (auto &x : th) x.join();
 }
 
+// Bytes per pinned staging buffer of the device-to-host copies.
+constexpr size_t kStage = 64u << 20;
+
+static bmi_status ensure_staging(bmi_context *ctx) {
+    for (int i = 0; i < 2; ++i) {
+        if (!ctx->stage[i]) BMI_HIP(hipHostMalloc(&ctx->stage[i], kStage, hipHostMallocDefault));
+        if (!ctx->stage_ev[i]) BMI_HIP(hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
+    }
+    return BMI_OK;
+}
+
+// Host copy of elements [o, o + c) of a column-major block of nrb rows (global
+// rows r0 .. r0 + nrb - 1 of every column) into the column-major rows-row host
+// matrix dst; element size es.  Column runs are copied whole, over a few host
+// threads for large pieces.
+static void scatter_rows_piece(const unsigned char *src, size_t es, size_t o, size_t c, int64_t nrb, int64_t r0,
+                               int64_t rows, unsigned char *dst) {
+    auto work = [&](size_t a, size_t b) {
+        size_t e = o + a;
+        const size_t end = o + b;
+        while (e < end) {
+            const size_t col = e / (size_t)nrb, row = e % (size_t)nrb;
+            const size_t run = std::min(end - e, (size_t)nrb - row);
+            std::memcpy(dst + ((size_t)r0 + row + col * (size_t)rows) * es, src + (e - o) * es, run * es);
+            e += run;
+        }
+    };
+    const unsigned nth = c > (1u << 20) ? 4u : 1u;
+    if (nth == 1) {
+        work(0, c);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (c + nth - 1) / nth;
+    for (unsigned t = 0; t < nth; ++t) {
+        const size_t a = t * per, b = std::min(c, a + per);
+        if (a < b) th.emplace_back(work, a, b);
+    }
+    for (auto &x : th) x.join();
+}
+
+bmi_status bmi_assemble_to_host(bmi_plan *P, void *dst, int32_t dtype, int64_t device_budget) try {
+    if (!P || !dst) return fail(BMI_EINVAL, "NULL argument");
+    if (dtype != BMI_C64 && dtype != BMI_C32) return fail(BMI_EINVAL, "bad dtype");
+    if (device_budget <= 0) return fail(BMI_EINVAL, "device budget must be positive");
+    bmi_context *ctx = P->ctx;
+    BMI_HIP(hipSetDevice(ctx->device));
+    WorkStream ws(ctx->stream);
+    hipStream_t s = ctx->stream;
+    const int64_t rows = P->X->nfun, cols = P->Y->nfun;
+    const size_t es = dtype == BMI_C64 ? 16 : 8;
+    const int PR = P->X->prows;
+    const int npanel = P->X->npanel;
+    // panels per block: two staging slots within the budget; at most an
+    // eighth of the panels, so that the copies of a block overlap the
+    // assembly of the next ones
+    const double panel_bytes = (double)PR * (double)cols * (double)es;
+    int64_t ppb = (int64_t)((double)device_budget / (2.0 * panel_bytes));
+    if (ppb < 1)
+        return fail(BMI_ENOMEM, "the device budget holds less than two blocks of one test panel (" +
+                                    std::to_string((int64_t)(2.0 * panel_bytes)) + " bytes)");
+    ppb = std::min<int64_t>(ppb, std::max<int64_t>(1, (npanel + 7) / 8));
+    const int nblk = (int)((npanel + ppb - 1) / ppb);
+    const int nslot = nblk > 1 ? 2 : 1;
+    const size_t slot_elems = (size_t)std::min<int64_t>(rows, ppb * PR) * (size_t)cols;
+    DevBuf<unsigned char> slot[2];
+    for (int i = 0; i < nslot; ++i) {
+        hipError_t e = slot[i].alloc(slot_elems * es);
+        if (e != hipSuccess) return fail(BMI_ENOMEM, std::string("staging block: ") + hipGetErrorString(e));
+    }
+    bmi_status st = ensure_staging(ctx);
+    if (st != BMI_OK) return st;
+    if (!ctx->copy_stream) BMI_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i)
+        if (!ctx->blk_ev[i]) BMI_HIP(hipEventCreateWithFlags(&ctx->blk_ev[i], hipEventDisableTiming));
+    hipStream_t cs = ctx->copy_stream;
+
+    BMI_HIP(hipEventRecord(P->ev[0], s));
+    if ((st = enqueue_prologue(P, nullptr, s, P->ev[1])) != BMI_OK) return st;
+    BMI_HIP(hipEventRecord(P->ev[2], s));
+    int launches = 0;
+    auto block_rows = [&](int b, int64_t &r0, int64_t &nrb) {
+        const int64_t p0 = (int64_t)b * ppb, p1 = std::min<int64_t>(npanel, p0 + ppb);
+        r0 = p0 * PR;
+        nrb = std::min<int64_t>(rows, p1 * PR) - r0;
+    };
+    auto issue_block = [&](int b) -> bmi_status {
+        int64_t r0, nrb;
+        block_rows(b, r0, nrb);
+        unsigned char *base = slot[b % nslot].p;
+        if (P->X->uncovered || P->Y->uncovered) BMI_HIP(launch_fill_zero(base, (size_t)nrb * cols * es, s));
+        // global row r of column c lands at base[(r - r0) + c nrb]
+        const int p0 = (int)((int64_t)b * ppb), p1 = (int)std::min<int64_t>(npanel, (int64_t)(b + 1) * ppb);
+        const bmi_status r = enqueue_far_range(P, base - (size_t)r0 * es, nrb, dtype, p0, p1, s, &launches);
+        if (r != BMI_OK) return r;
+        BMI_HIP(hipEventRecord(ctx->blk_ev[b % nslot], s));
+        if (b == nblk - 1) BMI_HIP(hipEventRecord(P->ev[3], s));
+        return BMI_OK;
+    };
+    if ((st = issue_block(0)) != BMI_OK) return st;
+    const size_t ce = kStage / es;
+    for (int b = 0; b < nblk; ++b) {
+        // the other slot was copied out by the previous iteration: the next
+        // block's kernels may overwrite it while this one is copied
+        if (b + 1 < nblk && (st = issue_block(b + 1)) != BMI_OK) return st;
+        int64_t r0, nrb;
+        block_rows(b, r0, nrb);
+        BMI_HIP(hipStreamWaitEvent(cs, ctx->blk_ev[b % nslot], 0));
+        const unsigned char *base = slot[b % nslot].p;
+        const size_t n = (size_t)nrb * (size_t)cols;
+        const size_t nk = (n + ce - 1) / ce;
+        auto issue = [&](size_t k) -> hipError_t {
+            const size_t o = k * ce, c = std::min(ce, n - o);
+            hipError_t e = hipMemcpyAsync(ctx->stage[k & 1], base + o * es, c * es, hipMemcpyDeviceToHost, cs);
+            if (e != hipSuccess) return e;
+            return hipEventRecord(ctx->stage_ev[k & 1], cs);
+        };
+        // two-deep pipeline: piece k+1 is in flight while piece k is scattered
+        BMI_HIP(issue(0));
+        for (size_t k = 0; k < nk; ++k) {
+            if (k + 1 < nk) BMI_HIP(issue(k + 1));
+            BMI_HIP(hipEventSynchronize(ctx->stage_ev[k & 1]));
+            const size_t o = k * ce, c = std::min(ce, n - o);
+            scatter_rows_piece((const unsigned char *)ctx->stage[k & 1], es, o, c, nrb, r0, rows,
+                               (unsigned char *)dst);
+        }
+    }
+    BMI_HIP(hipStreamSynchronize(s));
+    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+    BMI_HIP(hipEventElapsedTime(&t01, P->ev[0], P->ev[1]));
+    BMI_HIP(hipEventElapsedTime(&t12, P->ev[1], P->ev[2]));
+    BMI_HIP(hipEventElapsedTime(&t23, P->ev[2], P->ev[3]));
+    BMI_HIP(hipEventElapsedTime(&t03, P->ev[0], P->ev[3]));
+    P->last.classify_ms = t01;
+    P->last.special_ms = t12;
+    P->last.far_ms = t23;
+    P->last.total_ms = t03;
+    P->last.far_launches = launches;
+    P->last.n_colours_test = 1;
+    P->last.n_colours_trial = P->Y->ncolour;
+    return BMI_OK;
+} catch (...) {
+    return fail_exception();
+}
+
 bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype) try {
     if (!m || !dst) return fail(BMI_EINVAL, "NULL argument");
     if (dtype != BMI_C64 && dtype != BMI_C32) return fail(BMI_EINVAL, "bad dtype");
@@ -1495,10 +1673,9 @@ bmi_status bmi_matrix_to_host(const bmi_matrix *m, void *dst, int32_t dtype) try
     BMI_HIP(hipSetDevice(ctx->device));
     WorkStream ws(ctx->stream);
     hipStream_t s = ctx->stream;
-    const size_t kStage = 64u << 20;  // bytes per pinned staging buffer
-    for (int i = 0; i < 2; ++i) {
-        if (!ctx->stage[i]) BMI_HIP(hipHostMalloc(&ctx->stage[i], kStage, hipHostMallocDefault));
-        if (!ctx->stage_ev[i]) BMI_HIP(hipEventCreateWithFlags(&ctx->stage_ev[i], hipEventDisableTiming));
+    {
+        const bmi_status st = ensure_staging(ctx);
+        if (st != BMI_OK) return st;
     }
     const int src32 = m->dtype == BMI_C32;
     const size_t es = src32 ? 8 : 16;

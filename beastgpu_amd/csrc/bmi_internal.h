@@ -90,7 +90,8 @@ static_assert((BMI_GEOF_VREL + 9 * BMI_PATCH) * sizeof(float) <= BMI_GEO_CEN * s
 // per-group records have fixed sizes, so a work-group needs no dependent
 // loads to find them.
 struct PanelArgs {
-    int npanel;
+    int npanel;                  // panels of this launch: panel0 .. panel0 + npanel - 1
+    int panel0;
     int nrows;                   // functions of the test space
     const int *sb_off;           // npanel + 1: sub-batches of each panel
     const double *tgeo;          // nsb x BMI_GEO

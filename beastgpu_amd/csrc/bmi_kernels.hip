@@ -1754,7 +1754,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
     const int t = threadIdx.x;
     if (!MIXED) fill_sincos_table(s_sc);
     const int w = blockIdx.x;
-    const int P = w / a.nbq;
+    const int P = a.panel0 + w / a.nbq;
     const int q = a.bq_list[w - P * a.nbq];
     const int r0 = P * BMI_PANEL_ROWS;
     const int nr = min(BMI_PANEL_ROWS, a.nrows - r0);
@@ -2202,14 +2202,14 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
         }
 }
 
-// One work-group per (test panel P, trial patch q of this launch's colour).
+// One work-group per (test panel P of the launch's range, trial patch q of its colour).
 template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_rr(PanelArgs a) {
     extern __shared__ double2 tile[];             // panel_rows(OP) x a.ncm
     __shared__ PanelShared sh;
     if (!MIXED) fill_sincos_table(sh.s_sc);
     const int w = blockIdx.x;
-    const int P = w / a.nbq;
+    const int P = a.panel0 + w / a.nbq;
     const int q = a.bq_list[w - P * a.nbq];
     panel_block<OP, MIXED, FAST, OutT, true>(a, P, q, tile, sh);
 }
