@@ -25,7 +25,7 @@ EXPORTS = (
     "bmi_context_create", "bmi_context_destroy",
     "bmi_mesh_create", "bmi_mesh_free",
     "bmi_space_create", "bmi_space_free",
-    "bmi_plan_create", "bmi_plan_free",
+    "bmi_plan_create", "bmi_plan_create_rules", "bmi_plan_free",
     "bmi_matrix_create", "bmi_matrix_free", "bmi_matrix_device_ptr",
     "bmi_assemble", "bmi_plan_stats", "bmi_matrix_to_host", "bmi_assemble_to_host",
     "bmi_matrix_apply", "bmi_matrix_compare", "bmi_device_memory",
@@ -60,6 +60,20 @@ class Options(ctypes.Structure):
         ("near_order", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("near_factor", ctypes.c_double),
+    ]
+
+
+class Rules(ctypes.Structure):
+    _fields_ = [
+        ("test_points", ctypes.c_int32),
+        ("trial_points", ctypes.c_int32),
+        ("test_uv", (ctypes.c_double * 2) * 4),
+        ("test_weight", ctypes.c_double * 4),
+        ("trial_uv", (ctypes.c_double * 2) * 4),
+        ("trial_weight", ctypes.c_double * 4),
+        ("gl_points", ctypes.c_int32 * 3),
+        ("gl_node", (ctypes.c_double * 8) * 3),
+        ("gl_weight", (ctypes.c_double * 8) * 3),
     ]
 
 
@@ -177,6 +191,8 @@ def lib():
     L.bmi_space_create.argtypes = [vp, i32, vp, vp, i64, pp]
     L.bmi_space_free.argtypes = [vp]
     L.bmi_plan_create.argtypes = [vp, ctypes.POINTER(Operator), vp, vp, ctypes.POINTER(Options), pp]
+    L.bmi_plan_create_rules.argtypes = [vp, ctypes.POINTER(Operator), vp, vp, ctypes.POINTER(Options),
+                                        ctypes.POINTER(Rules), pp]
     L.bmi_plan_free.argtypes = [vp]
     L.bmi_matrix_create.argtypes = [vp, i64, i64, i32, pp]
     L.bmi_matrix_free.argtypes = [vp]

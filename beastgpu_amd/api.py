@@ -22,6 +22,66 @@ from .spaces import P1, RWG, Space
 
 
 @dataclasses.dataclass
+class QuadratureRules:
+    """Quadrature tables used in place of the rule orders of a configuration
+    (bmi_rules): the far-pair triangle rules of test and trial cells -- chart
+    coordinates (u, v) of the point v3 + u (v1 - v3) + v (v2 - v3) and
+    weights on the reference triangle (area 1/2), at most 4 points each --
+    and the Gauss-Legendre rules on [0, 1] of the Sauter-Schwab hypercube for
+    common vertex, edge and face (at most 8 points each).  BEAST's quaddata
+    holds the far rules per cell, as Cartesian points and weights times the
+    Jacobian 2A (GPU_scheduler.jl:61-90); ``reference_rule`` recovers the
+    reference rule from one cell's points."""
+
+    test_uv: np.ndarray
+    test_weights: np.ndarray
+    trial_uv: np.ndarray
+    trial_weights: np.ndarray
+    gauss_legendre: tuple  # ((nodes, weights) for CV, CE, CF)
+
+    def __post_init__(self):
+        self.test_uv = np.asarray(self.test_uv, dtype=np.float64).reshape(-1, 2)
+        self.trial_uv = np.asarray(self.trial_uv, dtype=np.float64).reshape(-1, 2)
+        self.test_weights = np.asarray(self.test_weights, dtype=np.float64).ravel()
+        self.trial_weights = np.asarray(self.trial_weights, dtype=np.float64).ravel()
+        if self.test_uv.shape[0] != self.test_weights.size or self.trial_uv.shape[0] != self.trial_weights.size:
+            raise ValueError("one weight per rule point")
+        if len(self.gauss_legendre) != 3:
+            raise ValueError("three Gauss-Legendre rules: common vertex, edge, face")
+        self.gauss_legendre = tuple((np.asarray(x, dtype=np.float64).ravel(), np.asarray(w, dtype=np.float64).ravel())
+                                    for x, w in self.gauss_legendre)
+
+    @staticmethod
+    def reference_rule(vertices, points, weights):
+        """(uv, reference weights) of a rule given on one cell (vertices 3 x 3,
+        rows v1 v2 v3) by Cartesian points and weights that include the
+        Jacobian 2A, as BEAST's quaddata stores it."""
+        v = np.asarray(vertices, dtype=np.float64)
+        t = np.stack([v[0] - v[2], v[1] - v[2]], axis=1)  # 3 x 2
+        uv, *_ = np.linalg.lstsq(t, (np.asarray(points, dtype=np.float64) - v[2]).T, rcond=None)
+        two_area = np.linalg.norm(np.cross(t[:, 0], t[:, 1]))
+        return uv.T, np.asarray(weights, dtype=np.float64) / two_area
+
+    def c(self) -> _bmi.Rules:
+        r = _bmi.Rules()
+        r.test_points, r.trial_points = self.test_uv.shape[0], self.trial_uv.shape[0]
+        if not (1 <= r.test_points <= 4 and 1 <= r.trial_points <= 4):
+            raise ValueError("1..4 far-rule points per cell")
+        for i in range(r.test_points):
+            r.test_uv[i][0], r.test_uv[i][1], r.test_weight[i] = self.test_uv[i, 0], self.test_uv[i, 1], self.test_weights[i]
+        for i in range(r.trial_points):
+            r.trial_uv[i][0], r.trial_uv[i][1], r.trial_weight[i] = (self.trial_uv[i, 0], self.trial_uv[i, 1],
+                                                                     self.trial_weights[i])
+        for g, (x, w) in enumerate(self.gauss_legendre):
+            if not 1 <= x.size <= 8 or x.size != w.size:
+                raise ValueError("1..8 Gauss-Legendre points, one weight each")
+            r.gl_points[g] = x.size
+            for i in range(x.size):
+                r.gl_node[g][i], r.gl_weight[g][i] = x[i], w[i]
+        return r
+
+
+@dataclasses.dataclass
 class GPUConfiguration:
     """Options of one assembly; mirrors the reference's GPUConfiguration
     (utils/configuration.jl:11-26) for the fields that change what is computed
@@ -50,6 +110,7 @@ class GPUConfiguration:
     far_trial_order: int = 3
     near_order: int = 4
     near_factor: float = 2.0
+    rules: QuadratureRules | None = None  # replaces far_test_order, far_trial_order and ss_order
 
     def options(self) -> _bmi.Options:
         o = _bmi.default_options()
@@ -142,8 +203,13 @@ class Plan(_Handle):
         self.ctx, self.dX, self.dY = ctx, dX, dY
         o = config.options()
         cop = _c_operator(op)
-        _bmi.check(_bmi.lib().bmi_plan_create(ctx.h, ctypes.byref(cop), dX.h, dY.h, ctypes.byref(o),
-                                              ctypes.byref(self.h)))
+        if config.rules is None:
+            _bmi.check(_bmi.lib().bmi_plan_create(ctx.h, ctypes.byref(cop), dX.h, dY.h, ctypes.byref(o),
+                                                  ctypes.byref(self.h)))
+        else:
+            r = config.rules.c()
+            _bmi.check(_bmi.lib().bmi_plan_create_rules(ctx.h, ctypes.byref(cop), dX.h, dY.h, ctypes.byref(o),
+                                                        ctypes.byref(r), ctypes.byref(self.h)))
 
     def stats(self) -> dict:
         s = _bmi.Stats()
@@ -197,17 +263,24 @@ class Assembler:
     def __init__(self, op: Operator, X: Space, Y: Space, config: GPUConfiguration | None = None,
                  ctx: Context | None = None, resident: bool = True):
         self.config = config or GPUConfiguration()
+        self.matrix = self.plan = self.dY = self.dX = self.dmy = self.dmx = None
         self.ctx = ctx or Context(self.config.device)
-        self.dmx = DeviceMesh(self.ctx, X.mesh)
-        self.dX = DeviceSpace(self.dmx, X)
-        if Y is X:
-            self.dmy, self.dY = self.dmx, self.dX
-        else:
-            self.dmy = self.dmx if Y.mesh is X.mesh else DeviceMesh(self.ctx, Y.mesh)
-            self.dY = DeviceSpace(self.dmy, Y)
-        self.plan = Plan(self.ctx, op, self.dX, self.dY, self.config)
-        self.shape = (X.nfun, Y.nfun)
-        self.matrix = DeviceMatrix(self.ctx, X.nfun, Y.nfun, self.config.dtype()) if resident else None
+        try:
+            self.dmx = DeviceMesh(self.ctx, X.mesh)
+            self.dX = DeviceSpace(self.dmx, X)
+            if Y is X:
+                self.dmy, self.dY = self.dmx, self.dX
+            else:
+                self.dmy = self.dmx if Y.mesh is X.mesh else DeviceMesh(self.ctx, Y.mesh)
+                self.dY = DeviceSpace(self.dmy, Y)
+            self.plan = Plan(self.ctx, op, self.dX, self.dY, self.config)
+            self.shape = (X.nfun, Y.nfun)
+            self.matrix = DeviceMatrix(self.ctx, X.nfun, Y.nfun, self.config.dtype()) if resident else None
+        except BaseException:
+            # release what was built, children first (the exception's
+            # traceback would otherwise keep it alive)
+            self.close()
+            raise
 
     def assemble(self) -> DeviceMatrix:
         if self.matrix is None:

@@ -272,6 +272,7 @@ struct bmi_context {
 
 struct bmi_mesh {
     bmi_context *ctx = nullptr;
+    int device = 0;  // kept here so that freeing needs no other object
     int64_t nv = 0, nt = 0;
     std::vector<double> vtx, cen, area;  // per cell
     double hmax = 0.0;
@@ -280,6 +281,7 @@ struct bmi_mesh {
 
 struct bmi_space {
     bmi_mesh *mesh = nullptr;
+    int device = 0;
     int kind = 0;
     int64_t nfun = 0;
     int npatch = 0;
@@ -316,6 +318,7 @@ struct bmi_space {
 
 struct bmi_matrix {
     bmi_context *ctx = nullptr;
+    int device = 0;
     int64_t rows = 0, cols = 0;
     int dtype = BMI_C64;
     void *d = nullptr;
@@ -324,6 +327,7 @@ struct bmi_matrix {
 
 struct bmi_plan {
     bmi_context *ctx = nullptr;
+    int device = 0;
     const bmi_space *X = nullptr, *Y = nullptr;
     bmi_operator op{};
     bmi_options opts{};
@@ -348,6 +352,7 @@ struct bmi_plan {
     int cs = 256, nchunks = 0;
     int64_t pairs[6] = {0, 0, 0, 0, 0, 0};
     bool pairs_ready = false;
+    bool rule_default = true;  // the far rule is the default 3 x 4 point rule
     mutable bool filled = false;  // the special-pair entries have been written
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bmi_stats last{};
@@ -897,6 +902,7 @@ bmi_status matrix_wrap(bmi_context *ctx, void *mem, int64_t rows, int64_t cols, 
     if (!ctx || !mem || !out) return fail(BMI_EINVAL, "NULL argument");
     auto m = std::make_unique<bmi_matrix>();
     m->ctx = ctx;
+    m->device = ctx->device;
     m->rows = rows;
     m->cols = cols;
     m->dtype = dtype;
@@ -918,13 +924,113 @@ bmi_status check_target(const bmi_plan *P, const bmi_matrix *out) {
 
 // 1 if the far kernels' compile-time rule applies: default 3 x 4 point
 // rule, lossless operator.
-int fast_rule(const bmi_plan *P) {
-    return P->opts.far_test_order == 2 && P->opts.far_trial_order == 3 && !P->opc.lossy;
+int fast_rule(const bmi_plan *P) { return P->rule_default && !P->opc.lossy; }
+
+// 1 if the plan's Sauter-Schwab rules all have 5 points per dimension and the
+// same nodes and weights (the compile-time-order kernel takes one table).
+int gl5(const bmi_plan *P) {
+    const SpecialArgs &S = P->sargs;
+    for (int g = 0; g < 3; ++g) {
+        if (S.ss_n[g] != 5) return 0;
+        for (int i = 0; i < 5; ++i)
+            if (S.glx[g][i] != S.glx[0][i] || S.glw[g][i] != S.glw[0][i]) return 0;
+    }
+    return 1;
 }
 
-// 1 if the plan's Sauter-Schwab rules all have 5 points per dimension.
-int gl5(const bmi_plan *P) {
-    return P->opts.ss_order[0] == 5 && P->opts.ss_order[1] == 5 && P->opts.ss_order[2] == 5;
+// The default far rules as tables: 3 points on test cells, 4 on trial cells
+// (in the order of the kernels' compile-time rule).
+bool rules_are_default(const bmi_rules &r) {
+    int n = 0, m = 0;
+    double l[BMI_MAX_RULE][3], w[BMI_MAX_RULE], lb[BMI_MAX_RULE][3], wb[BMI_MAX_RULE];
+    far_rule(2, n, l, w);
+    far_rule(3, m, lb, wb);
+    if (r.test_points != n || r.trial_points != m) return false;
+    for (int i = 0; i < n; ++i)
+        if (r.test_uv[i][0] != l[i][0] || r.test_uv[i][1] != l[i][1] || r.test_weight[i] != w[i]) return false;
+    for (int i = 0; i < m; ++i)
+        if (r.trial_uv[i][0] != lb[i][0] || r.trial_uv[i][1] != lb[i][1] || r.trial_weight[i] != wb[i]) return false;
+    return true;
+}
+
+// true if a vertex of mesh a lies inside a cell of mesh b (within a relative
+// tolerance, on its plane and in its closure) without being one of that
+// cell's vertices: the meshes overlap, as when one refines the other.
+// Candidate cells come from a uniform grid over the cells' bounding boxes.
+bool vertex_inside_other(const bmi_mesh *a, const bmi_mesh *b) {
+    const int64_t nt = b->nt;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t k = 0; k < nt; ++k)
+        for (int i = 0; i < 3; ++i)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], b->vtx[9 * k + 3 * i + d]);
+                hi[d] = std::max(hi[d], b->vtx[9 * k + 3 * i + d]);
+            }
+    const double h = std::max(b->hmax, 1e-300);
+    const double tol = 1e-9 * h;
+    int64_t dims[3];
+    for (int d = 0; d < 3; ++d) dims[d] = std::max<int64_t>(1, std::min<int64_t>(1 << 10, (int64_t)((hi[d] - lo[d]) / h) + 1));
+    auto bin = [&](double x, int d) {
+        return std::max<int64_t>(0, std::min<int64_t>(dims[d] - 1, (int64_t)((x - lo[d]) / h)));
+    };
+    std::vector<std::pair<int64_t, int64_t>> ent;  // (bin, cell)
+    for (int64_t k = 0; k < nt; ++k) {
+        int64_t b0[3], b1[3];
+        for (int d = 0; d < 3; ++d) {
+            double mn = 1e300, mx = -1e300;
+            for (int i = 0; i < 3; ++i) {
+                mn = std::min(mn, b->vtx[9 * k + 3 * i + d]);
+                mx = std::max(mx, b->vtx[9 * k + 3 * i + d]);
+            }
+            b0[d] = bin(mn - tol, d);
+            b1[d] = bin(mx + tol, d);
+        }
+        for (int64_t x = b0[0]; x <= b1[0]; ++x)
+            for (int64_t y = b0[1]; y <= b1[1]; ++y)
+                for (int64_t z = b0[2]; z <= b1[2]; ++z) ent.push_back({(x * dims[1] + y) * dims[2] + z, k});
+    }
+    std::sort(ent.begin(), ent.end());
+    for (int64_t K = 0; K < a->nt; ++K)
+        for (int i = 0; i < 3; ++i) {
+            const double *p = &a->vtx[9 * K + 3 * i];
+            if (p[0] < lo[0] - tol || p[0] > hi[0] + tol || p[1] < lo[1] - tol || p[1] > hi[1] + tol ||
+                p[2] < lo[2] - tol || p[2] > hi[2] + tol)
+                continue;
+            const int64_t key = (bin(p[0], 0) * dims[1] + bin(p[1], 1)) * dims[2] + bin(p[2], 2);
+            auto it = std::lower_bound(ent.begin(), ent.end(), std::make_pair(key, (int64_t)-1));
+            for (; it != ent.end() && it->first == key; ++it) {
+                const double *v = &b->vtx[9 * it->second];
+                bool at_vertex = false;
+                for (int j = 0; j < 3; ++j) {
+                    const double dx = p[0] - v[3 * j], dy = p[1] - v[3 * j + 1], dz = p[2] - v[3 * j + 2];
+                    at_vertex |= dx * dx + dy * dy + dz * dz <= tol * tol;
+                }
+                if (at_vertex) continue;
+                // barycentric coordinates of p's projection, distance to the plane
+                double e1[3], e2[3], q[3];
+                for (int d = 0; d < 3; ++d) {
+                    e1[d] = v[d] - v[6 + d];
+                    e2[d] = v[3 + d] - v[6 + d];
+                    q[d] = p[d] - v[6 + d];
+                }
+                const double g11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+                const double g12 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+                const double g22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+                const double r1 = q[0] * e1[0] + q[1] * e1[1] + q[2] * e1[2];
+                const double r2 = q[0] * e2[0] + q[1] * e2[1] + q[2] * e2[2];
+                const double det = g11 * g22 - g12 * g12;
+                if (!(det > 0.0)) continue;
+                const double u = (g22 * r1 - g12 * r2) / det, w = (g11 * r2 - g12 * r1) / det;
+                double dist2 = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    const double o = q[d] - u * e1[d] - w * e2[d];
+                    dist2 += o * o;
+                }
+                const double bt = 1e-9;
+                if (dist2 <= tol * tol && u >= -bt && w >= -bt && u + w <= 1.0 + bt) return true;
+            }
+        }
+    return false;
 }
 
 // Classification, compaction and special pairs of one assembly on stream s,
@@ -1096,6 +1202,7 @@ bmi_status bmi_mesh_create(bmi_context *ctx, const double *xyz, int64_t nv, cons
     WorkStream ws(ctx->stream);
     auto m = std::make_unique<bmi_mesh>();
     m->ctx = ctx;
+    m->device = ctx->device;
     m->nv = nv;
     m->nt = nt;
     m->vtx.resize(9 * nt);
@@ -1140,7 +1247,7 @@ bmi_status bmi_mesh_create(bmi_context *ctx, const double *xyz, int64_t nv, cons
 
 bmi_status bmi_mesh_free(bmi_mesh *m) try {
     if (m) {
-        (void)hipSetDevice(m->ctx->device);
+        (void)hipSetDevice(m->device);
         delete m;
     }
     return BMI_OK;
@@ -1161,6 +1268,7 @@ bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, c
     WorkStream ws(mesh->ctx->stream);
     auto s = std::make_unique<bmi_space>();
     s->mesh = mesh;
+    s->device = mesh->device;
     s->kind = kind;
     s->nfun = nfun;
     if (kind == BMI_SPACE_LAGRANGE_C0D1) {
@@ -1180,7 +1288,7 @@ bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, c
 
 bmi_status bmi_space_free(bmi_space *s) try {
     if (s) {
-        (void)hipSetDevice(s->mesh->ctx->device);
+        (void)hipSetDevice(s->device);
         delete s;
     }
     return BMI_OK;
@@ -1198,6 +1306,7 @@ bmi_status bmi_matrix_create(bmi_context *ctx, int64_t rows, int64_t cols, int32
     const size_t bytes = (size_t)rows * (size_t)cols * es;
     auto m = std::make_unique<bmi_matrix>();
     m->ctx = ctx;
+    m->device = ctx->device;
     m->rows = rows;
     m->cols = cols;
     m->dtype = dtype;
@@ -1211,7 +1320,7 @@ bmi_status bmi_matrix_create(bmi_context *ctx, int64_t rows, int64_t cols, int32
 
 bmi_status bmi_matrix_free(bmi_matrix *m) try {
     if (m) {
-        (void)hipSetDevice(m->ctx->device);
+        (void)hipSetDevice(m->device);
         if (m->owned) dev_free(m->d);
         delete m;
     }
@@ -1229,8 +1338,29 @@ bmi_status bmi_matrix_device_ptr(const bmi_matrix *m, void **ptr, int64_t *ld) t
     return fail_exception();
 }
 
+static bmi_status plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_space *test,
+                              const bmi_space *trial, const bmi_options *opts, const bmi_rules *rules,
+                              bmi_plan **out);
+
 bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_space *test, const bmi_space *trial,
                            const bmi_options *opts, bmi_plan **out) try {
+    return plan_create(ctx, op, test, trial, opts, nullptr, out);
+} catch (...) {
+    return fail_exception();
+}
+
+bmi_status bmi_plan_create_rules(bmi_context *ctx, const bmi_operator *op, const bmi_space *test,
+                                 const bmi_space *trial, const bmi_options *opts, const bmi_rules *rules,
+                                 bmi_plan **out) try {
+    if (!rules) return fail(BMI_EINVAL, "NULL rules");
+    return plan_create(ctx, op, test, trial, opts, rules, out);
+} catch (...) {
+    return fail_exception();
+}
+
+static bmi_status plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_space *test,
+                              const bmi_space *trial, const bmi_options *opts, const bmi_rules *rules,
+                              bmi_plan **out) {
     if (!ctx || !op || !test || !trial || !out) return fail(BMI_EINVAL, "NULL argument");
     if (test->mesh->ctx != ctx || trial->mesh->ctx != ctx) return fail(BMI_EINVAL, "spaces belong to another context");
     bmi_options o;
@@ -1245,8 +1375,29 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     } else {
         return fail(BMI_EUNSUPPORTED, "unknown operator kind");
     }
-    for (int i = 0; i < 3; ++i)
-        if (o.ss_order[i] < 1 || o.ss_order[i] > BMI_MAX_GL) return fail(BMI_EINVAL, "ss_order must be 1..8");
+    if (rules) {
+        if (rules->test_points < 1 || rules->test_points > BMI_MAX_RULE || rules->trial_points < 1 ||
+            rules->trial_points > BMI_MAX_RULE)
+            return fail(BMI_EINVAL, "rule points must be 1..4 per cell");
+        for (int i = 0; i < BMI_MAX_RULE; ++i) {
+            const double v[6] = {rules->test_uv[i][0], rules->test_uv[i][1], rules->test_weight[i],
+                                 rules->trial_uv[i][0], rules->trial_uv[i][1], rules->trial_weight[i]};
+            for (int j = 0; j < 6; ++j)
+                if ((j < 3 ? i < rules->test_points : i < rules->trial_points) && !std::isfinite(v[j]))
+                    return fail(BMI_EINVAL, "non-finite rule point or weight");
+        }
+        for (int g = 0; g < 3; ++g) {
+            if (rules->gl_points[g] < 1 || rules->gl_points[g] > BMI_MAX_GL)
+                return fail(BMI_EINVAL, "Gauss-Legendre points must be 1..8");
+            for (int i = 0; i < rules->gl_points[g]; ++i)
+                if (!(rules->gl_node[g][i] >= 0.0 && rules->gl_node[g][i] <= 1.0) ||
+                    !std::isfinite(rules->gl_weight[g][i]))
+                    return fail(BMI_EINVAL, "Gauss-Legendre nodes must lie in [0, 1] with finite weights");
+        }
+    } else {
+        for (int i = 0; i < 3; ++i)
+            if (o.ss_order[i] < 1 || o.ss_order[i] > BMI_MAX_GL) return fail(BMI_EINVAL, "ss_order must be 1..8");
+    }
     if (o.near_order < 1 || o.near_order > 8) return fail(BMI_EINVAL, "near_order must be 1..8");
     if (o.precision != BMI_PREC_FP64 && o.precision != BMI_PREC_MIXED) return fail(BMI_EINVAL, "bad precision");
     if (!(o.near_factor >= 0.0)) return fail(BMI_EINVAL, "bad near_factor");
@@ -1254,10 +1405,15 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     for (double v : vals)
         if (!std::isfinite(v)) return fail(BMI_EINVAL, "non-finite operator coefficient");
 
+    if (test->mesh != trial->mesh &&
+        (vertex_inside_other(test->mesh, trial->mesh) || vertex_inside_other(trial->mesh, test->mesh)))
+        return fail(BMI_EUNSUPPORTED, "the test and trial meshes overlap (a vertex of one lies inside a cell of the "
+                                      "other): refinement-aware quadrature is not supported");
     BMI_HIP(hipSetDevice(ctx->device));
     WorkStream ws(ctx->stream);
     auto P = std::make_unique<bmi_plan>();
     P->ctx = ctx;
+    P->device = ctx->device;
     P->X = test;
     P->Y = trial;
     P->op = *op;
@@ -1273,8 +1429,27 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
 
     double lt[BMI_MAX_RULE][3] = {}, lb[BMI_MAX_RULE][3] = {}, wt[BMI_MAX_RULE] = {}, wb[BMI_MAX_RULE] = {};
     int nt = 0, nb = 0;
-    if (!far_rule(o.far_test_order, nt, lt, wt) || !far_rule(o.far_trial_order, nb, lb, wb))
-        return fail(BMI_EINVAL, "far rule orders must be 1, 2 or 3");
+    if (rules) {
+        nt = rules->test_points;
+        nb = rules->trial_points;
+        for (int i = 0; i < nt; ++i) {
+            lt[i][0] = rules->test_uv[i][0];
+            lt[i][1] = rules->test_uv[i][1];
+            lt[i][2] = 1.0 - lt[i][0] - lt[i][1];
+            wt[i] = rules->test_weight[i];
+        }
+        for (int i = 0; i < nb; ++i) {
+            lb[i][0] = rules->trial_uv[i][0];
+            lb[i][1] = rules->trial_uv[i][1];
+            lb[i][2] = 1.0 - lb[i][0] - lb[i][1];
+            wb[i] = rules->trial_weight[i];
+        }
+        P->rule_default = rules_are_default(*rules);
+    } else {
+        if (!far_rule(o.far_test_order, nt, lt, wt) || !far_rule(o.far_trial_order, nb, lb, wb))
+            return fail(BMI_EINVAL, "far rule orders must be 1, 2 or 3");
+        P->rule_default = o.far_test_order == 2 && o.far_trial_order == 3;
+    }
     RuleConst &R = P->rule;
     R.nt = nt;
     R.nb = nb;
@@ -1374,8 +1549,17 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     S.op = P->opc;
     S.rule = P->rule;
     for (int g = 0; g < 3; ++g) {
-        S.ss_n[g] = o.ss_order[g];
-        gauss_legendre01(o.ss_order[g], S.glx[g], S.glw[g]);
+        if (rules) {
+            S.ss_n[g] = rules->gl_points[g];
+            for (int i = 0; i < S.ss_n[g]; ++i) {
+                S.glx[g][i] = rules->gl_node[g][i];
+                S.glw[g][i] = rules->gl_weight[g][i];
+            }
+        } else {
+            S.ss_n[g] = o.ss_order[g];
+            gauss_legendre01(o.ss_order[g], S.glx[g], S.glw[g]);
+        }
+        P->opts.ss_order[g] = S.ss_n[g];
     }
     {
         double x[8], w[8];
@@ -1403,13 +1587,11 @@ bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_s
     for (int i = 0; i < 4; ++i) BMI_HIP(hipEventCreate(&P->ev[i]));
     *out = P.release();
     return BMI_OK;
-} catch (...) {
-    return fail_exception();
 }
 
 bmi_status bmi_plan_free(bmi_plan *P) try {
     if (P) {
-        (void)hipSetDevice(P->ctx->device);
+        (void)hipSetDevice(P->device);
         for (auto &e : P->ev)
             if (e) (void)hipEventDestroy(e);
         delete P;
@@ -1778,6 +1960,7 @@ struct BatchOpGroup {
 
 struct bmi_batch {
     bmi_context *ctx = nullptr;
+    int device = 0;
     std::vector<bmi_plan *> plans;
     std::vector<bmi_matrix *> mats;
     DevBuf<ClassifyArgs> d_cargs;   // every problem
@@ -2089,6 +2272,7 @@ bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix
     WorkStream wsc(ctx->stream);
     std::unique_ptr<bmi_batch, void (*)(bmi_batch *)> Bt(new bmi_batch, batch_destroy);
     Bt->ctx = ctx;
+    Bt->device = ctx->device;
     Bt->plans.assign(plans, plans + n);
     Bt->mats.assign(matrices, matrices + n);
     BMI_HIP(hipDeviceGetAttribute(&Bt->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
@@ -2185,7 +2369,7 @@ bmi_status bmi_batch_stats_get(const bmi_batch *Bt, bmi_batch_stats *st) try {
 
 bmi_status bmi_batch_free(bmi_batch *Bt) try {
     if (Bt) {
-        (void)hipSetDevice(Bt->ctx->device);
+        (void)hipSetDevice(Bt->device);
         batch_destroy(Bt);
     }
     return BMI_OK;

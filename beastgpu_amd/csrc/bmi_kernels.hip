@@ -1754,8 +1754,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
     const int t = threadIdx.x;
     if (!MIXED) fill_sincos_table(s_sc);
     const int w = blockIdx.x;
-    const int P = a.panel0 + w / a.nbq;
-    const int q = a.bq_list[w - P * a.nbq];
+    const int pl = w / a.nbq;  // panel of the launch's range
+    const int P = a.panel0 + pl;
+    const int q = a.bq_list[w - pl * a.nbq];
     const int r0 = P * BMI_PANEL_ROWS;
     const int nr = min(BMI_PANEL_ROWS, a.nrows - r0);
     const int nc = a.bnc[q];
@@ -2209,8 +2210,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
     __shared__ PanelShared sh;
     if (!MIXED) fill_sincos_table(sh.s_sc);
     const int w = blockIdx.x;
-    const int P = a.panel0 + w / a.nbq;
-    const int q = a.bq_list[w - P * a.nbq];
+    const int pl = w / a.nbq;  // panel of the launch's range
+    const int P = a.panel0 + pl;
+    const int q = a.bq_list[w - pl * a.nbq];
     panel_block<OP, MIXED, FAST, OutT, true>(a, P, q, tile, sh);
 }
 

@@ -15,6 +15,10 @@
  *   - No exception or longjmp crosses the ABI; every call returns a status and
  *     bmi_last_error() gives a thread-local message for the last failure.
  *   - A context is externally synchronised: one thread at a time per context.
+ *   - Objects may be freed in any order: a free touches only its own object
+ *     (a mesh, space, plan, matrix or batch freed after its context, or a
+ *     space after its mesh, releases its device memory all the same).  An
+ *     object whose parents are gone accepts no other call.
  */
 #ifndef BMI_H
 #define BMI_H
@@ -94,6 +98,32 @@ typedef struct {
     double near_factor;      /* mixed: far pairs with centroid distance < near_factor * max edge stay FP64 */
 } bmi_options;
 
+/* Caller-supplied quadrature tables, in place of the orders of bmi_options:
+ * the far-pair triangle rules of test and trial cells and the Gauss-Legendre
+ * rules of the Sauter-Schwab hypercube per touching class.  A point (u, v)
+ * of a cell is v3 + u (v1 - v3) + v (v2 - v3), the chart of the reference
+ * (load_parameters.jl:25-46); weights are on the reference triangle (they sum
+ * to 1/2 for an exact rule; the library applies the Jacobian 2A).  BEAST's
+ * quaddata holds the same rules as per-cell Cartesian points and weights
+ * times the Jacobian (qd.tpoints / qd.bpoints, read by GPU_scheduler.jl:61-106);
+ * its Gauss-Legendre tables qd.gausslegendre feed the Sauter-Schwab rules
+ * (nonMainCaseQuadratures.jl:95-97, SauterSchwab.jl:407-414).  Rules equal to
+ * the defaults (3 x 4 points, 5-point Gauss-Legendre) run the same kernels as
+ * the defaults. */
+#define BMI_MAX_RULE_POINTS 4
+#define BMI_MAX_GL_POINTS 8
+typedef struct {
+    int32_t test_points;                          /* 1..BMI_MAX_RULE_POINTS */
+    int32_t trial_points;
+    double test_uv[BMI_MAX_RULE_POINTS][2];
+    double test_weight[BMI_MAX_RULE_POINTS];
+    double trial_uv[BMI_MAX_RULE_POINTS][2];
+    double trial_weight[BMI_MAX_RULE_POINTS];
+    int32_t gl_points[3];                         /* common vertex, edge, face: 1..BMI_MAX_GL_POINTS */
+    double gl_node[3][BMI_MAX_GL_POINTS];         /* on [0, 1] */
+    double gl_weight[3][BMI_MAX_GL_POINTS];       /* sum to 1 */
+} bmi_rules;
+
 /* Per-assembly phase timings (milliseconds, HIP events on the library stream)
  * and pair counts of the last bmi_assemble on a plan. */
 typedef struct {
@@ -141,6 +171,18 @@ bmi_status bmi_space_free(bmi_space *space);
  * of assemblechunk_gpu! / assemblechunk_body_gpu! (assamble_chunk_gpu.jl:49-144). */
 bmi_status bmi_plan_create(bmi_context *ctx, const bmi_operator *op, const bmi_space *test, const bmi_space *trial,
                            const bmi_options *opts, bmi_plan **out);
+/* The same with the caller's quadrature tables (rules); opts still gives the
+ * precision, near_order and near_factor.  Replaces the quaddata argument of
+ * the reference's chunk assembly (assamble_chunk_gpu.jl:75) as consumed by
+ * gether_wimps_and_womps / load_data (GPU_scheduler.jl:61-106).
+ * Test and trial spaces on different meshes: BMI_EUNSUPPORTED if a vertex of
+ * one mesh lies inside a cell of the other without being one of its vertices
+ * (one mesh refines or overlaps the other; the reference switches to
+ * BEAST's TestRefinesTrialQStrat / TrialRefinesTestQStrat there,
+ * assamble_chunk_gpu.jl:67-73).  bmi_plan_create applies the same check. */
+bmi_status bmi_plan_create_rules(bmi_context *ctx, const bmi_operator *op, const bmi_space *test,
+                                 const bmi_space *trial, const bmi_options *opts, const bmi_rules *rules,
+                                 bmi_plan **out);
 bmi_status bmi_plan_free(bmi_plan *plan);
 
 /* Device-resident matrix, column-major rows x cols.

@@ -82,6 +82,15 @@ typedef struct {
     int32_t far_trial;
     int32_t ss_order[3]; /* Gauss-Legendre points per hypercube dimension: CV, CE, CF */
     int32_t near_order;  /* collapsed Gauss rule order for class-4 pairs */
+    /* custom = 1: caller tables replace far_test / far_trial and ss_order.
+     * A point (u, v) of a cell is v3 + u (v1 - v3) + v (v2 - v3); weights on
+     * the reference triangle; Gauss-Legendre nodes and weights on [0, 1]. */
+    int32_t custom;
+    int32_t n_test, n_trial;
+    double test_uv[16][2], test_w[16];
+    double trial_uv[16][2], trial_w[16];
+    int32_t gl_n[3];
+    double gl_x[3][64], gl_w[3][64];
 } orc_quadrature;
 
 /* ------------------------------------------------------------------ rules */
@@ -527,15 +536,23 @@ static void ss_eval(const orc_operator *op, const orc_space *X, const orc_space 
     else accumulate(op, X->kind, Y->kind, &f, &g, kernel_value(op, x, y), 4.0 * t->area * s->area * mul, Z);
 }
 
+/* tx, tw: the caller's Gauss-Legendre table of n points on [0, 1], or NULL
+ * for the n-point rule. */
 static void sauter_schwab(const orc_operator *op, const orc_space *X, const orc_space *Y,
-                          const cell *t, const cell *s, int cls, int n, cplx Z[3][3]) {
+                          const cell *t, const cell *s, int cls, int n, const double *tx, const double *tw,
+                          cplx Z[3][3]) {
     int PT[3], PS[3];
     ss_permutation(cls, t, s, PT, PS);
     ref_chart c1, c2;
     make_ref_chart(PT, &c1);
     make_ref_chart(PS, &c2);
     double gx[64], gw[64];
-    orc_gauss_legendre01(n, gx, gw);
+    if (tx) {
+        memcpy(gx, tx, sizeof(double) * (size_t)n);
+        memcpy(gw, tw, sizeof(double) * (size_t)n);
+    } else {
+        orc_gauss_legendre01(n, gx, gw);
+    }
     for (int a = 0; a < n; a++)
         for (int b = 0; b < n; b++)
             for (int c = 0; c < n; c++)
@@ -587,7 +604,15 @@ int orc_pair_block(const orc_operator *op, const orc_space *X, const orc_space *
     memset(Z, 0, sizeof(Z));
     if (cls == ORC_FAR) {
         tri_rule rt, rs;
-        if (make_far_rule(q->far_test, &rt) || make_far_rule(q->far_trial, &rs)) return -1;
+        if (q->custom) {
+            if (q->n_test < 1 || q->n_test > 16 || q->n_trial < 1 || q->n_trial > 16) return -1;
+            rt.n = q->n_test;
+            rs.n = q->n_trial;
+            for (int i = 0; i < rt.n; i++) { rt.b1[i] = q->test_uv[i][0]; rt.b2[i] = q->test_uv[i][1]; rt.w[i] = q->test_w[i]; }
+            for (int i = 0; i < rs.n; i++) { rs.b1[i] = q->trial_uv[i][0]; rs.b2[i] = q->trial_uv[i][1]; rs.w[i] = q->trial_w[i]; }
+        } else if (make_far_rule(q->far_test, &rt) || make_far_rule(q->far_trial, &rs)) {
+            return -1;
+        }
         double_rule(op, X, Y, &t, &s, &rt, &rs, Z);
     } else if (cls == ORC_NEAR) {
         tri_rule rn;
@@ -595,9 +620,10 @@ int orc_pair_block(const orc_operator *op, const orc_space *X, const orc_space *
         if (op->kernel == ORC_KERNEL_GREEN && op->op != ORC_MW_DL) near_subtraction(op, X, Y, &t, &s, &rn, Z);
         else double_rule(op, X, Y, &t, &s, &rn, &rn, Z);
     } else if (cls <= 3) {
-        int n = q->ss_order[cls - 1];
+        int n = q->custom ? q->gl_n[cls - 1] : q->ss_order[cls - 1];
         if (n < 1 || n > 64) return -1;
-        sauter_schwab(op, X, Y, &t, &s, cls, n, Z);
+        sauter_schwab(op, X, Y, &t, &s, cls, n, q->custom ? q->gl_x[cls - 1] : NULL,
+                      q->custom ? q->gl_w[cls - 1] : NULL, Z);
     } else {
         return -1;
     }

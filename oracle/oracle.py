@@ -92,6 +92,16 @@ class _Quad(ctypes.Structure):
         ("far_trial", ctypes.c_int32),
         ("ss_order", ctypes.c_int32 * 3),
         ("near_order", ctypes.c_int32),
+        ("custom", ctypes.c_int32),
+        ("n_test", ctypes.c_int32),
+        ("n_trial", ctypes.c_int32),
+        ("test_uv", (ctypes.c_double * 2) * 16),
+        ("test_w", ctypes.c_double * 16),
+        ("trial_uv", (ctypes.c_double * 2) * 16),
+        ("trial_w", ctypes.c_double * 16),
+        ("gl_n", ctypes.c_int32 * 3),
+        ("gl_x", (ctypes.c_double * 64) * 3),
+        ("gl_w", (ctypes.c_double * 64) * 3),
     ]
 
 
@@ -157,22 +167,40 @@ def _space(X, keep):
     return s
 
 
-def _quad(far_test=2, far_trial=3, ss_order=(5, 5, 5), near_order=4) -> _Quad:
+def _quad(far_test=2, far_trial=3, ss_order=(5, 5, 5), near_order=4, rules=None) -> _Quad:
+    """rules: optional tables (test_uv (n, 2), test_weights (n,), trial_uv,
+    trial_weights, gauss_legendre = three (nodes, weights) pairs on [0, 1] for
+    CV, CE, CF), given as an object with those attributes."""
     q = _Quad()
     q.far_test = far_test
     q.far_trial = far_trial
     for i in range(3):
         q.ss_order[i] = int(ss_order[i])
     q.near_order = near_order
+    if rules is not None:
+        q.custom = 1
+        tu, tw = np.asarray(rules.test_uv, float).reshape(-1, 2), np.asarray(rules.test_weights, float).ravel()
+        bu, bw = np.asarray(rules.trial_uv, float).reshape(-1, 2), np.asarray(rules.trial_weights, float).ravel()
+        q.n_test, q.n_trial = tu.shape[0], bu.shape[0]
+        for i in range(q.n_test):
+            q.test_uv[i][0], q.test_uv[i][1], q.test_w[i] = tu[i, 0], tu[i, 1], tw[i]
+        for i in range(q.n_trial):
+            q.trial_uv[i][0], q.trial_uv[i][1], q.trial_w[i] = bu[i, 0], bu[i, 1], bw[i]
+        for g, (x, w) in enumerate(rules.gauss_legendre):
+            x, w = np.asarray(x, float).ravel(), np.asarray(w, float).ravel()
+            q.gl_n[g] = x.size
+            for i in range(x.size):
+                q.gl_x[g][i], q.gl_w[g][i] = x[i], w[i]
     return q
 
 
 def assemble(op, X, Y, *, rows=None, ss_order=(5, 5, 5), far_test=2, far_trial=3, near_order=4,
-             kernel=KERNEL_GREEN, lin=(0.0, 0.0, 0.0), nthreads=0):
-    """Z (complex128, rows x Y.nfun) and per-class pair counts (far, CV, CE, CF, near)."""
+             kernel=KERNEL_GREEN, lin=(0.0, 0.0, 0.0), nthreads=0, rules=None):
+    """Z (complex128, rows x Y.nfun) and per-class pair counts (far, CV, CE, CF, near);
+    rules: caller quadrature tables in place of far_test / far_trial / ss_order."""
     keep: list = []
     o, sx, sy = _op(op, kernel, lin), _space(X, keep), _space(Y, keep)
-    q = _quad(far_test, far_trial, ss_order, near_order)
+    q = _quad(far_test, far_trial, ss_order, near_order, rules)
     if rows is not None:
         rows = np.ascontiguousarray(rows, dtype=np.int64)
         nout = rows.size
@@ -189,11 +217,11 @@ def assemble(op, X, Y, *, rows=None, ss_order=(5, 5, 5), far_test=2, far_trial=3
 
 
 def pair_block(op, X, Y, p, q, *, ss_order=(5, 5, 5), far_test=2, far_trial=3, near_order=4,
-               kernel=KERNEL_GREEN, lin=(0.0, 0.0, 0.0)):
+               kernel=KERNEL_GREEN, lin=(0.0, 0.0, 0.0), rules=None):
     """Local 3x3 element matrix (before coefficients) of test cell p, trial cell q."""
     keep: list = []
     o, sx, sy = _op(op, kernel, lin), _space(X, keep), _space(Y, keep)
-    qd = _quad(far_test, far_trial, ss_order, near_order)
+    qd = _quad(far_test, far_trial, ss_order, near_order, rules)
     out = np.zeros(18)
     cls = lib().orc_pair_block(ctypes.byref(o), ctypes.byref(sx), ctypes.byref(sy), ctypes.byref(qd),
                                int(p), int(q), out.ctypes.data)

@@ -163,7 +163,7 @@ def _c_layout(struct: str, fields: list) -> list:
 @pytest.mark.parametrize("cname,pyname", [
     ("bmi_operator", "Operator"), ("bmi_options", "Options"), ("bmi_stats", "Stats"),
     ("bmi_server_options", "ServerOptions"), ("bmi_request", "Request"), ("bmi_result", "Result"),
-    ("bmi_batch_stats", "BatchStats"),
+    ("bmi_batch_stats", "BatchStats"), ("bmi_rules", "Rules"),
 ])
 def test_ctypes_layouts_match_c_compiler(cname, pyname):
     S = getattr(_bmi, pyname)

@@ -597,3 +597,77 @@ def test_dl_ss_convergence():
     # a cell against itself is coplanar: its block vanishes
     cf, _ = O.pair_block(op, X, X, *pairs[O.CF])
     assert np.abs(cf).max() <= 1e-14 * np.abs(far).max() * 1e3
+
+
+# ------------------------------------------------------ caller rule tables
+
+def _table(test_order, trial_order, gl):
+    """Quadrature tables from the oracle's own rules (for the identity check)."""
+    from beastgpu_amd.api import QuadratureRules
+
+    tb1, tb2, tw = O.far_rule(test_order)
+    bb1, bb2, bw = O.far_rule(trial_order)
+    return QuadratureRules(np.stack([tb1, tb2], 1), tw, np.stack([bb1, bb2], 1), bw,
+                           tuple(O.gauss_legendre01(n) for n in gl))
+
+
+def _edge_midpoint_tables(gl=(6, 4, 7)):
+    """Test cells: the edge-midpoint rule (degree 2; not the interior 3-point
+    rule of the defaults).  Trial cells: the 2 x 2 collapsed Gauss product
+    rule (degree 3, no symmetry).  Gauss-Legendre from numpy's nodes."""
+    from beastgpu_amd.api import QuadratureRules
+
+    cb1, cb2, cw = O.collapsed_rule(2)
+    gls = []
+    for n in gl:
+        x, w = np.polynomial.legendre.leggauss(n)
+        gls.append((0.5 * (x + 1.0), 0.5 * w))
+    return QuadratureRules([[0.5, 0.5], [0.0, 0.5], [0.5, 0.0]], [1.0 / 6] * 3, np.stack([cb1, cb2], 1), cw,
+                           tuple(gls))
+
+
+def test_rule_tables_equal_to_the_builtin_rules_give_the_same_blocks():
+    m = meshes.perturb_radial(meshes.icosphere(1), 5, 0.1)
+    op, X = Maxwell3D.singlelayer(wavenumber=1.3), spaces.raviartthomas(m)
+    rules = _table(2, 3, (5, 6, 4))
+    for c, (p, q) in _pairs_by_class(op, X).items():
+        Zd, cd = O.pair_block(op, X, X, p, q, ss_order=(5, 6, 4))
+        Zt, ct = O.pair_block(op, X, X, p, q, rules=rules)
+        assert cd == ct == c and np.array_equal(Zd, Zt), c
+
+
+@pytest.mark.parametrize("kind", ["efie", "hh3d"])
+@pytest.mark.parametrize("kernel", [O.KERNEL_ONE, O.KERNEL_LINEAR])
+def test_rule_tables_far_pair_polynomial_kernels(kind, kernel):
+    """The caller's far tables (edge midpoints on the test cell, a collapsed
+    product rule on the trial cell) integrate linear shapes times a kernel of
+    degree <= 1 exactly: against the closed form."""
+    m = meshes.perturb_radial(meshes.icosphere(1), 5, 0.1)
+    if kind == "efie":
+        op, X = Maxwell3D.singlelayer(wavenumber=1.3), spaces.raviartthomas(m)
+    else:
+        op, X = Helmholtz3D.singlelayer(wavenumber=1.3), spaces.lagrangec0d1(m)
+    p, q = _pairs_by_class(op, X)[O.FAR]
+    lin = (0.3, -1.1, 0.7)
+    Zo, cls = O.pair_block(op, X, X, p, q, kernel=kernel, lin=lin, rules=_edge_midpoint_tables())
+    assert cls == O.FAR
+    Zr = _poly_reference(op, X, X, p, q, kernel, lin)
+    assert np.abs(Zo - Zr).max() <= 1e-13 * np.abs(Zr).max()
+    # a mistake in the table's weights or points shows (the rule is not the default)
+    Zd, _ = O.pair_block(op, X, X, p, q, kernel=O.KERNEL_GREEN)
+    Zt, _ = O.pair_block(op, X, X, p, q, kernel=O.KERNEL_GREEN, rules=_edge_midpoint_tables())
+    assert np.abs(Zd - Zt).max() > 1e-9 * np.abs(Zd).max()
+
+
+def test_rule_tables_gauss_legendre_match_the_builtin_orders():
+    """numpy's Gauss-Legendre nodes as caller tables reproduce the oracle's
+    own n-point rules in the Sauter-Schwab integrals to rounding."""
+    m = meshes.perturb_radial(meshes.icosphere(1), 5, 0.1)
+    op, X = Maxwell3D.singlelayer(wavenumber=1.3), spaces.raviartthomas(m)
+    t = _edge_midpoint_tables(gl=(6, 4, 7))
+    for c, (p, q) in _pairs_by_class(op, X).items():
+        if c == O.FAR:
+            continue
+        Zd, _ = O.pair_block(op, X, X, p, q, ss_order=(6, 4, 7))
+        Zt, _ = O.pair_block(op, X, X, p, q, rules=t)
+        assert np.abs(Zd - Zt).max() <= 1e-14 * np.abs(Zd).max(), c
