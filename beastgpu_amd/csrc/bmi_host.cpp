@@ -878,7 +878,8 @@ bmi_status matrix_apply_on(bmi_context *run, const bmi_matrix *m, const double *
     hipStream_t s = run->stream;
     WorkStream wsc(s);
     const int rows = (int)m->rows, cols = (int)m->cols;
-    const int nchunk = (cols + BMI_APPLY_CHUNK - 1) / BMI_APPLY_CHUNK;
+    int chunk = 0;
+    const int nchunk = apply_chunks(rows, cols, &chunk);
     const size_t nx = (size_t)cols * nrhs, ny = (size_t)rows * nrhs, np = (size_t)nchunk * rows * nrhs;
     const size_t need = sizeof(double2) * (nx + ny + np);
     if (need > run->ws_bytes) {

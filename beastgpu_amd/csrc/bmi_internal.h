@@ -29,7 +29,7 @@ static_assert(BMI_PANEL_ROWS_WIDE * BMI_MAX_COLS_WIDE <= BMI_PANEL_ROWS * BMI_MA
 #define BMI_MAX_GL 8        // Gauss-Legendre points per hypercube dimension
 #define BMI_MAX_NEAR 64     // points of the collapsed near rule per cell
 #define BMI_MAX_RHS 8       // right-hand sides per bmi_matrix_apply call
-#define BMI_APPLY_CHUNK 512 // columns per work-group of the apply kernel
+#define BMI_APPLY_CHUNK 512 // columns of x staged in LDS at a time by the apply kernel
 
 enum { OPK_HH3D = 1, OPK_MW = 2, OPK_DL = 3 };
 enum { CLS_FAR = 0, CLS_CV = 1, CLS_CE = 2, CLS_CF = 3, CLS_NEAR = 4, CLS_FAR64 = 5 };
@@ -190,6 +190,9 @@ hipError_t launch_panels_queue(const PanelArgs *args, const int4 *items, const i
                                int fast, int c32, hipStream_t s);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
+// column chunks of the apply kernels (and the chunk width); the partial
+// sums take nchunk x rows x nrhs complex values
+int apply_chunks(int rows, int cols, int *chunk);
 hipError_t launch_apply(const void *Z, int c32, long long ld, int rows, int cols, const double2 *x, int nrhs,
                         double2 *partial, double2 *y, hipStream_t s);
 hipError_t launch_compare(const void *a, int a32, const void *b, int b32, long long n, int groups, double *out,
@@ -218,4 +221,5 @@ int32_t set_error(int32_t status, const std::string &msg);
 // status for an exception caught at the C boundary (call inside a catch block)
 int32_t fail_exception() noexcept;
 int32_t matrix_apply_on(bmi_context *run, const bmi_matrix *m, const double *x, double *y, int32_t nrhs);
+
 int32_t matrix_wrap(bmi_context *ctx, void *mem, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out);

@@ -2428,8 +2428,9 @@ hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
 
 // y = Z x for up to BMI_MAX_RHS right-hand sides.  Grid: (row blocks of 256,
 // column chunks); each thread owns one row of one chunk, reads its column
-// entries coalesced across the wave, and writes a partial sum; k_apply_reduce
-// adds the chunk partials in chunk order (deterministic).
+// entries coalesced across the wave (x staged in LDS BMI_APPLY_CHUNK columns
+// at a time), and writes a partial sum; k_apply_reduce adds the chunk
+// partials in chunk order (deterministic).
 template <typename T>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_apply(const T *Z, long long ld, int rows, int cols,
                                                              const double2 *x, int nrhs, int chunk,
@@ -2438,38 +2439,53 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_apply(const T *Z, long lo
     const int row = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
     const int c0 = blockIdx.y * chunk;
     const int c1 = min(c0 + chunk, cols);
-    for (int i = threadIdx.x; i < (c1 - c0) * nrhs; i += BMI_BLOCK_THREADS) {
-        const int c = i / nrhs, r = i - c * nrhs;
-        sx[i] = x[(long long)r * cols + c0 + c];
-    }
-    __syncthreads();
-    if (row >= rows) return;
     double ar[BMI_MAX_RHS], ai[BMI_MAX_RHS];
 #pragma unroll
     for (int r = 0; r < BMI_MAX_RHS; ++r) ar[r] = ai[r] = 0.0;
-    // eight columns' entries are loaded before any is used, so that each
-    // wave keeps 8 KB of the matrix in flight
-    constexpr int U = 8;
-    for (int cb = c0; cb < c1; cb += U) {
-        T zz[U];
+    for (int s0 = c0; s0 < c1; s0 += BMI_APPLY_CHUNK) {
+        const int s1 = min(s0 + BMI_APPLY_CHUNK, c1);
+        __syncthreads();  // the previous sub-chunk's x is consumed
+        for (int i = threadIdx.x; i < (s1 - s0) * nrhs; i += BMI_BLOCK_THREADS) {
+            const int c = i / nrhs, r = i - c * nrhs;
+            sx[i] = x[(long long)r * cols + s0 + c];
+        }
+        __syncthreads();
+        if (row >= rows) continue;
+        // eight columns' entries are loaded before any is used, so that each
+        // wave keeps 8 KB of the matrix in flight
+        constexpr int U = 8;
+        for (int cb = s0; cb < s1; cb += U) {
+            T zz[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) zz[u] = cb + u < c1 ? Z[row + (long long)(cb + u) * ld] : T{};
+            for (int u = 0; u < U; ++u) zz[u] = cb + u < s1 ? Z[row + (long long)(cb + u) * ld] : T{};
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (cb + u >= c1) break;
-            const double zr = (double)zz[u].x, zi = (double)zz[u].y;
-            const double2 *xc = &sx[(cb + u - c0) * nrhs];
+            for (int u = 0; u < U; ++u) {
+                if (cb + u >= s1) break;
+                const double zr = (double)zz[u].x, zi = (double)zz[u].y;
+                const double2 *xc = &sx[(cb + u - s0) * nrhs];
 #pragma unroll
-            for (int r = 0; r < BMI_MAX_RHS; ++r) {
-                if (r >= nrhs) break;
-                const double2 v = xc[r];
-                ar[r] = fma(zr, v.x, fma(-zi, v.y, ar[r]));
-                ai[r] = fma(zr, v.y, fma(zi, v.x, ai[r]));
+                for (int r = 0; r < BMI_MAX_RHS; ++r) {
+                    if (r >= nrhs) break;
+                    const double2 v = xc[r];
+                    ar[r] = fma(zr, v.x, fma(-zi, v.y, ar[r]));
+                    ai[r] = fma(zr, v.y, fma(zi, v.x, ai[r]));
+                }
             }
         }
     }
+    if (row >= rows) return;
     for (int r = 0; r < nrhs; ++r)
         partial[((long long)blockIdx.y * nrhs + r) * rows + row] = make_double2(ar[r], ai[r]);
+}
+
+// Column chunks of the apply: enough (row block, chunk) work-groups to fill
+// the device (at least 2048), chunks of at least 256 columns.
+int apply_chunks(int rows, int cols, int *chunk) {
+    const int rb = (rows + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
+    const int most = std::max(1, (cols + 255) / 256);
+    const int n = std::max(1, std::min(most, (2048 + rb - 1) / rb));
+    *chunk = (cols + n - 1) / n;
+    return (cols + *chunk - 1) / *chunk;
 }
 
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_apply_reduce(const double2 *partial, int nchunk, int rows,
@@ -2487,14 +2503,15 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_apply_reduce(const double
 
 hipError_t launch_apply(const void *Z, int c32, long long ld, int rows, int cols, const double2 *x, int nrhs,
                         double2 *partial, double2 *y, hipStream_t s) {
-    const int nchunk = (cols + BMI_APPLY_CHUNK - 1) / BMI_APPLY_CHUNK;
+    int chunk = 0;
+    const int nchunk = apply_chunks(rows, cols, &chunk);
     dim3 grid((rows + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS, nchunk);
     if (c32)
         hipLaunchKernelGGL(k_apply<float2>, grid, dim3(BMI_BLOCK_THREADS), 0, s, (const float2 *)Z, ld, rows, cols, x,
-                           nrhs, BMI_APPLY_CHUNK, partial);
+                           nrhs, chunk, partial);
     else
         hipLaunchKernelGGL(k_apply<double2>, grid, dim3(BMI_BLOCK_THREADS), 0, s, (const double2 *)Z, ld, rows, cols,
-                           x, nrhs, BMI_APPLY_CHUNK, partial);
+                           x, nrhs, chunk, partial);
     const long long n = (long long)rows * nrhs;
     hipLaunchKernelGGL(k_apply_reduce, dim3((unsigned)((n + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS)),
                        dim3(BMI_BLOCK_THREADS), 0, s, partial, nchunk, rows, nrhs, y);
