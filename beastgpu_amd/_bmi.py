@@ -28,7 +28,7 @@ EXPORTS = (
     "bmi_plan_create", "bmi_plan_create_rules", "bmi_plan_free",
     "bmi_matrix_create", "bmi_matrix_free", "bmi_matrix_device_ptr",
     "bmi_assemble", "bmi_plan_stats", "bmi_matrix_to_host", "bmi_assemble_to_host",
-    "bmi_matrix_apply", "bmi_matrix_compare", "bmi_device_memory",
+    "bmi_matrix_apply", "bmi_matrix_apply_noalloc", "bmi_matrix_compare", "bmi_device_memory",
     "bmi_server_create", "bmi_server_destroy", "bmi_server_submit", "bmi_server_wait",
     "bmi_server_release", "bmi_server_memory", "bmi_request_bytes",
     "bmi_batch_create", "bmi_batch_assemble", "bmi_batch_stats_get", "bmi_batch_free",
@@ -202,6 +202,7 @@ def lib():
     L.bmi_matrix_to_host.argtypes = [vp, vp, i32]
     L.bmi_assemble_to_host.argtypes = [vp, vp, i32, i64]
     L.bmi_matrix_apply.argtypes = [vp, vp, vp, i32]
+    L.bmi_matrix_apply_noalloc.argtypes = [vp, vp, vp, i32]
     L.bmi_matrix_compare.argtypes = [vp, vp, vp]
     L.bmi_device_memory.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.bmi_server_create.argtypes = [i32, ctypes.POINTER(ServerOptions), pp]

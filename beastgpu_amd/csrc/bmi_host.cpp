@@ -898,6 +898,10 @@ bmi_status matrix_apply_on(bmi_context *run, const bmi_matrix *m, const double *
     return BMI_OK;
 }
 
+MatrixView matrix_view(const bmi_matrix *m) {
+    return MatrixView{m->d, m->rows, m->cols, m->dtype, m->device, m->ctx->stream};
+}
+
 // A matrix over caller-owned device memory of at least rows*cols elements.
 bmi_status matrix_wrap(bmi_context *ctx, void *mem, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out) {
     if (!ctx || !mem || !out) return fail(BMI_EINVAL, "NULL argument");
@@ -1153,6 +1157,14 @@ bmi_status bmi_context_create(int32_t device, bmi_context **out) try {
     auto ctx = std::make_unique<bmi_context>();
     ctx->device = device;
     BMI_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // once per device: the address of the allocation-free apply's scratch
+    {
+        const bmi_status st = scratch_setup(device);
+        if (st != BMI_OK) {
+            (void)hipStreamDestroy(ctx->stream);
+            return st;
+        }
+    }
     // keep freed pool memory cached for the next allocation
     hipMemPool_t pool = nullptr;
     if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {

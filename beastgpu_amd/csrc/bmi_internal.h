@@ -221,5 +221,16 @@ int32_t set_error(int32_t status, const std::string &msg);
 // status for an exception caught at the C boundary (call inside a catch block)
 int32_t fail_exception() noexcept;
 int32_t matrix_apply_on(bmi_context *run, const bmi_matrix *m, const double *x, double *y, int32_t nrhs);
-
+// the fields of a matrix the allocation-free apply needs
+struct MatrixView {
+    const void *d;
+    int64_t rows, cols;
+    int32_t dtype;
+    int device;
+    hipStream_t stream;  // its context's
+};
+MatrixView matrix_view(const bmi_matrix *m);
+// address lookup of the allocation-free apply's scratch on the current
+// device dev, once per device (bmi_noalloc.hip)
+int32_t scratch_setup(int dev);
 int32_t matrix_wrap(bmi_context *ctx, void *mem, int64_t rows, int64_t cols, int32_t dtype, bmi_matrix **out);

@@ -228,6 +228,17 @@ bmi_status bmi_assemble_to_host(bmi_plan *plan, void *dst, int32_t dtype, int64_
  * 15 GB device-to-host copy. */
 bmi_status bmi_matrix_apply(const bmi_matrix *m, const double *x, double *y, int32_t nrhs);
 
+/* bmi_matrix_apply without any device allocation, for callers that cannot
+ * allocate (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The device copies of
+ * x and y and the apply kernels' partial sums live in a 128 MiB device array
+ * that each GPU the library runs on holds until the process ends; sizes that
+ * need more ((cols + rows + chunks x rows) x nrhs x 16 bytes, chunks <= 17 at
+ * 30,720 rows) fail with BMI_EINVAL.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU race.  Values are bitwise those of
+ * bmi_matrix_apply. */
+bmi_status bmi_matrix_apply_noalloc(const bmi_matrix *m, const double *x, double *y, int32_t nrhs);
+
 /* out[0] = ||a - ref||_F, out[1] = ||ref||_F, out[2] = max |a_ij - ref_ij|, computed
  * on the device (a and ref of equal shape, any element types).
  * Replaces test(M, M_ref) of test/difference.jl:1-6, which compared one column. */
