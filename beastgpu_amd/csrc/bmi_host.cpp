@@ -928,8 +928,11 @@ bmi_status check_target(const bmi_plan *P, const bmi_matrix *out) {
 }
 
 // 1 if the far kernels' compile-time rule applies: default 3 x 4 point
-// rule, lossless operator.
-int fast_rule(const bmi_plan *P) { return P->rule_default && !P->opc.lossy; }
+// rule, lossless operator, alpha != 0 (those kernels accumulate entries
+// divided by alpha).
+int fast_rule(const bmi_plan *P) {
+    return P->rule_default && !P->opc.lossy && (P->opc.alpha_re != 0.0 || P->opc.alpha_im != 0.0);
+}
 
 // 1 if the plan's Sauter-Schwab rules all have 5 points per dimension and the
 // same nodes and weights (the compile-time-order kernel takes one table).
@@ -1439,6 +1442,16 @@ static bmi_status plan_create(bmi_context *ctx, const bmi_operator *op, const bm
     P->opc.gamma_re = op->gamma[0];
     P->opc.k2pi = op->gamma[1] / (2.0 * kPi);
     P->opc.lossy = op->gamma[0] != 0.0;
+    {
+        const double an = op->alpha[0] * op->alpha[0] + op->alpha[1] * op->alpha[1];
+        if (an > 0.0) {
+            P->opc.ia_re = op->alpha[0] / an;
+            P->opc.ia_im = -op->alpha[1] / an;
+            // 4 beta / alpha = 4 beta conj(alpha) / |alpha|^2
+            P->opc.bp_re = 4.0 * (op->beta[0] * op->alpha[0] + op->beta[1] * op->alpha[1]) / an;
+            P->opc.bp_im = 4.0 * (op->beta[1] * op->alpha[0] - op->beta[0] * op->alpha[1]) / an;
+        }
+    }
 
     double lt[BMI_MAX_RULE][3] = {}, lb[BMI_MAX_RULE][3] = {}, wt[BMI_MAX_RULE] = {}, wb[BMI_MAX_RULE] = {};
     int nt = 0, nb = 0;

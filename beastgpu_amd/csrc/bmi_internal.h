@@ -57,6 +57,10 @@ struct OpConst {
     double gamma_re;   // attenuation; 0 for a real wavenumber
     double k2pi;       // Im(gamma) / (2 pi), the wavenumber in turns per unit length
     int lossy;         // gamma_re != 0
+    // far kernels with the compile-time rule accumulate entries divided by
+    // alpha (alpha != 0) and multiply by alpha when they write the matrix
+    double bp_re, bp_im;  // 4 beta / alpha
+    double ia_re, ia_im;  // 1 / alpha
 };
 
 struct RuleConst {

@@ -296,7 +296,7 @@ __device__ __forceinline__ void hh_epilogue(const OpConst &op, const double *pr,
 // so moments of h = w w' q carry the pair (eight, of which two pairs enter
 // only as differences and are accumulated so); the 1/(2A) of each RWG
 // shape cancels the Jacobian 2A of the point weights.  S is the arithmetic
-// type (double, or float for the mixed path).
+// type (double, or float for the mixed path).  FAST: entries divided by alpha.
 template <typename S, int FAST, bool TAB, typename G = double>
 __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule_, const G *xp, const G *xa,
                                         const double *xc, const G *yp, const G *ya, const double *yc,
@@ -387,11 +387,22 @@ __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule
             const S ab = fma(a2, b2, fma(a1, b1, a0 * b0)), nb = fma(n2, b2, fma(n1, b1, n0 * b0));
             const S vr = fma(nb, gr, fma(ab, Dr, fma(-b2, Br[2], fma(-b1, Br[1], fma(-b0, Br[0], ir)))));
             const S vi = fma(nb, gi, fma(ab, Di, fma(-b2, Bi[2], fma(-b1, Bi[1], fma(-b0, Bi[0], ii)))));
-            loc[3 * i + j].x = (double)fma(alr, vr, -ali * vi);
-            loc[3 * i + j].y = (double)fma(alr, vi, ali * vr);
+            if constexpr (FAST) {  // divided by alpha (applied at the write-back)
+                loc[3 * i + j].x = (double)vr;
+                loc[3 * i + j].y = (double)vi;
+            } else {
+                loc[3 * i + j].x = (double)fma(alr, vr, -ali * vi);
+                loc[3 * i + j].y = (double)fma(alr, vi, ali * vr);
+            }
         }
     }
 }
+
+// EFIE far pair with the default 3 x 4 point rules (below).
+template <typename S, bool TAB, typename G>
+__device__ __forceinline__ void pair_mw_rule(const OpConst &op, const G *xp, const G *xa, const double *xc,
+                                             const G *yp, const G *ya, const double *yc, double2 loc[9],
+                                             const double2 *tab);
 
 // -------------------------------------------------- far pair, FP64 rule
 
@@ -399,6 +410,7 @@ __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule
 // (doubleQuadRuleMainCase.jl:83-101), FP64 throughout.
 //   xp, yp: rule points relative to the cell centroids (BMI_MAX_RULE x 3)
 //   xa, ya: vertices relative to the centroids (3 x 3)
+// FAST (the default rule at compile time): the entries divided by alpha.
 template <int OP, int FAST, bool TAB>
 __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &rule_, const double *xp,
                                           const double *xa, const double *xc, double xs, const double *yp,
@@ -406,6 +418,10 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
                                           const double2 *tab) {
     if constexpr (OP == OPK_DL) {
         pair_dl<double, FAST, TAB>(op, rule_, xp, xa, xc, yp, ya, yc, loc, tab);
+        return;
+    }
+    if constexpr (OP == OPK_MW && FAST) {
+        pair_mw_rule<double, TAB, double>(op, xp, xa, xc, yp, ya, yc, loc, tab);
         return;
     }
     const FarRule<FAST> rule{rule_, op};
@@ -464,15 +480,15 @@ __device__ __forceinline__ void pair_fp64(const OpConst &op, const RuleConst &ru
                 ti[j] += vi[I][j];
             }
         }
-        double pr[9], pi[9];
+        // divided by alpha (applied at the write-back)
+        const double sc = 4.0 * xs * ys * (1.0 / 6.0);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                pr[3 * i + j] = fma(3.0, vr[i][j], tr[j]);
-                pi[3 * i + j] = fma(3.0, vi[i][j], ti[j]);
+                loc[3 * i + j].x = sc * fma(3.0, vr[i][j], tr[j]);
+                loc[3 * i + j].y = sc * fma(3.0, vi[i][j], ti[j]);
             }
-        hh_epilogue(op, pr, pi, 4.0 * xs * ys * (1.0 / 6.0), loc);
     } else {
         double pr[9] = {}, pi[9] = {};
 #pragma unroll
@@ -522,14 +538,114 @@ __device__ __forceinline__ void green32(const OpConst &op, bool lossy, float w4p
     hi = -h * s;
 }
 
+// EFIE far pair with the default 3 x 4 point rules, entries divided by alpha.
+// Relative to the centroids the rules' points are fixed fractions of the
+// vertices a_i = v_i - c_K, b_j = v_j - c_L: test point I (barycentric 2/3 at
+// vertex I) is a_I / 2, trial point 0 is the centroid and trial point J
+// (3/5 at vertex J - 1) is (2/5) b_(J-1).  With h_IJ = w_I w_J G / (4 pi),
+// C_Ij = a_I . b_j, T_I = sum_J h_IJ, R_j = sum_I h_I,j+1 and
+// S = sum_I,j h_I,j+1 C_Ij, the moment form of mw_epilogue becomes
+//   sum_IJ h_IJ (x_I - v_i).(y_J - v_j)
+//     = S / 5 - (2/5) sum_j' R_j' C_ij' - (1/2) sum_I T_I C_Ij + C_ij M0
+// (M0 = sum_I T_I): three sums per evaluation instead of eight moments, and
+// an epilogue on the 3 x 3 matrix C.  The divergence term 4 beta M0 enters
+// as (4 beta / alpha) M0.  R is taken between the records' points, as
+// computed.  S: arithmetic type (double, or float for the mixed path, whose
+// phase is reduced in FP64); G: type of the records' points and vertices.
+template <typename S, bool TAB, typename G>
+__device__ __forceinline__ void pair_mw_rule(const OpConst &op, const G *xp, const G *xa, const double *xc,
+                                             const G *yp, const G *ya, const double *yc, double2 loc[9],
+                                             const double2 *tab) {
+    const S dcx = (S)(xc[0] - yc[0]), dcy = (S)(xc[1] - yc[1]), dcz = (S)(xc[2] - yc[2]);
+    // FP64: the vertices relative to a centroid sum to zero, so row and
+    // column 2 of C follow from the 2 x 2 block (to rounding), which keeps
+    // five registers pairs free across the evaluations; FP32 takes all nine
+    // products (their rounding is that of the FP32 evaluation itself)
+    constexpr bool DERIVE = sizeof(S) == 8;
+    S C[3][3];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (!DERIVE || (I < 2 && j < 2))
+                C[I][j] = fma((S)xa[3 * I + 2], (S)ya[3 * j + 2], fma((S)xa[3 * I + 1], (S)ya[3 * j + 1],
+                                                                      (S)xa[3 * I] * (S)ya[3 * j]));
+    auto Cx = [&](int I, int j) -> S {
+        if (!DERIVE || (I < 2 && j < 2)) return C[I][j];
+        if (I < 2) return -(C[I][0] + C[I][1]);
+        if (j < 2) return -(C[0][j] + C[1][j]);
+        return (C[0][0] + C[0][1]) + (C[1][0] + C[1][1]);
+    };
+    S Tr[3], Ti[3], Rr[3] = {}, Ri[3] = {}, Sr = 0, Si = 0;
+#pragma unroll
+    for (int I = 0; I < 3; ++I) {
+        const S ux = dcx + (S)xp[3 * I], uy = dcy + (S)xp[3 * I + 1], uz = dcz + (S)xp[3 * I + 2];
+        S tr = 0, ti = 0;
+#pragma unroll
+        for (int J = 0; J < 4; ++J) {
+            const S w = (S)(frule_w(BMI_FAST_TEST_ORDER, I) * frule_w(BMI_FAST_TRIAL_ORDER, J) /
+                            (4.0 * 3.14159265358979323846));
+            S hr, hi;
+            if constexpr (sizeof(S) == 8) {
+                green64<TAB>(op, false, w, ux - (S)yp[3 * J], uy - (S)yp[3 * J + 1], uz - (S)yp[3 * J + 2], hr, hi,
+                             tab);
+            } else {
+                green32(op, false, w, ux - (S)yp[3 * J], uy - (S)yp[3 * J + 1], uz - (S)yp[3 * J + 2], hr, hi);
+            }
+            tr += hr;
+            ti += hi;
+            if (J > 0) {
+                const S c = Cx(I, J - 1);
+                Rr[J - 1] += hr;
+                Ri[J - 1] += hi;
+                Sr = fma(c, hr, Sr);
+                Si = fma(c, hi, Si);
+            }
+        }
+        Tr[I] = tr;
+        Ti[I] = ti;
+    }
+    const S M0r = Tr[0] + Tr[1] + Tr[2], M0i = Ti[0] + Ti[1] + Ti[2];
+    const S bpr = (S)op.bp_re, bpi = (S)op.bp_im;
+    const S dr = fma(bpr, M0r, -bpi * M0i), di = fma(bpr, M0i, bpi * M0r);
+    S Cf[3][3];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Cf[I][j] = Cx(I, j);
+    S wr[3], wi[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        wr[j] = fma(Tr[2], Cf[2][j], fma(Tr[1], Cf[1][j], Tr[0] * Cf[0][j]));
+        wi[j] = fma(Ti[2], Cf[2][j], fma(Ti[1], Cf[1][j], Ti[0] * Cf[0][j]));
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const S ur = fma(Rr[2], Cf[i][2], fma(Rr[1], Cf[i][1], Rr[0] * Cf[i][0]));
+        const S ui = fma(Ri[2], Cf[i][2], fma(Ri[1], Cf[i][1], Ri[0] * Cf[i][0]));
+        const S br = fma((S)0.2, Sr, fma((S)-0.4, ur, dr));
+        const S bi = fma((S)0.2, Si, fma((S)-0.4, ui, di));
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            loc[3 * i + j].x = (double)fma(Cf[i][j], M0r, fma((S)-0.5, wr[j], br));
+            loc[3 * i + j].y = (double)fma(Cf[i][j], M0i, fma((S)-0.5, wi[j], bi));
+        }
+    }
+}
+
 // G: type of the rule points and vertices (float when the caller keeps an
-// FP32 copy of the geometry, double otherwise).
+// FP32 copy of the geometry, double otherwise).  FAST: the entries divided by
+// alpha.
 template <int OP, int FAST, typename G = double>
 __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &rule_, const G *xp,
                                            const G *xa, const double *xc, double xs, const G *yp,
                                            const G *ya, const double *yc, double ys, double2 loc[9]) {
     if constexpr (OP == OPK_DL) {
         pair_dl<float, FAST, false, G>(op, rule_, xp, xa, xc, yp, ya, yc, loc, nullptr);
+        return;
+    }
+    if constexpr (OP == OPK_MW && FAST) {
+        pair_mw_rule<float, false, G>(op, xp, xa, xc, yp, ya, yc, loc, nullptr);
         return;
     }
     const FarRule<FAST> rule{rule_, op};
@@ -607,15 +723,14 @@ __device__ __forceinline__ void pair_mixed(const OpConst &op, const RuleConst &r
                 ti[j] += vi[I][j];
             }
         }
+        // divided by alpha (applied at the write-back)
         const float sc = 4.0f * (float)(xs * ys * (1.0 / 6.0));
-        const float ar = (float)op.alpha_re * sc, ai = (float)op.alpha_im * sc;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const float pr = fmaf(3.0f, vr[i][j], tr[j]), pi = fmaf(3.0f, vi[i][j], ti[j]);
-                loc[3 * i + j].x = (double)fmaf(ar, pr, -ai * pi);
-                loc[3 * i + j].y = (double)fmaf(ar, pi, ai * pr);
+                loc[3 * i + j].x = (double)(sc * fmaf(3.0f, vr[i][j], tr[j]));
+                loc[3 * i + j].y = (double)(sc * fmaf(3.0f, vi[i][j], ti[j]));
             }
     } else {
         float pr[9] = {}, pi[9] = {};
@@ -1610,7 +1725,14 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *
         double2 loc[9];
         pair_fp64<OP, FAST, true>(a.op, a.rule, xp, xa, xc, a.X.area[K], yp, ya, yc, a.Y.area[L], loc, s_sc);
 #pragma unroll
-        for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = loc[v];
+        for (int v = 0; v < 9; ++v) {
+            double2 z = loc[v];
+            if constexpr (FAST) {  // the compile-time-rule pair gives the entries divided by alpha
+                z.x = fma(a.op.alpha_re, loc[v].x, -a.op.alpha_im * loc[v].y);
+                z.y = fma(a.op.alpha_re, loc[v].y, a.op.alpha_im * loc[v].x);
+            }
+            a.spec_val[9 * (long long)e + v] = z;
+        }
     }
 }
 
@@ -1835,13 +1957,19 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
             for (int v = 0; v < 9; ++v) V[v * BMI_BLOCK_THREADS + t] = loc[v];
         }
 
-        // phase 2 (blocks next to the diagonal only)
+        // phase 2 (blocks next to the diagonal only); FAST: the pair values are
+        // divided by alpha, so are the special ones
         if (e1 > e0) {
             __syncthreads();
             for (int e = e0 + t; e < e1; e += BMI_BLOCK_THREADS) {
                 const int lo = a.spec_loc[e];
 #pragma unroll
-                for (int v = 0; v < 9; ++v) V[v * BMI_BLOCK_THREADS + lo] = a.spec_val[9 * (long long)e + v];
+                for (int v = 0; v < 9; ++v) {
+                    double2 z = a.spec_val[9 * (long long)e + v];
+                    if constexpr (FAST) z = make_double2(fma(a.op.ia_re, z.x, -a.op.ia_im * z.y),
+                                                         fma(a.op.ia_re, z.y, a.op.ia_im * z.x));
+                    V[v * BMI_BLOCK_THREADS + lo] = z;
+                }
             }
         }
         __syncthreads();
@@ -1884,7 +2012,12 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
             const int c = myc + 4 * j;
             if (c < nc) {
                 const long long off = (long long)(r0 + myr) + (long long)s_cfn[c] * a.ld;
-                out_write(Z, off, accr[j], acci[j], s_cfirst[c]);
+                double vr = accr[j], vi = acci[j];
+                if constexpr (FAST) {
+                    vr = fma(a.op.alpha_re, accr[j], -a.op.alpha_im * acci[j]);
+                    vi = fma(a.op.alpha_re, acci[j], a.op.alpha_im * accr[j]);
+                }
+                out_write(Z, off, vr, vi, s_cfirst[c]);
             }
         }
     }
@@ -2057,7 +2190,13 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
             const int o = s_ent[t];
             if (o >= 0) {
 #pragma unroll
-                for (int v = 0; v < 9; ++v) loc[v] = a.spec_val[9 * (long long)(e0 + o) + v];
+                for (int v = 0; v < 9; ++v) {
+                    const double2 z = a.spec_val[9 * (long long)(e0 + o) + v];
+                    // FAST: the tile holds the entries divided by alpha
+                    if constexpr (FAST) loc[v] = make_double2(fma(a.op.ia_re, z.x, -a.op.ia_im * z.y),
+                                                              fma(a.op.ia_re, z.y, a.op.ia_im * z.x));
+                    else loc[v] = z;
+                }
                 s_ent[t] = -1;
                 live = true;
             }
@@ -2194,7 +2333,10 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
             const int c = wv + k * (BMI_BLOCK_THREADS / 64);
             const int r = 64 * h + (lane ^ (c & 15));
             if (c < nc && r < nr) {
-                const double2 v = tile[c * PR + 64 * h + lane];
+                double2 v = tile[c * PR + 64 * h + lane];
+                if constexpr (FAST)  // the tile holds the entries divided by alpha
+                    v = make_double2(fma(a.op.alpha_re, v.x, -a.op.alpha_im * v.y),
+                                     fma(a.op.alpha_re, v.y, a.op.alpha_im * v.x));
                 OutT o;
                 o.x = (decltype(o.x))((double)old[NH * k + h].x + v.x);
                 o.y = (decltype(o.y))((double)old[NH * k + h].y + v.y);
