@@ -1086,10 +1086,9 @@ PanelArgs panel_args(const bmi_plan *P, void *Z, long long ld) {
     B.bscf = P->Y->d_bscf.p;
     B.rounds_t = P->X->rounds_t;
     B.fast = fast_rule(P);
-    // trial rounds by level: the Helmholtz kernels and the FP64 EFIE kernel
-    // with the compile-time rule; the others by rank (must match panel_block)
-    const bool by_level = P->op_kind == OPK_HH3D ||
-                          (B.fast && P->op_kind == OPK_MW && P->opts.precision != BMI_PREC_MIXED);
+    // trial rounds by level: the Helmholtz kernels and the kernels with the
+    // compile-time rule; the others by rank (must match panel_block)
+    const bool by_level = P->op_kind == OPK_HH3D || B.fast;
     B.rounds_b = by_level ? P->Y->levels_b : P->Y->rounds_b;
     B.brounds = P->Y->d_brounds.p + (by_level ? P->Y->npatch : 0);
     B.ncm = P->Y->ncmax;

@@ -2083,8 +2083,10 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     const int kl = t >> 4, ll = t & 15;
     const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
 
-    // trial rounds by level (bslot bits 16-23) or by rank (bits 8-15)
-    constexpr bool BY_LEVEL = OP == OPK_HH3D || (FAST && !MIXED && OP == OPK_MW);
+    // trial rounds by level (bslot bits 16-23) or by rank (bits 8-15): by level
+    // for the Helmholtz kernels and the compile-time-rule kernels, whose
+    // rounds read the three entries of one trial shape together
+    constexpr bool BY_LEVEL = OP == OPK_HH3D || FAST;
     constexpr int RSH = BY_LEVEL ? 16 : 8;
     const int lane = t & 63, wv = t >> 6;
     for (int c = wv; c < nc; c += BMI_BLOCK_THREADS / 64)
@@ -2222,46 +2224,21 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
         // the trial rounds need no barrier, as one wave's LDS accesses
         // are performed in program order.  More test ranks put colliding
         // rows in different waves: a barrier after each test round.
-        // Trial rounds by rank: within a round a thread's entries are
-        // distinct and no other thread touches them, so the mixed kernel
-        // (which has the registers) issues all of its reads before its
-        // writes: one LDS wait per round instead of one per entry; unused
-        // slots read entry 0 and write nothing.  Trial rounds by level (the
-        // Helmholtz kernels, the FP64 EFIE kernel with the fixed rule): two
-        // lanes may add to one entry in one round
-        // only through different local shapes j, and the updates go shape
-        // by shape in program order, each shape's reads after the previous
-        // shape's writes.
+        // Trial rounds by level (the Helmholtz kernels and the kernels with
+        // the compile-time rule): two lanes may add to one entry in one round
+        // only through different local shapes j, and the updates go shape by
+        // shape in program order, each shape's reads issued together (unused
+        // slots read entry 0 and write nothing) after the previous shape's
+        // writes.  Trial rounds by rank (the others): within a round a
+        // thread's entries are distinct and no other thread touches them;
+        // the updates go one at a time.
         const int nrb = a.brounds[q];
         for (int ra = 0; ra < a.rounds_t; ++ra) {
             bool tact[3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) tact[i] = live && (tsl[i] & 0xff) != 0xff && (tsl[i] >> 8) == ra;
             for (int rb = 0; rb < nrb; ++rb) {
-                if constexpr (MIXED && FAST && !BY_LEVEL) {
-                    int ix[9];
-                    bool act[9];
-                    double2 z[9];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-#pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            act[3 * i + j] = tact[i] && (bsl[j] & 0xff) != 0xff && ((bsl[j] >> 8) & 0xff) == rb;
-                            ix[3 * i + j] = act[3 * i + j] ? tile_ix<PR>(tsl[i] & 0xff, bsl[j] & 0xff) : 0;
-                            z[3 * i + j] = tile[ix[3 * i + j]];
-                        }
-#pragma unroll
-                    for (int i = 0; i < 3; ++i)
-#pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            if (!act[3 * i + j]) continue;
-                            const double cc = tcf[i] * bcf[j];
-                            double2 v = z[3 * i + j];
-                            v.x = fma(cc, loc[3 * i + j].x, v.x);
-                            v.y = fma(cc, loc[3 * i + j].y, v.y);
-                            tile[ix[3 * i + j]] = v;
-                        }
-                } else if constexpr (FAST && BY_LEVEL) {
+                if constexpr (FAST && BY_LEVEL) {
                     // by level: the three reads of one shape together
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
