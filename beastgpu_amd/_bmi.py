@@ -31,7 +31,7 @@ EXPORTS = (
     "bmi_matrix_apply", "bmi_matrix_apply_noalloc", "bmi_matrix_compare", "bmi_device_memory",
     "bmi_server_create", "bmi_server_destroy", "bmi_server_submit", "bmi_server_wait",
     "bmi_server_release", "bmi_server_memory", "bmi_request_bytes",
-    "bmi_batch_create", "bmi_batch_assemble", "bmi_batch_stats_get", "bmi_batch_free",
+    "bmi_batch_create", "bmi_batch_assemble", "bmi_batch_stats_get", "bmi_batch_set_schedule", "bmi_batch_free",
 )
 BATCH_MAX_BUCKETS = 8
 REQ_ASSEMBLE, REQ_ASSEMBLE_APPLY, REQ_APPLY = 0, 1, 2
@@ -155,15 +155,18 @@ class BatchStats(ctypes.Structure):
         ("bucket_mode", ctypes.c_int32 * 8),
         ("bucket_wgs_per_cu", ctypes.c_int32 * 8),
         ("bucket_tuned_ms", ctypes.c_double * 8),
+        ("bucket_candidate_ms", (ctypes.c_double * 4) * 8),
     ]
 
     def as_dict(self) -> dict:
         nb = self.buckets
         return {"last_ms": self.last_ms, "problems": self.problems, "buckets": nb, "singles": self.singles,
                 "bucket_problems": list(self.bucket_problems[:nb]),
-                "bucket_mode": ["queue" if m == 1 else "per-colour" for m in self.bucket_mode[:nb]],
+                "bucket_mode": ["persistent" if m == 1 else "per-colour" for m in self.bucket_mode[:nb]],
                 "bucket_wgs_per_cu": list(self.bucket_wgs_per_cu[:nb]),
-                "bucket_tuned_ms": list(self.bucket_tuned_ms[:nb])}
+                "bucket_tuned_ms": list(self.bucket_tuned_ms[:nb]),
+                # [per-colour, persistent 1, 2, 3 work-groups per CU]; 0: not timed
+                "bucket_candidate_ms": [list(self.bucket_candidate_ms[b]) for b in range(nb)]}
 
 
 _lib = None
@@ -215,6 +218,7 @@ def lib():
     L.bmi_batch_create.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i32, pp]
     L.bmi_batch_assemble.argtypes = [vp]
     L.bmi_batch_stats_get.argtypes = [vp, ctypes.POINTER(BatchStats)]
+    L.bmi_batch_set_schedule.argtypes = [vp, i32, i32, i32]
     L.bmi_batch_free.argtypes = [vp]
     for name in EXPORTS:
         f = getattr(L, name)

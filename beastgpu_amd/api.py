@@ -337,6 +337,13 @@ class Batch(_Handle):
         _bmi.check(_bmi.lib().bmi_batch_stats_get(self.h, ctypes.byref(s)))
         return s.as_dict()
 
+    def set_schedule(self, bucket: int, mode: str, wgs_per_cu: int = 0) -> None:
+        """Override the tuned far-phase schedule of one bucket: "per-colour",
+        or "persistent" with 1 .. 3 work-groups per CU."""
+        if mode not in ("per-colour", "persistent"):
+            raise ValueError("mode must be 'per-colour' or 'persistent'")
+        _bmi.check(_bmi.lib().bmi_batch_set_schedule(self.h, bucket, int(mode == "persistent"), wgs_per_cu))
+
 
 def assemble_gpu(operator: Operator, test_functions: Space, trial_functions: Space,
                  config: GPUConfiguration | None = None, writeBackStrategy: str | None = None) -> np.ndarray:

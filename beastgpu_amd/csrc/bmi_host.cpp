@@ -1096,6 +1096,7 @@ PanelArgs panel_args(const bmi_plan *P, void *Z, long long ld) {
     // kernels handle
     B.rr = P->X->prows != BMI_PANEL_ROWS || P->X->rounds_t * B.rounds_b <= 6;
     B.npb = P->npb;
+    B.bq_all = P->Y->d_by_colour.p;
     B.blk_off = P->d_blk_off.p;
     B.blk_cnt = P->d_blk_cnt.p;
     B.spec_loc = P->d_spec_loc.p;
@@ -1958,18 +1959,26 @@ struct BatchBucket {
     std::vector<int> problems;
     int ncm = 1;
     int nitems = 0;
-    int npanels = 0;                // global panel ids (done counters)
     std::vector<int> colour_off;    // items of colour c: [colour_off[c], colour_off[c + 1])
     DevBuf<PanelArgs> d_args;
-    DevBuf<int4> d_items;           // (problem in bucket, panel, patch, global panel), colour-major
-    DevBuf<int> d_need;
-    DevBuf<int> d_state;            // [0] queue head, [1] error flag, [2..] done counters
+    DevBuf<int4> d_items;           // (problem in bucket, panel, patch, -), colour-major
+    int npitems = 0;
+    DevBuf<int2> d_pitems;          // (problem in bucket, panel), largest first: the persistent queue
+    DevBuf<int> d_head;             // the persistent queue's head
     int mode = 0, wpc = 0;
     double tuned_ms = 0.0;
+    double cand_ms[BMI_BATCH_CANDIDATES] = {};  // per-colour, persistent with 1, 2, 3 work-groups per CU
+};
+
+// A bucket signature's tuned schedule, kept for the process.
+struct TunedSchedule {
+    int mode = 0, wpc = 0;
+    double tuned_ms = 0.0;
+    double cand_ms[BMI_BATCH_CANDIDATES] = {};
 };
 
 std::mutex g_tune_mu;
-std::vector<std::pair<std::string, std::pair<int, int>>> g_tune_cache;
+std::vector<std::pair<std::string, TunedSchedule>> g_tune_cache;
 
 }  // namespace
 
@@ -2009,29 +2018,20 @@ struct bmi_batch {
 
 namespace {
 
-constexpr int kSpinLimit = 1 << 22;
-
-// The far phase of one bucket on stream s with the bucket's schedule.
+// The far phase of one bucket on stream s with the bucket's schedule: one
+// launch per trial colour (mode 0), or one persistent panel-major launch of
+// wpc work-groups per CU (mode 1).
 bmi_status enqueue_bucket(BatchBucket &B, int mode, int wpc, int cus, hipStream_t s) {
     if (mode == 1) {
-        BMI_HIP(launch_zero_ints(B.d_state.p, 2 + B.npanels, s));
-        BMI_HIP(launch_panels_queue(B.d_args.p, B.d_items.p, B.d_need.p, B.nitems, B.ncm, B.d_state.p,
-                                    B.d_state.p + 2, B.d_state.p + 1, kSpinLimit, cus * wpc, B.op_kind, B.mixed,
-                                    B.fast, B.c32, s));
+        BMI_HIP(launch_zero_ints(B.d_head.p, 1, s));
+        BMI_HIP(launch_panels_persist(B.d_args.p, B.d_pitems.p, B.npitems, B.ncm, B.d_head.p, cus * wpc, B.op_kind,
+                                      B.mixed, B.fast, B.c32, s));
     } else {
         for (size_t c = 0; c + 1 < B.colour_off.size(); ++c)
             BMI_HIP(launch_panels_items(B.d_args.p, B.d_items.p + B.colour_off[c],
                                         B.colour_off[c + 1] - B.colour_off[c], B.ncm, B.op_kind, B.mixed, B.fast,
                                         B.c32, s));
     }
-    return BMI_OK;
-}
-
-bmi_status check_queue_error(BatchBucket &B, hipStream_t s) {
-    int err = 0;
-    BMI_HIP(hipMemcpyAsync(&err, B.d_state.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    BMI_HIP(hipStreamSynchronize(s));
-    if (err) return fail(BMI_EDEVICE, "a far-pair queue wait exceeded its spin limit");
     return BMI_OK;
 }
 
@@ -2050,79 +2050,96 @@ bmi_status build_bucket(bmi_batch *Bt, BatchBucket &B) {
         return (int64_t)pa->X->npanel * pa->npb > (int64_t)pb->X->npanel * pb->npb;
     });
     std::vector<PanelArgs> args;
-    std::vector<int> pbase;
     int maxcol = 0;
     for (int p : B.problems) {
         const bmi_plan *P = Bt->plans[p];
         args.push_back(panel_args(P, Bt->mats[p]));
-        pbase.push_back(B.npanels);
-        B.npanels += P->X->npanel;
         B.ncm = std::max(B.ncm, P->Y->ncmax);
         maxcol = std::max(maxcol, P->Y->ncolour);
     }
     std::vector<int4> items;
-    std::vector<int> need;
     B.colour_off.assign(1, 0);
     for (int c = 0; c < maxcol; ++c) {
         for (size_t k = 0; k < B.problems.size(); ++k) {
             const bmi_plan *P = Bt->plans[B.problems[k]];
             if (c >= P->Y->ncolour) continue;
-            const int lower = P->Y->colour_off[c];  // patches of colours < c
             const std::vector<int> &qs = P->Y->by_colour[c];
             for (int pan = 0; pan < P->X->npanel; ++pan)
-                for (int q : qs) {
-                    items.push_back(make_int4((int)k, pan, q, pbase[k] + pan));
-                    need.push_back(lower);
-                }
+                for (int q : qs) items.push_back(make_int4((int)k, pan, q, 0));
         }
         B.colour_off.push_back((int)items.size());
     }
     B.nitems = (int)items.size();
+    // persistent queue: one item per (problem, test panel), by decreasing
+    // work (sub-batches of the panel times trial patches), ties in order
+    std::vector<std::pair<int64_t, int2>> pq;
+    for (size_t k = 0; k < B.problems.size(); ++k) {
+        const bmi_plan *P = Bt->plans[B.problems[k]];
+        for (int pan = 0; pan < P->X->npanel; ++pan) {
+            const int64_t w = (int64_t)(P->X->sb_off[pan + 1] - P->X->sb_off[pan]) * P->Y->npatch;
+            pq.push_back({w, make_int2((int)k, pan)});
+        }
+    }
+    std::stable_sort(pq.begin(), pq.end(), [](const std::pair<int64_t, int2> &a, const std::pair<int64_t, int2> &b) {
+        return a.first > b.first;
+    });
+    std::vector<int2> pitems;
+    for (auto &v : pq) pitems.push_back(v.second);
+    B.npitems = (int)pitems.size();
     hipError_t e;
     if ((e = B.d_args.upload(args)) != hipSuccess || (e = B.d_items.upload(items)) != hipSuccess ||
-        (e = B.d_need.upload(need)) != hipSuccess || (e = B.d_state.alloc(2 + (size_t)B.npanels)) != hipSuccess)
+        (e = B.d_pitems.upload(pitems)) != hipSuccess || (e = B.d_head.alloc(1)) != hipSuccess)
         return fail(BMI_ENOMEM, std::string("batch buffers: ") + hipGetErrorString(e));
     return BMI_OK;
 }
 
 // Measures the candidate schedules of a bucket (the prologues have run) and
 // keeps the fastest; a bucket signature seen before reuses its choice.
+// Work-groups per CU that the far kernels' launch bounds let reside at once.
+int max_wgs_per_cu(const BatchBucket &B) { return (B.op_kind == OPK_DL && !B.mixed) ? 2 : 3; }
+
 bmi_status tune_bucket(bmi_batch *Bt, BatchBucket &B) {
     const std::string sig = bucket_signature(B);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         for (auto &kv : g_tune_cache)
             if (kv.first == sig) {
-                B.mode = kv.second.first;
-                B.wpc = kv.second.second;
+                const TunedSchedule &t = kv.second;
+                B.mode = t.mode;
+                B.wpc = t.wpc;
+                B.tuned_ms = t.tuned_ms;
+                std::copy(t.cand_ms, t.cand_ms + BMI_BATCH_CANDIDATES, B.cand_ms);
                 return BMI_OK;
             }
     }
     hipStream_t s = Bt->ctx->stream;
-    const int maxwpc = (B.op_kind == OPK_DL && !B.mixed) ? 2 : 3;
-    std::vector<std::pair<int, int>> cand = {{0, 0}};
-    for (int w = 1; w <= maxwpc; ++w) cand.push_back({1, w});
+    const int maxwpc = max_wgs_per_cu(B);
     double best = 1e300;
-    for (auto &c : cand) {
+    for (int c = 0; c <= maxwpc; ++c) {  // c = 0: per-colour, else persistent with c work-groups per CU
         float ms = 0.0f;
         for (int rep = 0; rep < 2; ++rep) {
             BMI_HIP(hipEventRecord(Bt->ev0, s));
-            bmi_status st = enqueue_bucket(B, c.first, c.second, Bt->cus, s);
+            bmi_status st = enqueue_bucket(B, c > 0, c, Bt->cus, s);
             if (st != BMI_OK) return st;
             BMI_HIP(hipEventRecord(Bt->ev1, s));
             BMI_HIP(hipStreamSynchronize(s));
-            if (c.first == 1 && (st = check_queue_error(B, s)) != BMI_OK) return st;
             BMI_HIP(hipEventElapsedTime(&ms, Bt->ev0, Bt->ev1));
         }
+        B.cand_ms[c] = ms;
         if (ms < best) {
             best = ms;
-            B.mode = c.first;
-            B.wpc = c.second;
+            B.mode = c > 0;
+            B.wpc = c;
             B.tuned_ms = ms;
         }
     }
+    TunedSchedule t;
+    t.mode = B.mode;
+    t.wpc = B.wpc;
+    t.tuned_ms = B.tuned_ms;
+    std::copy(B.cand_ms, B.cand_ms + BMI_BATCH_CANDIDATES, t.cand_ms);
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache.push_back({sig, {B.mode, B.wpc}});
+    g_tune_cache.push_back({sig, t});
     return BMI_OK;
 }
 
@@ -2364,11 +2381,6 @@ bmi_status bmi_batch_assemble(bmi_batch *Bt) try {
     float ms = 0.0f;
     BMI_HIP(hipEventElapsedTime(&ms, Bt->ev0, Bt->ev1));
     Bt->last_ms = ms;
-    for (auto &B : Bt->buckets)
-        if (B.mode == 1) {
-            const bmi_status st = check_queue_error(B, s);
-            if (st != BMI_OK) return st;
-        }
     return BMI_OK;
 } catch (...) {
     return fail_exception();
@@ -2386,8 +2398,30 @@ bmi_status bmi_batch_stats_get(const bmi_batch *Bt, bmi_batch_stats *st) try {
         st->bucket_mode[b] = Bt->buckets[b].mode;
         st->bucket_wgs_per_cu[b] = Bt->buckets[b].wpc;
         st->bucket_tuned_ms[b] = Bt->buckets[b].tuned_ms;
+        for (int c = 0; c < BMI_BATCH_CANDIDATES; ++c) st->bucket_candidate_ms[b][c] = Bt->buckets[b].cand_ms[c];
     }
     return BMI_OK;
+} catch (...) {
+    return fail_exception();
+}
+
+bmi_status bmi_batch_set_schedule(bmi_batch *Bt, int32_t bucket, int32_t mode, int32_t wgs_per_cu) try {
+    if (!Bt) return fail(BMI_EINVAL, "NULL argument");
+    if (bucket < 0 || bucket >= (int)Bt->buckets.size()) return fail(BMI_EINVAL, "no such bucket");
+    BatchBucket &B = Bt->buckets[bucket];
+    if (mode == 0) wgs_per_cu = 0;
+    else if (mode != 1 || wgs_per_cu < 1 || wgs_per_cu > max_wgs_per_cu(B))
+        return fail(BMI_EINVAL, "mode 0, or mode 1 with 1 .. " + std::to_string(max_wgs_per_cu(B)) +
+                                    " work-groups per CU for this bucket");
+    BMI_HIP(hipSetDevice(Bt->device));
+    BMI_HIP(hipStreamSynchronize(Bt->ctx->stream));
+    if (Bt->exec) BMI_HIP(hipGraphExecDestroy(Bt->exec));
+    Bt->exec = nullptr;
+    if (Bt->graph) BMI_HIP(hipGraphDestroy(Bt->graph));
+    Bt->graph = nullptr;
+    B.mode = mode;
+    B.wpc = wgs_per_cu;
+    return capture_batch(Bt);
 } catch (...) {
     return fail_exception();
 }

@@ -125,6 +125,7 @@ struct PanelArgs {
     int fast;                    // 1: default 3 x 4 point far rule and a lossless operator (compile-time rule)
     const int *bq_list;          // trial patches of this launch's colour
     int nbq;
+    const int *bq_all;           // every trial patch, colour by colour
     int npb;                     // trial patches in total (block id = sub-batch * npb + q)
     const int *blk_off;          // special-pair entries of each block: first ...
     const int *blk_cnt;          // ... and count
@@ -189,9 +190,10 @@ hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hi
 // batched far-pair launches over items (problem, panel, patch, global panel)
 hipError_t launch_panels_items(const PanelArgs *args, const int4 *items, int nitems, int ncm, int op_kind, int mixed,
                                int fast, int c32, hipStream_t s);
-hipError_t launch_panels_queue(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm,
-                               int *head, int *done, int *err, int spin_limit, int grid, int op_kind, int mixed,
-                               int fast, int c32, hipStream_t s);
+// persistent, panel-major: items (problem, test panel) popped through *head
+// (zeroed before the launch), each against every trial patch in colour order
+hipError_t launch_panels_persist(const PanelArgs *args, const int2 *items, int nitems, int ncm, int *head, int grid,
+                                 int op_kind, int mixed, int fast, int c32, hipStream_t s);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 // column chunks of the apply kernels (and the chunk width); the partial

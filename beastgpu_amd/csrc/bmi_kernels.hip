@@ -2347,50 +2347,35 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
     panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, tile, sh);
 }
 
-// Batched, persistent: the work-groups pop items (problem, panel, patch,
-// global panel id) from one queue ordered colour by colour.  An item whose
-// panel has blocks of lower colours waits until those are done (their columns
-// are stored first and then read-modify-written): need[i] blocks of its panel
-// must have been counted in done[global panel id].  Every such block was
-// popped earlier by a running work-group that waits only on blocks popped
-// before it, so the queue always drains; the wait is bounded by spin_limit
-// all the same and a timeout sets *err instead of hanging.
+// Batched, persistent and panel-major: the work-groups pop items (problem,
+// test panel) from one queue, largest first, and each integrates its panel
+// against every trial patch of its problem in colour order -- the order of
+// the per-colour launches, so the values are the same bitwise.  A column
+// shared by two patches is stored by the first of the two blocks and
+// read-modify-written by the later one, both of this work-group, so the only
+// ordering needed is the work-group barrier (with its workgroup-scope
+// fences) between blocks: no work-group waits for another and no
+// device-scope fence is needed.
 template <int OP, int MIXED, int FAST, typename OutT>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_queue(
-    const PanelArgs *args, const int4 *items, const int *need, int nitems, int *head, int *done, int *err,
-    int spin_limit) {
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_persist(
+    const PanelArgs *args, const int2 *items, int nitems, int *head) {
     extern __shared__ double2 tile[];
     __shared__ PanelShared sh;
     __shared__ int s_item;
     const int t = threadIdx.x;
     if (!MIXED) fill_sincos_table(sh.s_sc);
     for (;;) {
-        __syncthreads();  // the previous block is done with the LDS and s_item
+        __syncthreads();  // the previous block is done with the LDS and s_item; its stores are visible here
         if (t == 0) s_item = atomicAdd(head, 1);
         __syncthreads();
         const int i = s_item;
         if (i >= nitems) break;
-        const int4 it = items[i];
-        const int nd = need[i];
-        if (nd > 0) {
-            if (t == 0) {
-                int spins = 0;
-                while (__hip_atomic_load(&done[it.w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < nd) {
-                    __builtin_amdgcn_s_sleep(2);
-                    if (++spins > spin_limit) {
-                        atomicOr(err, 1);
-                        break;
-                    }
-                }
-            }
-            __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int2 it = items[i];
+        const PanelArgs &a = args[it.x];
+        for (int u = 0; u < a.npb; ++u) {
+            if (u > 0) __syncthreads();
+            panel_block<OP, MIXED, FAST, OutT>(a, it.y, a.bq_all[u], tile, sh);
         }
-        panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, tile, sh);
-        // publish this block's stores before counting it
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (t == 0) __hip_atomic_fetch_add(&done[it.w], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -2446,11 +2431,10 @@ void items_t(const PanelArgs *args, const int4 *items, int nitems, int ncm, hipS
 }
 
 template <int OP, int MIXED, int FAST, typename OutT>
-void queue_t(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm, int *head, int *done,
-             int *err, int spin_limit, int grid, hipStream_t s) {
+void persist_t(const PanelArgs *args, const int2 *items, int nitems, int ncm, int *head, int grid, hipStream_t s) {
     const size_t lds = sizeof(double2) * panel_rows(OP) * ncm;
-    hipLaunchKernelGGL((k_panels_queue<OP, MIXED, FAST, OutT>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, args,
-                       items, need, nitems, head, done, err, spin_limit);
+    hipLaunchKernelGGL((k_panels_persist<OP, MIXED, FAST, OutT>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, args,
+                       items, nitems, head);
 }
 
 // f(op, mixed, fast, c32) -> the instantiation
@@ -2476,10 +2460,10 @@ struct ItemsK {
 };
 
 template <int OP, int MIXED, int FAST, typename OutT>
-struct QueueK {
-    static void run(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm, int *head,
-                    int *done, int *err, int spin_limit, int grid, hipStream_t s) {
-        queue_t<OP, MIXED, FAST, OutT>(args, items, need, nitems, ncm, head, done, err, spin_limit, grid, s);
+struct PersistK {
+    static void run(const PanelArgs *args, const int2 *items, int nitems, int ncm, int *head, int grid,
+                    hipStream_t s) {
+        persist_t<OP, MIXED, FAST, OutT>(args, items, nitems, ncm, head, grid, s);
     }
 };
 
@@ -2506,11 +2490,10 @@ hipError_t launch_panels_items(const PanelArgs *args, const int4 *items, int nit
     return hipGetLastError();
 }
 
-hipError_t launch_panels_queue(const PanelArgs *args, const int4 *items, const int *need, int nitems, int ncm,
-                               int *head, int *done, int *err, int spin_limit, int grid, int op_kind, int mixed,
-                               int fast, int c32, hipStream_t s) {
+hipError_t launch_panels_persist(const PanelArgs *args, const int2 *items, int nitems, int ncm, int *head, int grid,
+                                 int op_kind, int mixed, int fast, int c32, hipStream_t s) {
     if (nitems <= 0) return hipSuccess;
-    dispatch4<QueueK>(op_kind, mixed, fast, c32, args, items, need, nitems, ncm, head, done, err, spin_limit, grid, s);
+    dispatch4<PersistK>(op_kind, mixed, fast, c32, args, items, nitems, ncm, head, grid, s);
     return hipGetLastError();
 }
 

@@ -61,6 +61,26 @@ def test_batch_bitwise_equals_single_assemblies_on_every_replay(gpu_ctx):
         assert b.stats()["last_ms"] > 0
         for i, (a, r) in enumerate(zip(asms, ref)):
             assert np.array_equal(a.matrix.to_host(), r), (rep, i)
+    # every schedule of every bucket: persistent with 1 and with the most
+    # work-groups per CU, then per-colour
+    for bk in range(st["buckets"]):
+        for mode, w in (("persistent", 1), ("persistent", 3), ("per-colour", 0)):
+            try:
+                b.set_schedule(bk, mode, w)
+            except _bmi.BMIError as e:  # 3 work-groups per CU beyond the FP64 double layer's occupancy
+                assert e.status == _bmi.EINVAL and w == 3
+                b.set_schedule(bk, mode, 2)
+            for a, o in zip(asms, others):
+                _bmi.check(_bmi.lib().bmi_assemble(o.plan.h, a.matrix.h))
+            b.assemble()
+            s2 = b.stats()
+            assert s2["bucket_mode"][bk] == mode
+            for i, (a, r) in enumerate(zip(asms, ref)):
+                assert np.array_equal(a.matrix.to_host(), r), (bk, mode, w, i)
+    with pytest.raises(_bmi.BMIError):
+        b.set_schedule(st["buckets"], "per-colour")
+    with pytest.raises(_bmi.BMIError):
+        b.set_schedule(0, "persistent", 0)
     b.close()
     for a in asms + others:
         a.close()
