@@ -1574,6 +1574,8 @@ static bmi_status plan_create(bmi_context *ctx, const bmi_operator *op, const bm
     S.spec_val = P->d_spec_val.p;
     S.op = P->opc;
     S.rule = P->rule;
+    S.sym = P->X == P->Y && (P->op_kind == OPK_MW || P->op_kind == OPK_HH3D);
+    S.cf_zero = P->op_kind == OPK_DL;
     for (int g = 0; g < 3; ++g) {
         if (rules) {
             S.ss_n[g] = rules->gl_points[g];

@@ -168,6 +168,14 @@ struct SpecialArgs {
     double glx[3][BMI_MAX_GL], glw[3][BMI_MAX_GL];
     int near_np;
     double nb1[BMI_MAX_NEAR], nb2[BMI_MAX_NEAR], nw[BMI_MAX_NEAR];
+    // sym: a symmetric kernel (single layers) on one space as test and trial,
+    // so the pair (L, K) has the transposed block of (K, L): common-vertex
+    // pairs with K > L are copied from (L, K) transposed, and the common-face
+    // regions that are each other's x <-> y swap are evaluated once
+    int sym;
+    // cf_zero: common-face blocks are zero (the double layer on a flat cell:
+    // n.(x - y) and n.(y - c) vanish) and are not integrated
+    int cf_zero;
 };
 
 // kernel launchers (bmi_kernels.hip)

@@ -1102,8 +1102,12 @@ struct CellG {
 };
 
 // SMOOTH: the smooth remainder of G (near pairs of the single layers) in
-// place of G.
-template <int OP, bool SMOOTH = false>
+// place of G.  PAIR (single layers on one cell, the charts equal): the node
+// stands for itself and for its x <-> y swap, whose kernel value is the same
+// (R is symmetric), so h is counted for both orders: the moments of x and of
+// y each gather x + y and x.y is counted twice (Maxwell), the shape products
+// f_i g_j + g_i f_j (Helmholtz).
+template <int OP, bool SMOOTH = false, bool PAIR = false>
 __device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const double2 *tab, SpecAcc<OP> &acc, const CellG &X, const CellG &Y,
                                            const double *dc, double bx1, double bx2, double by1, double by2,
                                            double w, const double *nrm) {
@@ -1157,6 +1161,20 @@ __device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const 
         green64<true>(op, lossy, w * (1.0 / (4.0 * M_PI)), dc[0] + x[0] - y[0], dc[1] + x[1] - y[1],
                       dc[2] + x[2] - y[2], hr, hi, tab);
     if (OP == OPK_MW) {
+        if constexpr (PAIR) {
+            const double xy = x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+            acc.v[0] = fma(2.0, hr, acc.v[0]);
+            acc.v[1] = fma(2.0, hi, acc.v[1]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double sd = x[d] + y[d];
+                acc.v[2 + d] = fma(hr, sd, acc.v[2 + d]);
+                acc.v[5 + d] = fma(hi, sd, acc.v[5 + d]);
+            }
+            acc.v[14] = fma(2.0 * hr, xy, acc.v[14]);
+            acc.v[15] = fma(2.0 * hi, xy, acc.v[15]);
+            return;  // the moments of y equal those of x: copied after the reduction
+        }
         acc.v[0] += hr;
         acc.v[1] += hi;
 #pragma unroll
@@ -1177,8 +1195,14 @@ __device__ __forceinline__ void spec_point(const OpConst &op, bool lossy, const 
             const double fr = hr * f[i], fi = hi * f[i];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                acc.v[2 * (3 * i + j)] = fma(fr, g[j], acc.v[2 * (3 * i + j)]);
-                acc.v[2 * (3 * i + j) + 1] = fma(fi, g[j], acc.v[2 * (3 * i + j) + 1]);
+                if constexpr (PAIR) {
+                    const double sij = fma(f[i], g[j], g[i] * f[j]);
+                    acc.v[2 * (3 * i + j)] = fma(hr, sij, acc.v[2 * (3 * i + j)]);
+                    acc.v[2 * (3 * i + j) + 1] = fma(hi, sij, acc.v[2 * (3 * i + j) + 1]);
+                } else {
+                    acc.v[2 * (3 * i + j)] = fma(fr, g[j], acc.v[2 * (3 * i + j)]);
+                    acc.v[2 * (3 * i + j) + 1] = fma(fi, g[j], acc.v[2 * (3 * i + j) + 1]);
+                }
             }
         }
     }
@@ -1426,6 +1450,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
         if (t < cs && ec < ns) {
             cl = a.spec_cls[ec];
             take = !(cl & SPEC_NOT_OWNER) && (NEARK ? cl == CLS_NEAR : (cl >= CLS_CV && cl <= CLS_CF));
+            // blocks that k_link fills: zero common faces, mirrored common vertices
+            if (!NEARK && take && cl == CLS_CF && a.cf_zero) take = 0;
+            if (!NEARK && take && cl == CLS_CV && a.sym && a.spec_K[ec] > a.spec_L[ec]) take = 0;
         }
         int nitem = 0;
 #pragma unroll
@@ -1545,8 +1572,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             // Each (class, region) loop is its own code, the region map
             // folded in at compile time (no per-node dispatch).
             constexpr int U = OP == OPK_HH3D ? NH3 : 1;
-            auto region = [&](auto cls_c, auto r_c) {
+            auto region = [&](auto cls_c, auto r_c, auto pair_c) {
                 constexpr int C = decltype(cls_c)::value, RR = decltype(r_c)::value;
+                constexpr bool PR = decltype(pair_c)::value;
 #pragma unroll 1
                 for (int i1 = 0; i1 < NG; ++i1) {
                     const double e1 = s_glx[0][i1], w1 = s_glw[0][i1];
@@ -1557,7 +1585,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                             const double4 nd = s_node[lane + 64 * (h0 + h2)];
                             double u1, u2, v1, v2, m;
                             ss_map(C, RR, e1, nd.x, nd.y, nd.z, w1 * nd.w, u1, u2, v1, v2, m);
-                            spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
+                            spec_point<OP, false, PR>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
                         }
                     }
                 }
@@ -1571,30 +1599,40 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             using CCV = std::integral_constant<int, CLS_CV>;
             using CCE = std::integral_constant<int, CLS_CE>;
             using CCF = std::integral_constant<int, CLS_CF>;
+            using NP = std::false_type;
             if (cls == CLS_CV) {
-                region(CCV{}, I0{});
-                region(CCV{}, I1{});
+                region(CCV{}, I0{}, NP{});
+                region(CCV{}, I1{}, NP{});
             } else if (cls == CLS_CE) {
-                region(CCE{}, I0{});
-                region(CCE{}, I1{});
-                region(CCE{}, I2{});
-                region(CCE{}, I3{});
-                region(CCE{}, I4{});
+                region(CCE{}, I0{}, NP{});
+                region(CCE{}, I1{}, NP{});
+                region(CCE{}, I2{}, NP{});
+                region(CCE{}, I3{}, NP{});
+                region(CCE{}, I4{}, NP{});
+            } else if (OP != OPK_DL && a.sym) {
+                // regions 2m + 1 are the x <-> y swaps of regions 2m
+                using PP = std::true_type;
+                region(CCF{}, I0{}, PP{});
+                region(CCF{}, I2{}, PP{});
+                region(CCF{}, I4{}, PP{});
             } else {
-                region(CCF{}, I0{});
-                region(CCF{}, I1{});
-                region(CCF{}, I2{});
-                region(CCF{}, I3{});
-                region(CCF{}, I4{});
-                region(CCF{}, I5{});
+                region(CCF{}, I0{}, NP{});
+                region(CCF{}, I1{}, NP{});
+                region(CCF{}, I2{}, NP{});
+                region(CCF{}, I3{}, NP{});
+                region(CCF{}, I4{}, NP{});
+                region(CCF{}, I5{}, NP{});
             }
         } else {
             const int g = cls - 1;
             const int n = a.ss_n[g];
             const int n2 = n * n, n4 = n2 * n2;
             const int nreg = cls == CLS_CV ? 2 : (cls == CLS_CE ? 5 : 6);
+            // common face of a symmetric kernel: regions 2m + 1 are the swaps
+            // of regions 2m, evaluated with them
+            const bool pair = OP != OPK_DL && cls == CLS_CF && a.sym;
             // region loop outermost: the region map is uniform over the wave
-            for (int r = 0; r < nreg; ++r) {
+            for (int r = 0; r < nreg; r += pair ? 2 : 1) {
                 // U nodes per lane and iteration, so that independent
                 // evaluations overlap their latencies (the double layer's
                 // registers allow one); past the last node the extra ones
@@ -1612,7 +1650,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                                            s_glw[g][i3] * s_glw[g][i4];
                         double u1, u2, v1, v2, m;
                         ss_map(cls, r, s_glx[g][i1], s_glx[g][i2], s_glx[g][i3], s_glx[g][i4], wgt, u1, u2, v1, v2, m);
-                        spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
+                        if (pair) spec_point<OP, false, true>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
+                        else spec_point<OP>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
                     }
                 }
             }
@@ -1627,6 +1666,10 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
         for (int o = 32; o >= 1; o >>= 1) {
 #pragma unroll
             for (int i = 0; i < spec_nacc(OP); ++i) acc.v[i] += __shfl_xor(acc.v[i], o);
+        }
+        if (OP == OPK_MW && !NEARK && cls == CLS_CF && a.sym) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) acc.v[8 + i] = acc.v[2 + i];  // paired regions: moments of y = those of x
         }
         const double *red = acc.v;
         if (lane < 9) {
@@ -1695,16 +1738,36 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
 template <int OP, int FAST>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *args, const int *ebase, int nprob) {
     __shared__ double2 s_sc[64];
+    __shared__ int s_items[BMI_BLOCK_THREADS];
+    __shared__ int s_wsum[BMI_BLOCK_THREADS / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     fill_sincos_table(s_sc);
+    // the work-group's entries of class 5 (owners) packed to its first
+    // threads, so that the waves past them exit at once
+    const int u0 = blockIdx.x * BMI_BLOCK_THREADS + t;
+    bool mine = false;
+    if (u0 < ebase[nprob]) {
+        const int p0 = find_problem(ebase, nprob, u0);
+        const int e0 = u0 - ebase[p0];
+        mine = e0 < min(*args[p0].n_special, args[p0].capacity) && args[p0].spec_cls[e0] == CLS_FAR64;
+    }
+    const unsigned long long m = __ballot(mine);
+    if (lane == 0) s_wsum[wv] = __popcll(m);
     __syncthreads();
-    const int u = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
-    if (u >= ebase[nprob]) return;
+    int base = 0, n = 0;
+#pragma unroll
+    for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
+        base += i < wv ? s_wsum[i] : 0;
+        n += s_wsum[i];
+    }
+    if (mine) s_items[base + __popcll(m & ((1ull << lane) - 1ull))] = u0;
+    __syncthreads();
+    if (t >= n) return;
+    const int u = s_items[t];
     const int pi = find_problem(ebase, nprob, u);
     const SpecialArgs &a = args[pi];
-    const int ns = min(*a.n_special, a.capacity);
     {
         const int e = u - ebase[pi];
-        if (e >= ns || a.spec_cls[e] != CLS_FAR64) return;  // owner entries of class 5 only
         const int K = a.spec_K[e], L = a.spec_L[e];
         double xp[3 * BMI_MAX_RULE], yp[3 * BMI_MAX_RULE], xa[9], ya[9], xc[3], yc[3];
 #pragma unroll
@@ -1736,43 +1799,60 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *
     }
 }
 
-// One thread per non-owner entry: the same (test cell, trial cell) pair sits
-// in the block (home sub-batch of the test cell, patch of the trial cell);
-// copy its values.  A missing owner (which the classifier rules out: both
-// blocks classify the same pair with the same arithmetic) leaves NaN.
+// One thread per entry that k_special and k_far64 leave: zero common faces
+// (cf_zero), common-vertex entries (K, L) with K > L of a symmetric problem
+// (the transpose of the block of (L, K)), and non-owner entries, whose pair
+// sits in the block (home sub-batch of the test cell, patch of the trial
+// cell).  The source entry is found in its block by binary search: a block's
+// entries are in the order of their pair's thread kl * 16 + ll, kl the test
+// cell's place in its sub-batch and ll the trial cell's place in its patch.
+// A missing source (which the classifier rules out: both blocks classify the
+// same pair with the same arithmetic, and the vertex test is symmetric)
+// leaves NaN.
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *args, const int *ebase, int nprob) {
     const int u = blockIdx.x * BMI_BLOCK_THREADS + threadIdx.x;
     if (u >= ebase[nprob]) return;
     const int pi = find_problem(ebase, nprob, u);
     const SpecialArgs &a = args[pi];
     const int ns = min(*a.n_special, a.capacity);
-    {
-        const int e = u - ebase[pi];
-        if (e >= ns || !(a.spec_cls[e] & SPEC_NOT_OWNER)) return;
-        const int K = a.spec_K[e], L = a.spec_L[e];
-        const int hs = a.X.home[K];
-        const long long b = (long long)hs * a.npb + a.Y.home[L];
-        const int o0 = a.blk_off[b], o1 = min(o0 + a.blk_cnt[b], a.capacity);
-        // a block's entries are in the order of their pair's thread
-        // kl * 16 + ll; L sits in one patch, so ll is this entry's, and kl is
-        // K's place in its home sub-batch: binary search for that thread
-        int kl = 0;
-        for (int k = 0; k < BMI_PATCH; ++k)
-            if (a.X.patch_tri[(long long)hs * BMI_PATCH + k] == K) kl = k;
-        const int want = kl * BMI_PATCH + (a.spec_loc[e] & (BMI_PATCH - 1));
-        int lo = o0, hi = o1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (a.spec_loc[mid] < want) lo = mid + 1;
-            else hi = mid;
-        }
-        const int o = lo < o1 && a.spec_K[lo] == K && a.spec_L[lo] == L ? lo : -1;
+    const int e = u - ebase[pi];
+    if (e >= ns) return;
+    const int raw = a.spec_cls[e], cls = raw & ~SPEC_NOT_OWNER;
+    if (cls == CLS_CF && a.cf_zero) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v)
-            a.spec_val[9 * (long long)e + v] =
-                o >= 0 ? a.spec_val[9 * (long long)o + v] : make_double2(__longlong_as_double(0x7ff8000000000000LL),
-                                                                          __longlong_as_double(0x7ff8000000000000LL));
+        for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = make_double2(0.0, 0.0);
+        return;
     }
+    const int K = a.spec_K[e], L = a.spec_L[e];
+    const bool mirror = a.sym && cls == CLS_CV && K > L;
+    if (!(raw & SPEC_NOT_OWNER) && !mirror) return;
+    // the source pair: (K, L) in its owner entry, or (L, K)
+    const int S = mirror ? L : K, T = mirror ? K : L;
+    const int hs = a.X.home[S], hp = a.Y.home[T];
+    const long long b = (long long)hs * a.npb + hp;
+    const int o0 = a.blk_off[b], o1 = min(o0 + a.blk_cnt[b], a.capacity);
+    int kl = 0, ll = a.spec_loc[e] & (BMI_PATCH - 1);  // a non-owner's trial cell sits in the same patch
+    for (int k = 0; k < BMI_PATCH; ++k)
+        if (a.X.patch_tri[(long long)hs * BMI_PATCH + k] == S) kl = k;
+    if (mirror)
+        for (int k = 0; k < BMI_PATCH; ++k)
+            if (a.Y.patch_tri[(long long)hp * BMI_PATCH + k] == T) ll = k;
+    const int want = kl * BMI_PATCH + ll;
+    int lo = o0, hi = o1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a.spec_loc[mid] < want) lo = mid + 1;
+        else hi = mid;
+    }
+    const int o = lo < o1 && a.spec_K[lo] == S && a.spec_L[lo] == T ? lo : -1;
+    const double2 nan2 = make_double2(__longlong_as_double(0x7ff8000000000000LL),
+                                      __longlong_as_double(0x7ff8000000000000LL));
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            a.spec_val[9 * (long long)e + 3 * i + j] =
+                o >= 0 ? a.spec_val[9 * (long long)o + (mirror ? 3 * j + i : 3 * i + j)] : nan2;
 }
 
 hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *ebase, int nprob, int total_chunks,
