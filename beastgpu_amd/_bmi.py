@@ -145,6 +145,9 @@ class Result(ctypes.Structure):
                 "total_ms": self.total_ms, "entries": int(self.entries)}
 
 
+BATCH_MODES = ("per-colour", "persistent", "per-problem")
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [
         ("last_ms", ctypes.c_double),
@@ -155,17 +158,17 @@ class BatchStats(ctypes.Structure):
         ("bucket_mode", ctypes.c_int32 * 8),
         ("bucket_wgs_per_cu", ctypes.c_int32 * 8),
         ("bucket_tuned_ms", ctypes.c_double * 8),
-        ("bucket_candidate_ms", (ctypes.c_double * 4) * 8),
+        ("bucket_candidate_ms", (ctypes.c_double * 5) * 8),
     ]
 
     def as_dict(self) -> dict:
         nb = self.buckets
         return {"last_ms": self.last_ms, "problems": self.problems, "buckets": nb, "singles": self.singles,
                 "bucket_problems": list(self.bucket_problems[:nb]),
-                "bucket_mode": ["persistent" if m == 1 else "per-colour" for m in self.bucket_mode[:nb]],
+                "bucket_mode": [BATCH_MODES[m] for m in self.bucket_mode[:nb]],
                 "bucket_wgs_per_cu": list(self.bucket_wgs_per_cu[:nb]),
                 "bucket_tuned_ms": list(self.bucket_tuned_ms[:nb]),
-                # [per-colour, persistent 1, 2, 3 work-groups per CU]; 0: not timed
+                # [per-colour, persistent 1, 2, 3 work-groups per CU, per-problem]; 0: not timed
                 "bucket_candidate_ms": [list(self.bucket_candidate_ms[b]) for b in range(nb)]}
 
 

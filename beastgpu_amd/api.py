@@ -339,10 +339,10 @@ class Batch(_Handle):
 
     def set_schedule(self, bucket: int, mode: str, wgs_per_cu: int = 0) -> None:
         """Override the tuned far-phase schedule of one bucket: "per-colour",
-        or "persistent" with 1 .. 3 work-groups per CU."""
-        if mode not in ("per-colour", "persistent"):
-            raise ValueError("mode must be 'per-colour' or 'persistent'")
-        _bmi.check(_bmi.lib().bmi_batch_set_schedule(self.h, bucket, int(mode == "persistent"), wgs_per_cu))
+        "persistent" with 1 .. 3 work-groups per CU, or "per-problem"."""
+        if mode not in _bmi.BATCH_MODES:
+            raise ValueError(f"mode must be one of {_bmi.BATCH_MODES}")
+        _bmi.check(_bmi.lib().bmi_batch_set_schedule(self.h, bucket, _bmi.BATCH_MODES.index(mode), wgs_per_cu))
 
 
 def assemble_gpu(operator: Operator, test_functions: Space, trial_functions: Space,

@@ -1969,7 +1969,7 @@ struct BatchBucket {
     DevBuf<int> d_head;             // the persistent queue's head
     int mode = 0, wpc = 0;
     double tuned_ms = 0.0;
-    double cand_ms[BMI_BATCH_CANDIDATES] = {};  // per-colour, persistent with 1, 2, 3 work-groups per CU
+    double cand_ms[BMI_BATCH_CANDIDATES] = {};  // per-colour, persistent with 1, 2, 3 work-groups per CU, per-problem
 };
 
 // A bucket signature's tuned schedule, kept for the process.
@@ -2021,13 +2021,20 @@ struct bmi_batch {
 namespace {
 
 // The far phase of one bucket on stream s with the bucket's schedule: one
-// launch per trial colour (mode 0), or one persistent panel-major launch of
-// wpc work-groups per CU (mode 1).
-bmi_status enqueue_bucket(BatchBucket &B, int mode, int wpc, int cus, hipStream_t s) {
+// launch per trial colour over every problem's blocks (mode 0), one
+// persistent panel-major launch of wpc work-groups per CU (mode 1), or each
+// problem's own per-colour launches, one problem after another (mode 2).
+bmi_status enqueue_bucket(bmi_batch *Bt, BatchBucket &B, int mode, int wpc, hipStream_t s) {
     if (mode == 1) {
         BMI_HIP(launch_zero_ints(B.d_head.p, 1, s));
-        BMI_HIP(launch_panels_persist(B.d_args.p, B.d_pitems.p, B.npitems, B.ncm, B.d_head.p, cus * wpc, B.op_kind,
-                                      B.mixed, B.fast, B.c32, s));
+        BMI_HIP(launch_panels_persist(B.d_args.p, B.d_pitems.p, B.npitems, B.ncm, B.d_head.p, Bt->cus * wpc,
+                                      B.op_kind, B.mixed, B.fast, B.c32, s));
+    } else if (mode == 2) {
+        for (int p : B.problems) {
+            int launches = 0;
+            const bmi_status st = enqueue_far(Bt->plans[p], Bt->mats[p], s, &launches);
+            if (st != BMI_OK) return st;
+        }
     } else {
         for (size_t c = 0; c + 1 < B.colour_off.size(); ++c)
             BMI_HIP(launch_panels_items(B.d_args.p, B.d_items.p + B.colour_off[c],
@@ -2117,11 +2124,15 @@ bmi_status tune_bucket(bmi_batch *Bt, BatchBucket &B) {
     hipStream_t s = Bt->ctx->stream;
     const int maxwpc = max_wgs_per_cu(B);
     double best = 1e300;
-    for (int c = 0; c <= maxwpc; ++c) {  // c = 0: per-colour, else persistent with c work-groups per CU
+    // candidate c: 0 per-colour, 1 .. maxwpc persistent with c work-groups
+    // per CU, BMI_BATCH_CANDIDATES - 1 per-problem launches
+    for (int c = 0; c < BMI_BATCH_CANDIDATES; ++c) {
+        if (c > maxwpc && c < BMI_BATCH_CANDIDATES - 1) continue;
+        const int mode = c == 0 ? 0 : (c == BMI_BATCH_CANDIDATES - 1 ? 2 : 1), wpc = mode == 1 ? c : 0;
         float ms = 0.0f;
         for (int rep = 0; rep < 2; ++rep) {
             BMI_HIP(hipEventRecord(Bt->ev0, s));
-            bmi_status st = enqueue_bucket(B, c > 0, c, Bt->cus, s);
+            bmi_status st = enqueue_bucket(Bt, B, mode, wpc, s);
             if (st != BMI_OK) return st;
             BMI_HIP(hipEventRecord(Bt->ev1, s));
             BMI_HIP(hipStreamSynchronize(s));
@@ -2130,8 +2141,8 @@ bmi_status tune_bucket(bmi_batch *Bt, BatchBucket &B) {
         B.cand_ms[c] = ms;
         if (ms < best) {
             best = ms;
-            B.mode = c > 0;
-            B.wpc = c;
+            B.mode = mode;
+            B.wpc = wpc;
             B.tuned_ms = ms;
         }
     }
@@ -2258,7 +2269,7 @@ bmi_status capture_batch(bmi_batch *Bt) {
             BMI_HIP(launch_group_special(G, b));
             for (auto &B : Bt->buckets) {
                 if (B.op_kind != G.op_kind) continue;
-                bmi_status r = enqueue_bucket(B, B.mode, B.wpc, Bt->cus, b);
+                bmi_status r = enqueue_bucket(Bt, B, B.mode, B.wpc, b);
                 if (r != BMI_OK) return r;
             }
             BMI_HIP(hipEventRecord(Bt->join[gi], b));
@@ -2411,9 +2422,9 @@ bmi_status bmi_batch_set_schedule(bmi_batch *Bt, int32_t bucket, int32_t mode, i
     if (!Bt) return fail(BMI_EINVAL, "NULL argument");
     if (bucket < 0 || bucket >= (int)Bt->buckets.size()) return fail(BMI_EINVAL, "no such bucket");
     BatchBucket &B = Bt->buckets[bucket];
-    if (mode == 0) wgs_per_cu = 0;
+    if (mode == 0 || mode == 2) wgs_per_cu = 0;
     else if (mode != 1 || wgs_per_cu < 1 || wgs_per_cu > max_wgs_per_cu(B))
-        return fail(BMI_EINVAL, "mode 0, or mode 1 with 1 .. " + std::to_string(max_wgs_per_cu(B)) +
+        return fail(BMI_EINVAL, "mode 0, mode 2, or mode 1 with 1 .. " + std::to_string(max_wgs_per_cu(B)) +
                                     " work-groups per CU for this bucket");
     BMI_HIP(hipSetDevice(Bt->device));
     BMI_HIP(hipStreamSynchronize(Bt->ctx->stream));

@@ -2424,7 +2424,15 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
     __shared__ PanelShared sh;
     if (!MIXED) fill_sincos_table(sh.s_sc);
     const int4 it = items[blockIdx.x];
-    panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, tile, sh);
+    if constexpr (MIXED) {
+        // a copy: the kernel's stores to the matrix cannot alias it, so its
+        // fields are loaded once instead of after each store (the FP64
+        // kernels lack the registers for it)
+        const PanelArgs a = args[it.x];
+        panel_block<OP, MIXED, FAST, OutT>(a, it.y, it.z, tile, sh);
+    } else {
+        panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, tile, sh);
+    }
 }
 
 // Batched, persistent and panel-major: the work-groups pop items (problem,
@@ -2451,10 +2459,17 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
         const int i = s_item;
         if (i >= nitems) break;
         const int2 it = items[i];
-        const PanelArgs &a = args[it.x];
-        for (int u = 0; u < a.npb; ++u) {
-            if (u > 0) __syncthreads();
-            panel_block<OP, MIXED, FAST, OutT>(a, it.y, a.bq_all[u], tile, sh);
+        auto run = [&](const PanelArgs &a) {
+            for (int u = 0; u < a.npb; ++u) {
+                if (u > 0) __syncthreads();
+                panel_block<OP, MIXED, FAST, OutT>(a, it.y, a.bq_all[u], tile, sh);
+            }
+        };
+        if constexpr (MIXED) {
+            const PanelArgs a = args[it.x];  // as in k_panels_items
+            run(a);
+        } else {
+            run(args[it.x]);
         }
     }
 }

@@ -248,13 +248,14 @@ bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double
  * Many assemblies (plans with their output matrices, one context) run as one
  * replayed hipGraph: one classification launch over every problem, one
  * special-pair launch per operator, then, per bucket of problems that
- * share a far-pair kernel (operator, precision, rule, output type), either
- * one launch per trial colour over every problem's (test panel, trial patch)
- * blocks, or one persistent launch whose work-groups pop (problem, test
- * panel) items, largest first, from a device queue and integrate each panel
- * against all trial patches of its problem in colour order (no work-group
- * waits for another) -- whichever the autotuner at creation measured faster
- * (per-colour, or persistent with 1, 2 or 3 work-groups per CU; results are
+ * share a far-pair kernel (operator, precision, rule, output type), one of
+ * three schedules: one launch per trial colour over every problem's (test
+ * panel, trial patch) blocks; one persistent launch whose work-groups pop
+ * (problem, test panel) items, largest first, from a device queue and
+ * integrate each panel against all trial patches of its problem in colour
+ * order (no work-group waits for another); or each problem's own per-colour
+ * launches in turn -- whichever the autotuner at creation measured fastest
+ * (the persistent launch with 1, 2 or 3 work-groups per CU; results are
  * cached per bucket signature for the process).  Either way the matrices are
  * bitwise those of bmi_assemble on each plan.
  * No counterpart in the reference (its batching precedent is the producer /
@@ -263,7 +264,7 @@ bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double
 typedef struct bmi_batch bmi_batch;
 
 #define BMI_BATCH_MAX_BUCKETS 8
-#define BMI_BATCH_CANDIDATES 4
+#define BMI_BATCH_CANDIDATES 5
 
 typedef struct {
     double last_ms;               /* device time of the last bmi_batch_assemble (events around the graph) */
@@ -271,11 +272,12 @@ typedef struct {
     int32_t buckets;
     int32_t singles;              /* problems assembled with their own per-colour launches (no bucket) */
     int32_t bucket_problems[BMI_BATCH_MAX_BUCKETS];
-    int32_t bucket_mode[BMI_BATCH_MAX_BUCKETS];        /* 0: one launch per colour, 1: persistent queue */
+    int32_t bucket_mode[BMI_BATCH_MAX_BUCKETS];        /* 0: per colour, 1: persistent queue, 2: per problem */
     int32_t bucket_wgs_per_cu[BMI_BATCH_MAX_BUCKETS];  /* persistent queue: work-groups per CU */
     double bucket_tuned_ms[BMI_BATCH_MAX_BUCKETS];     /* far phase of the chosen schedule when tuned */
     /* far phase of each schedule the tuner timed for the bucket's signature:
-     * [0] per-colour, [c] persistent with c work-groups per CU; 0 if not timed */
+     * [0] per colour, [c] persistent with c = 1 .. 3 work-groups per CU,
+     * [4] per problem; 0 if not timed */
     double bucket_candidate_ms[BMI_BATCH_MAX_BUCKETS][BMI_BATCH_CANDIDATES];
 } bmi_batch_stats;
 
@@ -286,9 +288,9 @@ bmi_status bmi_batch_create(bmi_context *ctx, bmi_plan *const *plans, bmi_matrix
 /* Replays the graph; blocking. */
 bmi_status bmi_batch_assemble(bmi_batch *batch);
 bmi_status bmi_batch_stats_get(const bmi_batch *batch, bmi_batch_stats *stats);
-/* Overrides the tuned schedule of one bucket (mode 0: per-colour; mode 1:
- * persistent with wgs_per_cu work-groups per CU, 1 .. the kernel's occupancy)
- * and captures the graph again.  BMI_EINVAL for a bucket or schedule that
+/* Overrides the tuned schedule of one bucket (mode 0: per colour; mode 1:
+ * persistent with wgs_per_cu work-groups per CU, 1 .. the kernel's occupancy;
+ * mode 2: per problem) and captures the graph again.  BMI_EINVAL for a bucket or schedule that
  * does not exist. */
 bmi_status bmi_batch_set_schedule(bmi_batch *batch, int32_t bucket, int32_t mode, int32_t wgs_per_cu);
 bmi_status bmi_batch_free(bmi_batch *batch);

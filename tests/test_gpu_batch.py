@@ -62,9 +62,9 @@ def test_batch_bitwise_equals_single_assemblies_on_every_replay(gpu_ctx):
         for i, (a, r) in enumerate(zip(asms, ref)):
             assert np.array_equal(a.matrix.to_host(), r), (rep, i)
     # every schedule of every bucket: persistent with 1 and with the most
-    # work-groups per CU, then per-colour
+    # work-groups per CU, per-problem launches, then per-colour
     for bk in range(st["buckets"]):
-        for mode, w in (("persistent", 1), ("persistent", 3), ("per-colour", 0)):
+        for mode, w in (("persistent", 1), ("persistent", 3), ("per-problem", 0), ("per-colour", 0)):
             try:
                 b.set_schedule(bk, mode, w)
             except _bmi.BMIError as e:  # 3 work-groups per CU beyond the FP64 double layer's occupancy
