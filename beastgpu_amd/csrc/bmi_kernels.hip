@@ -1860,19 +1860,9 @@ hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *
     if (total_entries <= 0) return hipSuccess;
     const dim3 b(BMI_BLOCK_THREADS);
     const dim3 g(total_chunks);
-    if (ng5) {
-        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
-        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
-        else hipLaunchKernelGGL((k_special<OPK_HH3D, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
-    } else {
-        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
-        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
-        else hipLaunchKernelGGL((k_special<OPK_HH3D, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
-    }
-    if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
-    else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
-    else hipLaunchKernelGGL((k_special<OPK_HH3D, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
     const int g2 = (total_entries + BMI_BLOCK_THREADS - 1) / BMI_BLOCK_THREADS;
+    // the short kernels first (in a batch they then share the device with the
+    // other operators' Sauter-Schwab kernels rather than their far kernels)
     if (mixed && fast) {
         if (op_kind == OPK_MW) hipLaunchKernelGGL((k_far64<OPK_MW, 1>), dim3(g2), b, 0, s, args, ebase, nprob);
         else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_far64<OPK_DL, 1>), dim3(g2), b, 0, s, args, ebase, nprob);
@@ -1881,6 +1871,18 @@ hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *
         if (op_kind == OPK_MW) hipLaunchKernelGGL((k_far64<OPK_MW, 0>), dim3(g2), b, 0, s, args, ebase, nprob);
         else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_far64<OPK_DL, 0>), dim3(g2), b, 0, s, args, ebase, nprob);
         else hipLaunchKernelGGL((k_far64<OPK_HH3D, 0>), dim3(g2), b, 0, s, args, ebase, nprob);
+    }
+    if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
+    else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
+    else hipLaunchKernelGGL((k_special<OPK_HH3D, 0, true>), g, b, 0, s, args, cbase, nprob, cs);
+    if (ng5) {
+        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
+        else hipLaunchKernelGGL((k_special<OPK_HH3D, 5, false>), g, b, 0, s, args, cbase, nprob, cs);
+    } else {
+        if (op_kind == OPK_MW) hipLaunchKernelGGL((k_special<OPK_MW, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
+        else if (op_kind == OPK_DL) hipLaunchKernelGGL((k_special<OPK_DL, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
+        else hipLaunchKernelGGL((k_special<OPK_HH3D, 0, false>), g, b, 0, s, args, cbase, nprob, cs);
     }
     hipLaunchKernelGGL(k_link, dim3(g2), b, 0, s, args, ebase, nprob);
     return hipGetLastError();
