@@ -8,9 +8,11 @@
 #include <array>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -418,24 +420,90 @@ void group_sphere(const bmi_mesh *m, const int *cells, int n, double *sph, doubl
     sph[3] = std::sqrt(r2) * (1.0 + 1e-12);
 }
 
-// f(0) ... f(n - 1) over up to 8 host threads (the calling thread and
-// others started for the call), in chunks of 8 indices taken in turn.
+// Host worker threads, started once for the process (up to 7, beside the
+// calling thread): a call hands out the indices 0 .. n - 1 in chunks and
+// returns when all are done.  Calls from several threads at once (serving
+// workers preparing requests) share the workers; each caller also works on
+// its own call, so a call always completes.  Starting threads per call cost
+// about 0.35 ms each time (a request's space creation made four such calls).
+class HostPool {
+  public:
+    static HostPool &get() {
+        static HostPool *pool = new HostPool();  // never destroyed: workers may still wait at exit
+        return *pool;
+    }
+    int threads() const { return (int)workers_ + 1; }
+    void run(int n, int chunk, const std::function<void(int)> &f) {
+        auto job = std::make_shared<Job>();
+        job->f = &f;
+        job->n = n;
+        job->chunk = chunk;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            jobs_.push_back(job);
+        }
+        cv_.notify_all();
+        work(*job);
+        while (job->done.load(std::memory_order_acquire) < n) std::this_thread::yield();
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto it = jobs_.begin(); it != jobs_.end(); ++it)
+            if (*it == job) {
+                jobs_.erase(it);
+                break;
+            }
+    }
+
+  private:
+    struct Job {
+        const std::function<void(int)> *f = nullptr;
+        int n = 0, chunk = 1;
+        std::atomic<int> next{0}, done{0};
+    };
+    HostPool() {
+        workers_ = std::min<unsigned>(7u, std::max(1u, std::thread::hardware_concurrency()) - 1u);
+        for (unsigned t = 0; t < workers_; ++t) std::thread([this] { loop(); }).detach();
+    }
+    static void work(Job &j) {
+        for (int i; (i = j.next.fetch_add(j.chunk)) < j.n;) {
+            const int e = std::min(j.n, i + j.chunk);
+            for (int k = i; k < e; ++k) (*j.f)(k);
+            j.done.fetch_add(e - i, std::memory_order_release);
+        }
+    }
+    void loop() {
+        for (;;) {
+            std::shared_ptr<Job> job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] {
+                    for (auto &j : jobs_)
+                        if (j->next.load() < j->n) return true;
+                    return false;
+                });
+                for (auto &j : jobs_)
+                    if (j->next.load() < j->n) {
+                        job = j;
+                        break;
+                    }
+            }
+            if (job) work(*job);
+        }
+    }
+    unsigned workers_ = 0;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::shared_ptr<Job>> jobs_;
+};
+
+// f(0) ... f(n - 1) on the host pool, in chunks of 8 indices taken in turn.
 template <class F>
 void parallel_for(int n, F f) {
-    const int nth = (int)std::min<unsigned>(8u, std::max(1u, std::thread::hardware_concurrency()));
-    if (n < 64 || nth <= 1) {
+    if (n < 64) {
         for (int i = 0; i < n; ++i) f(i);
         return;
     }
-    std::atomic<int> next{0};
-    auto work = [&]() {
-        for (int i; (i = next.fetch_add(8)) < n;)
-            for (int j = i; j < std::min(n, i + 8); ++j) f(j);
-    };
-    std::vector<std::thread> th;
-    for (int t = 1; t < nth; ++t) th.emplace_back(work);
-    work();
-    for (auto &x : th) x.join();
+    const std::function<void(int)> g = [&](int i) { f(i); };
+    HostPool::get().run(n, 8, g);
 }
 
 // Panels: functions [64 P, 64 P + 64) and every cell carrying one of them,
@@ -1722,13 +1790,12 @@ static void consume_chunk(const void *src, int src32, void *dst, int dst32, size
         work(0, count);
         return;
     }
-    std::vector<std::thread> th;
     const size_t per = (count + nth - 1) / nth;
-    for (unsigned t = 0; t < nth; ++t) {
-        const size_t a = t * per, b = std::min(count, a + per);
-        if (a < b) th.emplace_back(work, a, b);
-    }
-    for (auto &x : th) x.join();
+    const std::function<void(int)> g = [&](int t) {
+        const size_t a = std::min(count, (size_t)t * per), b = std::min(count, a + per);
+        if (a < b) work(a, b);
+    };
+    HostPool::get().run((int)nth, 1, g);
 }
 
 // Bytes per pinned staging buffer of the device-to-host copies.
@@ -1763,13 +1830,12 @@ static void scatter_rows_piece(const unsigned char *src, size_t es, size_t o, si
         work(0, c);
         return;
     }
-    std::vector<std::thread> th;
     const size_t per = (c + nth - 1) / nth;
-    for (unsigned t = 0; t < nth; ++t) {
-        const size_t a = t * per, b = std::min(c, a + per);
-        if (a < b) th.emplace_back(work, a, b);
-    }
-    for (auto &x : th) x.join();
+    const std::function<void(int)> g = [&](int t) {
+        const size_t a = std::min(c, (size_t)t * per), b = std::min(c, a + per);
+        if (a < b) work(a, b);
+    };
+    HostPool::get().run((int)nth, 1, g);
 }
 
 bmi_status bmi_assemble_to_host(bmi_plan *P, void *dst, int32_t dtype, int64_t device_budget) try {
@@ -1992,6 +2058,11 @@ struct BatchOpGroup {
     int total_chunks = 0, total_entries = 0, cs = 256;
     DevBuf<SpecialArgs> d_sargs;
     DevBuf<int> d_cbase, d_ebase;
+    // classification of the group's problems (one launch)
+    int total_groups = 0;
+    DevBuf<ClassifyArgs> d_cargs;
+    DevBuf<int> d_gbase;
+    DevBuf<int *> d_counters;       // each problem's classifier counter
 };
 
 struct bmi_batch {
@@ -1999,10 +2070,6 @@ struct bmi_batch {
     int device = 0;
     std::vector<bmi_plan *> plans;
     std::vector<bmi_matrix *> mats;
-    DevBuf<ClassifyArgs> d_cargs;   // every problem
-    DevBuf<int> d_gbase;
-    DevBuf<int *> d_counters;       // each problem's classifier counter
-    int total_groups = 0;
     std::deque<BatchOpGroup> opgroups;
     std::vector<int> singles;
     std::deque<BatchBucket> buckets;
@@ -2156,16 +2223,16 @@ bmi_status tune_bucket(bmi_batch *Bt, BatchBucket &B) {
     return BMI_OK;
 }
 
-// Classification of every problem (one launch) and zero fill of uncovered
-// rows: on stream s.
-bmi_status enqueue_batch_classify(bmi_batch *Bt, hipStream_t s) {
-    const int n = (int)Bt->plans.size();
-    BMI_HIP(launch_zero_each(Bt->d_counters.p, n, s));
-    BMI_HIP(launch_classify(Bt->d_cargs.p, Bt->d_gbase.p, n, Bt->total_groups, s));
-    for (const bmi_plan *P : Bt->plans) P->filled = true;
-    for (int i = 0; i < n; ++i) {
+// Classification of an operator group's problems (one launch) and zero fill
+// of their uncovered rows: on stream s.
+bmi_status enqueue_group_classify(bmi_batch *Bt, const BatchOpGroup &G, hipStream_t s) {
+    const int n = (int)G.problems.size();
+    BMI_HIP(launch_zero_each(G.d_counters.p, n, s));
+    BMI_HIP(launch_classify(G.d_cargs.p, G.d_gbase.p, n, G.total_groups, s));
+    for (int i : G.problems) {
         const bmi_plan *P = Bt->plans[i];
         const bmi_matrix *M = Bt->mats[i];
+        P->filled = true;
         if (P->X->uncovered || P->Y->uncovered)
             BMI_HIP(launch_fill_zero(M->d, (size_t)M->rows * M->cols * (M->dtype == BMI_C64 ? 16 : 8), s));
     }
@@ -2181,28 +2248,18 @@ hipError_t launch_group_special(const BatchOpGroup &G, hipStream_t s) {
 // Classification, zero fills and the special pairs of every operator group,
 // all on stream s (the batch before the far phase, outside a graph).
 bmi_status enqueue_batch_prologue(bmi_batch *Bt, hipStream_t s) {
-    bmi_status st = enqueue_batch_classify(Bt, s);
-    if (st != BMI_OK) return st;
-    for (auto &G : Bt->opgroups) BMI_HIP(launch_group_special(G, s));
+    for (auto &G : Bt->opgroups) {
+        const bmi_status st = enqueue_group_classify(Bt, G, s);
+        if (st != BMI_OK) return st;
+        BMI_HIP(launch_group_special(G, s));
+    }
     return BMI_OK;
 }
 
 // Device tables of the batched prologue.
 bmi_status build_prologue_tables(bmi_batch *Bt) {
     const int n = (int)Bt->plans.size();
-    std::vector<ClassifyArgs> ca;
-    std::vector<int> gb = {0};
-    for (int i = 0; i < n; ++i) {
-        ca.push_back(Bt->plans[i]->cargs);
-        gb.push_back(gb.back() + Bt->plans[i]->npa);
-    }
-    Bt->total_groups = gb.back();
-    std::vector<int *> counters;
-    for (int i = 0; i < n; ++i) counters.push_back(Bt->plans[i]->d_total.p);
     hipError_t e;
-    if ((e = Bt->d_cargs.upload(ca)) != hipSuccess || (e = Bt->d_gbase.upload(gb)) != hipSuccess ||
-        (e = Bt->d_counters.upload(counters)) != hipSuccess)
-        return fail(BMI_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
     for (int i = 0; i < n; ++i) {
         const bmi_plan *P = Bt->plans[i];
         BatchOpGroup *G = nullptr;
@@ -2219,6 +2276,18 @@ bmi_status build_prologue_tables(bmi_batch *Bt) {
         G->fast &= fast_rule(P);
     }
     for (auto &G : Bt->opgroups) {
+        std::vector<ClassifyArgs> ca;
+        std::vector<int> gb = {0};
+        std::vector<int *> counters;
+        for (int i : G.problems) {
+            ca.push_back(Bt->plans[i]->cargs);
+            gb.push_back(gb.back() + Bt->plans[i]->npa);
+            counters.push_back(Bt->plans[i]->d_total.p);
+        }
+        G.total_groups = gb.back();
+        if ((e = G.d_cargs.upload(ca)) != hipSuccess || (e = G.d_gbase.upload(gb)) != hipSuccess ||
+            (e = G.d_counters.upload(counters)) != hipSuccess)
+            return fail(BMI_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
         int cap = 0;
         for (int i : G.problems) cap += Bt->plans[i]->capacity;
         G.cs = special_chunk(cap);
@@ -2239,10 +2308,11 @@ bmi_status build_prologue_tables(bmi_batch *Bt) {
     return BMI_OK;
 }
 
-// The whole batch as one graph: the classification of every problem (and
-// zero fills) on the context stream, then one branch per operator group, its
-// special pairs followed by the far buckets of that operator (the groups'
-// problems are disjoint, so the branches overlap each other's launch tails),
+// The whole batch as one graph: one branch per operator group, the
+// classification (and zero fills) of its problems, its special pairs, then
+// the far buckets of that operator (the groups' problems are disjoint, so the
+// branches overlap each other's launch tails and the latency-bound
+// classification of one group overlaps the other's Sauter-Schwab work),
 // joined back before the problems outside the buckets.
 bmi_status capture_batch(bmi_batch *Bt) {
     hipStream_t s = Bt->ctx->stream;
@@ -2259,13 +2329,13 @@ bmi_status capture_batch(bmi_batch *Bt) {
     BMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     bmi_status st = BMI_OK;
     auto body = [&]() -> bmi_status {
-        bmi_status r0 = enqueue_batch_classify(Bt, s);
-        if (r0 != BMI_OK) return r0;
         BMI_HIP(hipEventRecord(Bt->fork, s));
         for (size_t gi = 0; gi < ng; ++gi) {
             const BatchOpGroup &G = Bt->opgroups[gi];
             hipStream_t b = Bt->branch[gi];
             BMI_HIP(hipStreamWaitEvent(b, Bt->fork, 0));
+            bmi_status r0 = enqueue_group_classify(Bt, G, b);
+            if (r0 != BMI_OK) return r0;
             BMI_HIP(launch_group_special(G, b));
             for (auto &B : Bt->buckets) {
                 if (B.op_kind != G.op_kind) continue;

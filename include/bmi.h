@@ -246,8 +246,8 @@ bmi_status bmi_matrix_compare(const bmi_matrix *a, const bmi_matrix *ref, double
 
 /* --------------------------------------------------------------- batching
  * Many assemblies (plans with their output matrices, one context) run as one
- * replayed hipGraph: one classification launch over every problem, one
- * special-pair launch per operator, then, per bucket of problems that
+ * replayed hipGraph, one branch per operator: one classification launch and
+ * one special-pair launch over its problems, then, per bucket of problems that
  * share a far-pair kernel (operator, precision, rule, output type), one of
  * three schedules: one launch per trial colour over every problem's (test
  * panel, trial patch) blocks; one persistent launch whose work-groups pop
