@@ -1899,6 +1899,29 @@ int special_chunk(int capacity) {
 
 // ------------------------------------------------------ far-pair blocks
 
+// The write-back's matrix accesses, non-temporal: each block's entries are
+// stored once (or read back once, by the launch of a later colour), so they
+// stream past the caches instead of displacing the records every block reads
+// (far phase about 2 % faster).
+typedef double bmi_d2v __attribute__((ext_vector_type(2)));
+typedef float bmi_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_stream(double2 *p, double2 v) {
+    const bmi_d2v x = {v.x, v.y};
+    __builtin_nontemporal_store(x, (bmi_d2v *)p);
+}
+__device__ __forceinline__ void store_stream(float2 *p, float2 v) {
+    const bmi_f2v x = {v.x, v.y};
+    __builtin_nontemporal_store(x, (bmi_f2v *)p);
+}
+__device__ __forceinline__ double2 load_stream(const double2 *p) {
+    const bmi_d2v x = __builtin_nontemporal_load((const bmi_d2v *)p);
+    return make_double2(x.x, x.y);
+}
+__device__ __forceinline__ float2 load_stream(const float2 *p) {
+    const bmi_f2v x = __builtin_nontemporal_load((const bmi_f2v *)p);
+    return make_float2(x.x, x.y);
+}
+
 __device__ __forceinline__ void out_write(double2 *Z, long long off, double re, double im, bool first) {
     double2 v;
     if (first) {
@@ -2383,7 +2406,8 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
             const int c = wv + k * (BMI_BLOCK_THREADS / 64);
             const int r = 64 * h + (lane ^ (c & 15));
             old[NH * k + h] = OutT{};
-            if (c < nc && r < nr && !s_cfirst[c]) old[NH * k + h] = Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld];
+            if (c < nc && r < nr && !s_cfirst[c])
+                old[NH * k + h] = load_stream(&Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld]);
         }
 #pragma unroll
     for (int k = 0; k < NCW; ++k)
@@ -2399,7 +2423,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                 OutT o;
                 o.x = (decltype(o.x))((double)old[NH * k + h].x + v.x);
                 o.y = (decltype(o.y))((double)old[NH * k + h].y + v.y);
-                Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld] = o;
+                store_stream(&Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld], o);
             }
         }
 }
@@ -2656,7 +2680,14 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_apply(const T *Z, long lo
         for (int cb = s0; cb < s1; cb += U) {
             T zz[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) zz[u] = cb + u < s1 ? Z[row + (long long)(cb + u) * ld] : T{};
+            for (int u = 0; u < U; ++u) {
+                // ComplexF64 entries stream past the caches (10 % faster);
+                // ComplexF32 ones read slower that way (measured 2.7x)
+                if constexpr (std::is_same<T, double2>::value)
+                    zz[u] = cb + u < s1 ? load_stream(&Z[row + (long long)(cb + u) * ld]) : T{};
+                else
+                    zz[u] = cb + u < s1 ? Z[row + (long long)(cb + u) * ld] : T{};
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (cb + u >= s1) break;
