@@ -50,7 +50,7 @@ FAR_KERNEL = {"efie": "k_panels_rr<MW, FP64, fixed 3x4 rule, C64> (far pairs + w
 # device copy of known size); from rocprofv3 --pmc runs of this script, kept
 # in profiles/k_panels_rr_efie_L5_pmc.md
 TRAFFIC = {
-    "efie-L5": {"bytes_per_launch": 4.986e9, "read_bytes_per_launch": 1.516e9, "write_bytes_per_launch": 3.470e9,
+    "efie-L5": {"bytes_per_launch": 4.598e9, "read_bytes_per_launch": 1.128e9, "write_bytes_per_launch": 3.470e9,
                 "algorithmic_bytes_per_launch": 2.517e9, "fetch_scale": 2.0, "write_scale": 1.0,
                 "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, FP64 C64 far-pair kernel, 6 launches"},
 }
