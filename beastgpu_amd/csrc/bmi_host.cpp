@@ -646,39 +646,47 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
     std::vector<int> trs((size_t)s->nsb * (R + 1), 0);
     std::vector<unsigned char> tkl((size_t)s->nsb * BMI_MAX_ROWS, 0);
     std::vector<double> tcf((size_t)s->nsb * BMI_MAX_ROWS, 0.0);
-    for (int b = 0; b < s->nsb; ++b) {
+    // rank-round records: row of each cell slot in its panel and its rank
+    // among the sub-batch's slots of that row
+    std::vector<int> tslot((size_t)s->nsb * BMI_MAX_ROWS, 0xff);
+    std::vector<double> tscf((size_t)s->nsb * BMI_MAX_ROWS, 0.0);
+    std::vector<int> sb_panel(s->nsb), sb_rounds(s->nsb, 1);
+    for (int P = 0; P < np; ++P)
+        for (int b = s->sb_off[P]; b < s->sb_off[P + 1]; ++b) sb_panel[b] = P;
+    std::atomic<int> too_many_slots{0}, too_many_cells{0};
+    parallel_for(s->nsb, [&](int b) {
         const int *rp = &s->sb_row_ptr[(size_t)b * (R + 1)];
         const int base = rp[0];
-        if (rp[R] - base > BMI_MAX_ROWS) return fail(BMI_EINVAL, "more than 48 slots in a sub-batch");
+        if (rp[R] - base > BMI_MAX_ROWS) {
+            too_many_slots = 1;
+            return;
+        }
         for (int r = 0; r <= R; ++r) trs[(size_t)b * (R + 1) + r] = rp[r] - base;
         for (int u = base; u < rp[R]; ++u) {
             tkl[(size_t)b * BMI_MAX_ROWS + (u - base)] = s->sb_slot_kl[u];
             tcf[(size_t)b * BMI_MAX_ROWS + (u - base)] = s->sb_slot_cf[u];
         }
-    }
-    // rank-round records: row of each cell slot in its panel and its rank
-    // among the sub-batch's slots of that row
-    std::vector<int> tslot((size_t)s->nsb * BMI_MAX_ROWS, 0xff);
-    std::vector<double> tscf((size_t)s->nsb * BMI_MAX_ROWS, 0.0);
-    s->rounds_t = 1;
-    for (int P = 0; P < np; ++P)
-        for (int b = s->sb_off[P]; b < s->sb_off[P + 1]; ++b) {
-            int seen[BMI_PANEL_ROWS_WIDE] = {0};
-            for (int k = 0; k < BMI_PATCH; ++k) {
-                const int K = s->sb_cells[(size_t)b * BMI_PATCH + k];
-                if (K < 0) continue;
-                for (int i = 0; i < 3; ++i) {
-                    const int64_t f = gidx[3 * (int64_t)K + i];
-                    if (f < 0 || f / R != P) continue;
-                    const int r = (int)(f - (int64_t)P * R);
-                    const int rank = seen[r]++;
-                    if (rank > 255) return fail(BMI_EUNSUPPORTED, "function with more than 256 cells");
-                    tslot[(size_t)b * BMI_MAX_ROWS + 3 * k + i] = r | (rank << 8);
-                    tscf[(size_t)b * BMI_MAX_ROWS + 3 * k + i] = coeff[3 * (int64_t)K + i];
-                    s->rounds_t = std::max(s->rounds_t, rank + 1);
-                }
+        const int P = sb_panel[b];
+        int seen[BMI_PANEL_ROWS_WIDE] = {0};
+        for (int k = 0; k < BMI_PATCH; ++k) {
+            const int K = s->sb_cells[(size_t)b * BMI_PATCH + k];
+            if (K < 0) continue;
+            for (int i = 0; i < 3; ++i) {
+                const int64_t f = gidx[3 * (int64_t)K + i];
+                if (f < 0 || f / R != P) continue;
+                const int r = (int)(f - (int64_t)P * R);
+                const int rank = seen[r]++;
+                if (rank > 255) too_many_cells = 1;
+                tslot[(size_t)b * BMI_MAX_ROWS + 3 * k + i] = r | ((rank & 0xff) << 8);
+                tscf[(size_t)b * BMI_MAX_ROWS + 3 * k + i] = coeff[3 * (int64_t)K + i];
+                sb_rounds[b] = std::max(sb_rounds[b], rank + 1);
             }
         }
+    });
+    if (too_many_slots) return fail(BMI_EINVAL, "more than 48 slots in a sub-batch");
+    if (too_many_cells) return fail(BMI_EUNSUPPORTED, "function with more than 256 cells");
+    s->rounds_t = 1;
+    for (int b = 0; b < s->nsb; ++b) s->rounds_t = std::max(s->rounds_t, sb_rounds[b]);
     // home sub-batch of each cell: the first one that holds it
     std::vector<int> home(nt, -1);
     for (int b = 0; b < s->nsb; ++b)
@@ -714,29 +722,35 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
     double ext = 0.0;
     for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d]);
     if (ext <= 0.0) ext = 1.0;
-    std::vector<uint64_t> key(nt);
+    // (Morton key, cell) pairs sorted: the cells by key, ties by index
+    std::vector<std::pair<uint64_t, int>> key(nt);
     for (int64_t k = 0; k < nt; ++k) {
         uint64_t c[3];
         for (int d = 0; d < 3; ++d) {
             double u = (m->cen[3 * k + d] - lo[d]) / ext;
             c[d] = (uint64_t)std::min(2097151.0, std::max(0.0, u * 2097151.0));
         }
-        key[k] = spread3(c[0]) | spread3(c[1]) << 1 | spread3(c[2]) << 2;
+        key[k] = {spread3(c[0]) | spread3(c[1]) << 1 | spread3(c[2]) << 2, (int)k};
     }
+    std::sort(key.begin(), key.end());
     std::vector<int> order(nt);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
     morton_rank.assign(nt, 0);
-    for (int64_t i = 0; i < nt; ++i) morton_rank[order[i]] = (int)i;
+    for (int64_t i = 0; i < nt; ++i) {
+        order[i] = key[i].second;
+        morton_rank[order[i]] = (int)i;
+    }
 
     // runs of at most BMI_PATCH consecutive cells carrying at most
     // s->maxcols functions (bounds the LDS tile of the far kernel)
     s->patch_tri.clear();
     {
-        std::vector<int64_t> fs;
+        // in_run[f] = number of the run that holds function f (runs count from 1)
+        std::vector<int> in_run((size_t)s->nfun, 0);
+        int run = 0, nfs = 0;
         int64_t i = 0;
         while (i < nt) {
-            fs.clear();
+            ++run;
+            nfs = 0;
             int n = 0;
             while (i < nt && n < BMI_PATCH) {
                 const int K = order[i];
@@ -745,12 +759,11 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
                 int na = 0;
                 for (int j = 0; j < 3; ++j) {
                     const int64_t f = gidx[3 * (int64_t)K + j];
-                    if (f >= 0 && std::find(fs.begin(), fs.end(), f) == fs.end() &&
-                        std::find(add, add + na, f) == add + na)
-                        add[na++] = f;
+                    if (f >= 0 && in_run[f] != run && std::find(add, add + na, f) == add + na) add[na++] = f;
                 }
-                if (n > 0 && (int)fs.size() + na > s->maxcols) break;
-                fs.insert(fs.end(), add, add + na);
+                if (n > 0 && nfs + na > s->maxcols) break;
+                for (int j = 0; j < na; ++j) in_run[add[j]] = run;
+                nfs += na;
                 s->patch_tri.push_back(K);
                 ++n;
                 ++i;
@@ -868,7 +881,7 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
     std::vector<int> bnc(np, 0), bfn((size_t)np * BMI_MAX_ROWS, -1), bcs((size_t)np * (BMI_MAX_ROWS + 1), 0);
     std::vector<unsigned char> bfirst((size_t)np * BMI_MAX_ROWS, 0), bkl((size_t)np * BMI_MAX_ROWS, 0);
     std::vector<double> bcf((size_t)np * BMI_MAX_ROWS, 0.0);
-    for (int p = 0; p < np; ++p) {
+    parallel_for(np, [&](int p) {
         const int r0 = s->row_ptr[p], nr = s->row_ptr[p + 1] - r0;
         const int base = s->row_slot_ptr[r0];
         bnc[p] = nr;
@@ -883,7 +896,7 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
             bkl[(size_t)p * BMI_MAX_ROWS + (u - base)] = s->slot_kl[u];
             bcf[(size_t)p * BMI_MAX_ROWS + (u - base)] = s->slot_cf[u];
         }
-    }
+    });
     // rank-round records: column of each cell slot in its patch (bits 0-7),
     // its rank among the patch's slots of that column (bits 8-15) and its
     // level, the rank among the slots of that column with the same local
@@ -894,12 +907,8 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
     std::vector<int> bslot((size_t)np * BMI_MAX_ROWS, 0xff);
     std::vector<double> bscf((size_t)np * BMI_MAX_ROWS, 0.0);
     std::vector<unsigned char> brounds(2 * (size_t)np, 1);
-    s->rounds_b = 1;
-    s->levels_b = 1;
-    s->ncmax = 1;
-    for (int p = 0; p < np; ++p) {
+    parallel_for(np, [&](int p) {
         const int r0 = s->row_ptr[p], nr = s->row_ptr[p + 1] - r0;
-        s->ncmax = std::max(s->ncmax, nr);
         int seen[BMI_MAX_ROWS] = {0}, seen_j[BMI_MAX_ROWS][3] = {};
         int rk = 1, lv = 1;
         for (int l = 0; l < BMI_PATCH; ++l) {
@@ -919,8 +928,14 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
         }
         brounds[p] = (unsigned char)rk;
         brounds[np + p] = (unsigned char)lv;
-        s->rounds_b = std::max(s->rounds_b, rk);
-        s->levels_b = std::max(s->levels_b, lv);
+    });
+    s->rounds_b = 1;
+    s->levels_b = 1;
+    s->ncmax = 1;
+    for (int p = 0; p < np; ++p) {
+        s->ncmax = std::max(s->ncmax, s->row_ptr[p + 1] - s->row_ptr[p]);
+        s->rounds_b = std::max(s->rounds_b, (int)brounds[p]);
+        s->levels_b = std::max(s->levels_b, (int)brounds[np + p]);
     }
     if ((e = s->d_bslot.upload(bslot)) != hipSuccess || (e = s->d_bscf.upload(bscf)) != hipSuccess ||
         (e = s->d_brounds.upload(brounds)) != hipSuccess)
