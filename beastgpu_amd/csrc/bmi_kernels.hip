@@ -297,10 +297,124 @@ __device__ __forceinline__ void hh_epilogue(const OpConst &op, const double *pr,
 // only as differences and are accumulated so); the 1/(2A) of each RWG
 // shape cancels the Jacobian 2A of the point weights.  S is the arithmetic
 // type (double, or float for the mixed path).  FAST: entries divided by alpha.
+// Double-layer far pair with the default 3 x 4 point rules, entries divided
+// by alpha.  The rules' points are fixed fractions of the vertices relative
+// to the centroids (test point I: a_I / 2; trial point 0: the centroid,
+// trial point J: (2/5) b_(J-1)), so the moment sums of pair_dl collapse onto
+// the evaluations h_IJ themselves: with R_Ij = h_I,j+1, H0_I = h_I0,
+// C_ij = a_i.b_j, A_iI = a_i.a_I, ad_i = a_i.d, nb_j = n.b_j, nd = n.d and
+//   shr_I = H0_I + sum_j R_Ij,  str_I = (2/5) sum_j R_Ij nb_j,
+//   pr_I = nd shr_I - str_I,  Q_iI = sum_j R_Ij C_ij,  xu_I = ad_I/2 + A_II/4
+// the entries are ir_i - bB_j + C_ij D + nb_j g_i (pair_dl's notation) with
+//   D = sum_I pr_I,  bB_j = sum_I (C_Ij / 2) pr_I,
+//   ir_i = sum_I [nd Q_II / 5 - xu_I str_I + (ad_i + A_iI / 2) str_I - (2/5) nd Q_iI],
+//   g_i = sum_I [xu_I shr_I - Q_II / 5 - (ad_i + A_iI / 2) shr_I + (2/5) Q_iI]:
+// no moment accumulation per evaluation.  R is taken between the records'
+// points, as computed.  S: arithmetic type; G: type of the points and vertices.
+template <typename S, bool TAB, typename G>
+__device__ __forceinline__ void pair_dl_rule(const OpConst &op, const G *xp, const G *xa, const double *xc,
+                                             const G *yp, const G *ya, const double *yc, double2 loc[9],
+                                             const double2 *tab) {
+    const S dcx = (S)(xc[0] - yc[0]), dcy = (S)(xc[1] - yc[1]), dcz = (S)(xc[2] - yc[2]);
+    double nx, ny, nz;  // unit normal of the test cell, (v1 - v3) x (v2 - v3) direction
+    {
+        const double e1x = (double)xa[0] - xa[6], e1y = (double)xa[1] - xa[7], e1z = (double)xa[2] - xa[8];
+        const double e2x = (double)xa[3] - xa[6], e2y = (double)xa[4] - xa[7], e2z = (double)xa[5] - xa[8];
+        nx = e1y * e2z - e1z * e2y;
+        ny = e1z * e2x - e1x * e2z;
+        nz = e1x * e2y - e1y * e2x;
+        const double ninv = rsqrt64(nx * nx + ny * ny + nz * nz);
+        nx *= ninv;
+        ny *= ninv;
+        nz *= ninv;
+    }
+    const S n0 = (S)nx, n1 = (S)ny, n2 = (S)nz;
+    const S nd = fma(n2, dcz, fma(n1, dcy, n0 * dcx));
+    auto dot = [](const G *u, const G *v) -> S {
+        return fma((S)u[2], (S)v[2], fma((S)u[1], (S)v[1], (S)u[0] * (S)v[0]));
+    };
+    S C[3][3], A[3][3], ad[3], nb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C[i][j] = dot(xa + 3 * i, ya + 3 * j);
+#pragma unroll
+        for (int I = 0; I <= i; ++I) A[i][I] = A[I][i] = dot(xa + 3 * i, xa + 3 * I);
+        ad[i] = fma((S)xa[3 * i + 2], dcz, fma((S)xa[3 * i + 1], dcy, (S)xa[3 * i] * dcx));
+        nb[i] = fma(n2, (S)ya[3 * i + 2], fma(n1, (S)ya[3 * i + 1], n0 * (S)ya[3 * i]));
+    }
+    S Dr = 0, Di = 0, irr[3] = {}, iri[3] = {}, gr[3] = {}, gi[3] = {}, bBr[3] = {}, bBi[3] = {};
+#pragma unroll
+    for (int I = 0; I < 3; ++I) {
+        const S ux = dcx + (S)xp[3 * I], uy = dcy + (S)xp[3 * I + 1], uz = dcz + (S)xp[3 * I + 2];
+        S h0r, h0i, rr[3], ri[3];
+#pragma unroll
+        for (int J = 0; J < 4; ++J) {
+            const S w = (S)(frule_w(BMI_FAST_TEST_ORDER, I) * frule_w(BMI_FAST_TRIAL_ORDER, J) /
+                            (4.0 * 3.14159265358979323846));
+            S hr, hi;
+            if constexpr (sizeof(S) == 8) {
+                green_grad64<TAB>(op, false, w, ux - (S)yp[3 * J], uy - (S)yp[3 * J + 1], uz - (S)yp[3 * J + 2], hr,
+                                  hi, tab);
+            } else {
+                green_grad32(op, false, w, ux - (S)yp[3 * J], uy - (S)yp[3 * J + 1], uz - (S)yp[3 * J + 2], hr,
+                             hi);
+            }
+            if (J == 0) {
+                h0r = hr;
+                h0i = hi;
+            } else {
+                rr[J - 1] = hr;
+                ri[J - 1] = hi;
+            }
+        }
+        const S shr = h0r + (rr[0] + rr[1] + rr[2]), shi = h0i + (ri[0] + ri[1] + ri[2]);
+        const S str = (S)0.4 * fma(rr[2], nb[2], fma(rr[1], nb[1], rr[0] * nb[0]));
+        const S sti = (S)0.4 * fma(ri[2], nb[2], fma(ri[1], nb[1], ri[0] * nb[0]));
+        const S pr = fma(nd, shr, -str), pi = fma(nd, shi, -sti);
+        S qr[3], qi[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            qr[i] = fma(rr[2], C[i][2], fma(rr[1], C[i][1], rr[0] * C[i][0]));
+            qi[i] = fma(ri[2], C[i][2], fma(ri[1], C[i][1], ri[0] * C[i][0]));
+        }
+        const S xu = fma((S)0.25, A[I][I], (S)0.5 * ad[I]);
+        // the parts that do not depend on i: nd Q_II / 5 - xu str and xu shr - Q_II / 5
+        const S er = fma((S)0.2 * nd, qr[I], -xu * str), ei = fma((S)0.2 * nd, qi[I], -xu * sti);
+        const S fr = fma(xu, shr, (S)-0.2 * qr[I]), fi = fma(xu, shi, (S)-0.2 * qi[I]);
+        Dr += pr;
+        Di += pi;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const S cf = fma((S)0.5, A[i][I], ad[i]);
+            irr[i] += fma(cf, str, fma((S)-0.4 * nd, qr[i], er));
+            iri[i] += fma(cf, sti, fma((S)-0.4 * nd, qi[i], ei));
+            gr[i] += fma(-cf, shr, fma((S)0.4, qr[i], fr));
+            gi[i] += fma(-cf, shi, fma((S)0.4, qi[i], fi));
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            bBr[j] = fma((S)0.5 * C[I][j], pr, bBr[j]);
+            bBi[j] = fma((S)0.5 * C[I][j], pi, bBi[j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            loc[3 * i + j].x = (double)fma(nb[j], gr[i], fma(C[i][j], Dr, irr[i] - bBr[j]));
+            loc[3 * i + j].y = (double)fma(nb[j], gi[i], fma(C[i][j], Di, iri[i] - bBi[j]));
+        }
+}
+
 template <typename S, int FAST, bool TAB, typename G = double>
 __device__ __forceinline__ void pair_dl(const OpConst &op, const RuleConst &rule_, const G *xp, const G *xa,
                                         const double *xc, const G *yp, const G *ya, const double *yc,
                                         double2 loc[9], const double2 *tab) {
+    if constexpr (FAST) {
+        pair_dl_rule<S, TAB, G>(op, xp, xa, xc, yp, ya, yc, loc, tab);
+        return;
+    }
     const FarRule<FAST> rule{rule_, op};
     const bool lossy = rule.lossy();
     const S dcx = (S)(xc[0] - yc[0]), dcy = (S)(xc[1] - yc[1]), dcz = (S)(xc[2] - yc[2]);
