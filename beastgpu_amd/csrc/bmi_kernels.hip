@@ -216,6 +216,28 @@ __host__ __device__ constexpr double frule_l(int o, int i, int d) {
 #define BMI_FAST_TEST_ORDER 2
 #define BMI_FAST_TRIAL_ORDER 3
 
+// Far kernels with the compile-time rule accumulate the entries divided by
+// alpha * fast_scale and multiply by it at the write-back.  The mixed EFIE
+// pair (pair_mw_rule in FP32) leaves out the weight factor that all of the
+// rule's point pairs share, w_I w_J / (4 pi) = (1/6)(25/96)/(4 pi) times 1
+// for the trial vertices' points and -27/25 for the centroid, so that an
+// evaluation does not multiply by its weight (2-3 % of the mixed EFIE far
+// phase; the mixed double layer measured slower so, and keeps its weights).
+template <int OP, int MIXED, int FAST>
+__host__ __device__ constexpr double fast_scale() {
+    return (OP == OPK_MW && MIXED && FAST)
+               ? frule_w(BMI_FAST_TEST_ORDER, 0) * frule_w(BMI_FAST_TRIAL_ORDER, 1) / (4.0 * 3.14159265358979323846)
+               : 1.0;
+}
+// the weight of point pair (I, J) of the compile-time rule: w_I w_J / (4 pi),
+// or (FOLD) relative to fast_scale's factor
+template <bool FOLD>
+__host__ __device__ constexpr double fast_weight(int I, int J) {
+    return !FOLD ? frule_w(BMI_FAST_TEST_ORDER, I) * frule_w(BMI_FAST_TRIAL_ORDER, J) /
+                                (4.0 * 3.14159265358979323846)
+                          : (J == 0 ? frule_w(BMI_FAST_TRIAL_ORDER, 0) / frule_w(BMI_FAST_TRIAL_ORDER, 1) : 1.0);
+}
+
 // Far rule seen by the pair integrators: FAST = 1 is the default 3 x 4 point
 // rule of a lossless operator, fixed at compile time (straight-line code, no
 // rule loads); FAST = 0 reads the rule and the attenuation flag at run time.
@@ -350,8 +372,7 @@ __device__ __forceinline__ void pair_dl_rule(const OpConst &op, const G *xp, con
         S h0r, h0i, rr[3], ri[3];
 #pragma unroll
         for (int J = 0; J < 4; ++J) {
-            const S w = (S)(frule_w(BMI_FAST_TEST_ORDER, I) * frule_w(BMI_FAST_TRIAL_ORDER, J) /
-                            (4.0 * 3.14159265358979323846));
+            const S w = (S)fast_weight<false>(I, J);
             S hr, hi;
             if constexpr (sizeof(S) == 8) {
                 green_grad64<TAB>(op, false, w, ux - (S)yp[3 * J], uy - (S)yp[3 * J + 1], uz - (S)yp[3 * J + 2], hr,
@@ -652,7 +673,8 @@ __device__ __forceinline__ void green32(const OpConst &op, bool lossy, float w4p
     hi = -h * s;
 }
 
-// EFIE far pair with the default 3 x 4 point rules, entries divided by alpha.
+// EFIE far pair with the default 3 x 4 point rules, entries divided by alpha
+// (FP32: by alpha * fast_scale).
 // Relative to the centroids the rules' points are fixed fractions of the
 // vertices a_i = v_i - c_K, b_j = v_j - c_L: test point I (barycentric 2/3 at
 // vertex I) is a_I / 2, trial point 0 is the centroid and trial point J
@@ -697,8 +719,7 @@ __device__ __forceinline__ void pair_mw_rule(const OpConst &op, const G *xp, con
         S tr = 0, ti = 0;
 #pragma unroll
         for (int J = 0; J < 4; ++J) {
-            const S w = (S)(frule_w(BMI_FAST_TEST_ORDER, I) * frule_w(BMI_FAST_TRIAL_ORDER, J) /
-                            (4.0 * 3.14159265358979323846));
+            const S w = (S)fast_weight<sizeof(S) == 4>(I, J);
             S hr, hi;
             if constexpr (sizeof(S) == 8) {
                 green64<TAB>(op, false, w, ux - (S)yp[3 * J], uy - (S)yp[3 * J + 1], uz - (S)yp[3 * J + 2], hr, hi,
@@ -2177,7 +2198,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
         }
 
         // phase 2 (blocks next to the diagonal only); FAST: the pair values are
-        // divided by alpha, so are the special ones
+        // divided by alpha * fast_scale, so are the special ones
         if (e1 > e0) {
             __syncthreads();
             for (int e = e0 + t; e < e1; e += BMI_BLOCK_THREADS) {
@@ -2185,8 +2206,11 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
 #pragma unroll
                 for (int v = 0; v < 9; ++v) {
                     double2 z = a.spec_val[9 * (long long)e + v];
-                    if constexpr (FAST) z = make_double2(fma(a.op.ia_re, z.x, -a.op.ia_im * z.y),
-                                                         fma(a.op.ia_re, z.y, a.op.ia_im * z.x));
+                    if constexpr (FAST) {
+                        constexpr double is = 1.0 / fast_scale<OP, MIXED, FAST>();
+                        z = make_double2(is * fma(a.op.ia_re, z.x, -a.op.ia_im * z.y),
+                                         is * fma(a.op.ia_re, z.y, a.op.ia_im * z.x));
+                    }
                     V[v * BMI_BLOCK_THREADS + lo] = z;
                 }
             }
@@ -2233,8 +2257,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
                 const long long off = (long long)(r0 + myr) + (long long)s_cfn[c] * a.ld;
                 double vr = accr[j], vi = acci[j];
                 if constexpr (FAST) {
-                    vr = fma(a.op.alpha_re, accr[j], -a.op.alpha_im * acci[j]);
-                    vi = fma(a.op.alpha_re, acci[j], a.op.alpha_im * accr[j]);
+                    constexpr double ks = fast_scale<OP, MIXED, FAST>();
+                    vr = fma(ks * a.op.alpha_re, accr[j], -(ks * a.op.alpha_im) * acci[j]);
+                    vi = fma(ks * a.op.alpha_re, acci[j], (ks * a.op.alpha_im) * accr[j]);
                 }
                 out_write(Z, off, vr, vi, s_cfirst[c]);
             }
@@ -2413,10 +2438,14 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
 #pragma unroll
                 for (int v = 0; v < 9; ++v) {
                     const double2 z = a.spec_val[9 * (long long)(e0 + o) + v];
-                    // FAST: the tile holds the entries divided by alpha
-                    if constexpr (FAST) loc[v] = make_double2(fma(a.op.ia_re, z.x, -a.op.ia_im * z.y),
-                                                              fma(a.op.ia_re, z.y, a.op.ia_im * z.x));
-                    else loc[v] = z;
+                    // FAST: the tile holds the entries divided by alpha * fast_scale
+                    if constexpr (FAST) {
+                        constexpr double is = 1.0 / fast_scale<OP, MIXED, FAST>();
+                        loc[v] = make_double2(is * fma(a.op.ia_re, z.x, -a.op.ia_im * z.y),
+                                              is * fma(a.op.ia_re, z.y, a.op.ia_im * z.x));
+                    } else {
+                        loc[v] = z;
+                    }
                 }
                 s_ent[t] = -1;
                 live = true;
@@ -2531,9 +2560,11 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
             const int r = 64 * h + (lane ^ (c & 15));
             if (c < nc && r < nr) {
                 double2 v = tile[c * PR + 64 * h + lane];
-                if constexpr (FAST)  // the tile holds the entries divided by alpha
-                    v = make_double2(fma(a.op.alpha_re, v.x, -a.op.alpha_im * v.y),
-                                     fma(a.op.alpha_re, v.y, a.op.alpha_im * v.x));
+                if constexpr (FAST) {  // the tile holds the entries divided by alpha * fast_scale
+                    constexpr double ks = fast_scale<OP, MIXED, FAST>();
+                    const double ar = ks * a.op.alpha_re, ai = ks * a.op.alpha_im;
+                    v = make_double2(fma(ar, v.x, -ai * v.y), fma(ar, v.y, ai * v.x));
+                }
                 OutT o;
                 o.x = (decltype(o.x))((double)old[NH * k + h].x + v.x);
                 o.y = (decltype(o.y))((double)old[NH * k + h].y + v.y);
