@@ -1381,6 +1381,43 @@ __device__ __forceinline__ void ss_map(int cls, int r, double e1, double e2, dou
     }
 }
 
+// The region maps of ss_map are linear in the radial variable xi and put the
+// singular point at xi = 0, chart point (1, 0) on both cells:
+//   u1 = 1 + xi b1,  u2 = xi b2,  v1 = 1 + xi d1,  v2 = xi d2,
+// with the Jacobian xi^3 f (times the product weight).  ss_dir gives the
+// directions (b1, b2, d1, d2) and f of region r of class cls at (e1, e2, e3),
+// the same maps written as such.
+__host__ __device__ __forceinline__ void ss_dir(int cls, int r, double e1, double e2, double e3, double &b1,
+                                                double &b2, double &d1, double &d2, double &f) {
+    if (cls == CLS_CV) {
+        f = e2;
+        const double e23 = e2 * e3;
+        if (r == 0) { b1 = -1.0; b2 = e1; d1 = -e2; d2 = e23; }
+        else { b1 = -e2; b2 = e23; d1 = -1.0; d2 = e1; }
+    } else if (cls == CLS_CE) {
+        const double e12 = e1 * e2, e23 = e2 * e3, e123 = e12 * e3;
+        f = e1 * e1 * (r == 4 ? 1.0 : e2);
+        switch (r) {
+        case 0: b1 = -1.0; b2 = e1; d1 = e123 - 1.0; d2 = e12 * (1.0 - e3); break;
+        case 1: b1 = e12 - 1.0; b2 = e1 * (1.0 - e2); d1 = -1.0; d2 = e1 * e23; break;
+        case 2: b1 = e123 - 1.0; b2 = e12 * (1.0 - e3); d1 = -1.0; d2 = e1; break;
+        case 3: b1 = e123 - 1.0; b2 = e1 * (1.0 - e23); d1 = -1.0; d2 = e12; break;
+        default: b1 = -1.0; b2 = e1 * e3; d1 = e12 - 1.0; d2 = e1 * (1.0 - e2); break;
+        }
+    } else {
+        const double e12 = e1 * e2, e123 = e12 * e3;
+        f = e1 * e1 * e2;
+        switch (r) {
+        case 0: b1 = -1.0; b2 = 1.0 - e1 + e12; d1 = e123 - 1.0; d2 = 1.0 - e1; break;
+        case 1: b1 = e123 - 1.0; b2 = 1.0 - e1; d1 = -1.0; d2 = 1.0 - e1 + e12; break;
+        case 2: b1 = -1.0; b2 = e1 * (1.0 - e2 + e2 * e3); d1 = e12 - 1.0; d2 = e1 * (1.0 - e2); break;
+        case 3: b1 = e12 - 1.0; b2 = e1 * (1.0 - e2); d1 = -1.0; d2 = e1 * (1.0 - e2 + e2 * e3); break;
+        case 4: b1 = e123 - 1.0; b2 = e1 * (1.0 - e2 * e3); d1 = -1.0; d2 = e1 * (1.0 - e2); break;
+        default: b1 = -1.0; b2 = e1 * (1.0 - e2); d1 = e123 - 1.0; d2 = e1 * (1.0 - e2 * e3); break;
+        }
+    }
+}
+
 // Closed-form integrals over the flat trial triangle w[0..2] (unit normal n)
 // at the point x, R = |x - y|, xp the projection of x onto its plane:
 //   I0 = int 1/R,  I1 = int (y - xp)/R,  K1 = int R,  K1v = int (y - xp) R.
@@ -1561,9 +1598,11 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
     const int lane = t & 63, wv = t >> 6;
     const int ns = min(*a.n_special, a.capacity);
     const bool lossy = a.op.lossy != 0;
-    // compile-time order: node table over (i2, i3, i4), entry u = (x[i2],
-    // x[i3], x[i4], w[i2] w[i3] w[i4]) for u < NG^3, padded to whole waves
-    // with the last node at weight zero
+    // compile-time order: node table of three Gauss-Legendre indices, entry
+    // u = (x[i], x[j], x[k], w[i] w[j] w[k]) for u = (i NG + j) NG + k < NG^3,
+    // padded to whole waves with the last node at weight zero; (i, j, k) are
+    // (e1, e2, e3) in the single layers' radial form and (e2, e3, xi) in the
+    // double layer's loop
     constexpr int N3 = NG * NG * NG, NH3 = NG ? (N3 + 63) / 64 : 1;
     if constexpr (NG != 0) {
         for (int u = t; u < 64 * NH3; u += BMI_BLOCK_THREADS) {
@@ -1698,15 +1737,133 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                 spec_point<OP, SUB>(a.op, lossy, s_sc, acc, X, Y, dc, s_nb1[I], s_nb2[I], s_nb1[J], s_nb2[J],
                                     s_nw[I] * s_nw[J], nrm);
             }
+        } else if constexpr (NG != 0 && OP != OPK_DL) {
+            // compile-time order, single layers: radial form.  Every region
+            // map is linear in xi with x - y = xi B at the node (ss_dir: the
+            // singular point is the same physical point on both charts), so
+            // R = xi |B|, and x = P + xi Q, y = P' + xi Q' with P, P' that
+            // point relative to the centroids.  Lanes over the (e1, e2, e3)
+            // node table; per lane one 1/|B| and the xi nodes (uniform over
+            // the wave) in a loop that evaluates only the phase: with
+            // S_n = sum over xi of h xi^n the lane adds M0 += S_0 and
+            // (Maxwell) Q S_1, Q' S_1, (Q.Q') S_2, or (Helmholtz) the shape
+            // products of f = F0 + xi F1, g = G0 + xi G1; the P, P' terms
+            // are added after the reduction.  The Jacobian xi^3 and 1/R give
+            // the node weight w xi^2 / |B|.
+            auto region = [&](auto cls_c, auto r_c) {
+                constexpr int C = decltype(cls_c)::value, RR = decltype(r_c)::value;
+#pragma unroll 1
+                for (int h0 = 0; h0 < NH3; ++h0) {
+                    // (e1, e2, e3, w1 w2 w3), weight zero past the table
+                    const double4 nd = s_node[lane + 64 * h0];
+                    double b1, b2, d1, d2, f;
+                    ss_dir(C, RR, nd.x, nd.y, nd.z, b1, b2, d1, d2, f);
+                    double qx[3], qy[3], bb = 0.0;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        qx[d] = fma(X.t1[d], b1, X.t2[d] * b2);
+                        qy[d] = fma(Y.t1[d], d1, Y.t2[d] * d2);
+                        const double bd = qx[d] - qy[d];
+                        bb = fma(bd, bd, bb);
+                    }
+                    const double rb = rsqrt64(bb);
+                    const double nb = bb * rb;
+                    const double u64 = nb * (64.0 * a.op.k2pi);
+                    const double mr = nd.w * f * rb * (1.0 / (4.0 * M_PI));
+                    double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
+#pragma unroll
+                    for (int i4 = 0; i4 < NG; ++i4) {
+                        const double xi = a.glx[0][i4];
+                        double sn, cs;
+                        sincos_64ths_tab(u64 * xi, s_sc, sn, cs);
+                        double hm = mr * (a.glw[0][i4] * xi * xi);
+                        if (lossy) hm *= exp(-a.op.gamma_re * (nb * xi));
+                        const double hr = hm * cs, hi = -(hm * sn);
+                        s0r += hr;
+                        s0i += hi;
+                        s1r = fma(hr, xi, s1r);
+                        s1i = fma(hi, xi, s1i);
+                        s2r = fma(hr, xi * xi, s2r);
+                        s2i = fma(hi, xi * xi, s2i);
+                    }
+                    if constexpr (OP == OPK_MW) {
+                        acc.v[0] += s0r;
+                        acc.v[1] += s0i;
+                        double qq = 0.0;
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            acc.v[2 + d] = fma(qx[d], s1r, acc.v[2 + d]);
+                            acc.v[5 + d] = fma(qx[d], s1i, acc.v[5 + d]);
+                            acc.v[8 + d] = fma(qy[d], s1r, acc.v[8 + d]);
+                            acc.v[11 + d] = fma(qy[d], s1i, acc.v[11 + d]);
+                            qq = fma(qx[d], qy[d], qq);
+                        }
+                        acc.v[14] = fma(qq, s2r, acc.v[14]);
+                        acc.v[15] = fma(qq, s2i, acc.v[15]);
+                    } else {
+                        // chart barycentrics f = (u1, u2, 1 - u1 - u2) = F0 + xi F1
+                        // with F0 = (1, 0, 0), F1 = (b1, b2, -b1 - b2); g alike
+                        const double F1[3] = {b1, b2, -(b1 + b2)}, G1[3] = {d1, d2, -(d1 + d2)};
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int j = 0; j < 3; ++j) {
+                                const double c2 = F1[i] * G1[j];
+                                double &ar = acc.v[2 * (3 * i + j)], &ai = acc.v[2 * (3 * i + j) + 1];
+                                ar = fma(c2, s2r, ar);
+                                ai = fma(c2, s2i, ai);
+                                if (i == 0 || j == 0) {
+                                    const double c1 = (i == 0 ? G1[j] : 0.0) + (j == 0 ? F1[i] : 0.0);
+                                    ar = fma(c1, s1r, ar);
+                                    ai = fma(c1, s1i, ai);
+                                }
+                                if (i == 0 && j == 0) {
+                                    ar += s0r;
+                                    ai += s0i;
+                                }
+                            }
+                    }
+                }
+            };
+            using I0 = std::integral_constant<int, 0>;
+            using I1 = std::integral_constant<int, 1>;
+            using I2 = std::integral_constant<int, 2>;
+            using I3 = std::integral_constant<int, 3>;
+            using I4 = std::integral_constant<int, 4>;
+            using I5 = std::integral_constant<int, 5>;
+            using CCV = std::integral_constant<int, CLS_CV>;
+            using CCE = std::integral_constant<int, CLS_CE>;
+            using CCF = std::integral_constant<int, CLS_CF>;
+            if (cls == CLS_CV) {
+                region(CCV{}, I0{});
+                region(CCV{}, I1{});
+            } else if (cls == CLS_CE) {
+                region(CCE{}, I0{});
+                region(CCE{}, I1{});
+                region(CCE{}, I2{});
+                region(CCE{}, I3{});
+                region(CCE{}, I4{});
+            } else if (a.sym) {
+                // regions 2m + 1 are the x <-> y swaps of regions 2m: added
+                // after the reduction
+                region(CCF{}, I0{});
+                region(CCF{}, I2{});
+                region(CCF{}, I4{});
+            } else {
+                region(CCF{}, I0{});
+                region(CCF{}, I1{});
+                region(CCF{}, I2{});
+                region(CCF{}, I3{});
+                region(CCF{}, I4{});
+                region(CCF{}, I5{});
+            }
         } else if constexpr (NG != 0) {
-            // compile-time order (every class the same Gauss-Legendre rule):
-            // lanes over the (i2, i3, i4) node table, the region and i1
-            // uniform over the wave; the Helmholtz kernel takes both halves of
-            // the table in one iteration (two independent evaluations; the
-            // other operators' registers are better spent on one)
-            // Each (class, region) loop is its own code, the region map
-            // folded in at compile time (no per-node dispatch).
-            constexpr int U = OP == OPK_HH3D ? NH3 : 1;
+            // compile-time order, double layer (every class the same
+            // Gauss-Legendre rule): lanes over the (i2, i3, i4) node table,
+            // the region and i1 uniform over the wave.  Each (class, region)
+            // loop is its own code, the region map folded in at compile time
+            // (no per-node dispatch).
+            constexpr int U = 1;
             auto region = [&](auto cls_c, auto r_c, auto pair_c) {
                 constexpr int C = decltype(cls_c)::value, RR = decltype(r_c)::value;
                 constexpr bool PR = decltype(pair_c)::value;
@@ -1744,12 +1901,6 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                 region(CCE{}, I2{}, NP{});
                 region(CCE{}, I3{}, NP{});
                 region(CCE{}, I4{}, NP{});
-            } else if (OP != OPK_DL && a.sym) {
-                // regions 2m + 1 are the x <-> y swaps of regions 2m
-                using PP = std::true_type;
-                region(CCF{}, I0{}, PP{});
-                region(CCF{}, I2{}, PP{});
-                region(CCF{}, I4{}, PP{});
             } else {
                 region(CCF{}, I0{}, NP{});
                 region(CCF{}, I1{}, NP{});
@@ -1802,7 +1953,48 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
 #pragma unroll
             for (int i = 0; i < spec_nacc(OP); ++i) acc.v[i] += __shfl_xor(acc.v[i], o);
         }
-        if (OP == OPK_MW && !NEARK && cls == CLS_CF && a.sym) {
+        if constexpr (NG != 0 && !NEARK && OP == OPK_MW) {
+            // radial form: the moments of x = P + xi Q and y = P' + xi Q'
+            // from the lane sums (S_0, Q S_1, Q' S_1, (Q.Q') S_2), P and P'
+            // the singular point relative to the centroids; paired common
+            // faces add the x <-> y swaps: M0 and Mxy twice, Mx = My = Mx + My
+            const bool pr = cls == CLS_CF && a.sym;
+            const double a0r = acc.v[0], a0i = acc.v[1];
+            double xyr = acc.v[14], xyi = acc.v[15];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double px = X.o[d] + X.t1[d], py = Y.o[d] + Y.t1[d];
+                const double axr = acc.v[2 + d], axi = acc.v[5 + d], ayr = acc.v[8 + d], ayi = acc.v[11 + d];
+                xyr = fma(px, fma(py, a0r, ayr), fma(py, axr, xyr));
+                xyi = fma(px, fma(py, a0i, ayi), fma(py, axi, xyi));
+                const double mxr = fma(px, a0r, axr), mxi = fma(px, a0i, axi);
+                const double myr = fma(py, a0r, ayr), myi = fma(py, a0i, ayi);
+                acc.v[2 + d] = pr ? mxr + myr : mxr;
+                acc.v[5 + d] = pr ? mxi + myi : mxi;
+                acc.v[8 + d] = pr ? mxr + myr : myr;
+                acc.v[11 + d] = pr ? mxi + myi : myi;
+            }
+            const double sc = pr ? 2.0 : 1.0;
+            acc.v[0] = sc * a0r;
+            acc.v[1] = sc * a0i;
+            acc.v[14] = sc * xyr;
+            acc.v[15] = sc * xyi;
+        } else if constexpr (NG != 0 && !NEARK && OP == OPK_HH3D) {
+            // radial form, paired common faces: the swaps add the transposed
+            // shape products, f_i g_j + g_i f_j
+            if (cls == CLS_CF && a.sym) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = i; j < 3; ++j)
+#pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+                            const double v = acc.v[2 * (3 * i + j) + c] + acc.v[2 * (3 * j + i) + c];
+                            acc.v[2 * (3 * i + j) + c] = v;
+                            acc.v[2 * (3 * j + i) + c] = v;
+                        }
+            }
+        } else if (OP == OPK_MW && !NEARK && cls == CLS_CF && a.sym) {
 #pragma unroll
             for (int i = 0; i < 6; ++i) acc.v[8 + i] = acc.v[2 + i];  // paired regions: moments of y = those of x
         }
