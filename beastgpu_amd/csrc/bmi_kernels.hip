@@ -1601,8 +1601,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
     // compile-time order: node table of three Gauss-Legendre indices, entry
     // u = (x[i], x[j], x[k], w[i] w[j] w[k]) for u = (i NG + j) NG + k < NG^3,
     // padded to whole waves with the last node at weight zero; (i, j, k) are
-    // (e1, e2, e3) in the single layers' radial form and (e2, e3, xi) in the
-    // double layer's loop
+    // (e1, e2, e3) of the radial form
     constexpr int N3 = NG * NG * NG, NH3 = NG ? (N3 + 63) / 64 : 1;
     if constexpr (NG != 0) {
         for (int u = t; u < 64 * NH3; u += BMI_BLOCK_THREADS) {
@@ -1737,8 +1736,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                 spec_point<OP, SUB>(a.op, lossy, s_sc, acc, X, Y, dc, s_nb1[I], s_nb2[I], s_nb1[J], s_nb2[J],
                                     s_nw[I] * s_nw[J], nrm);
             }
-        } else if constexpr (NG != 0 && OP != OPK_DL) {
-            // compile-time order, single layers: radial form.  Every region
+        } else if constexpr (NG != 0) {
+            // compile-time order: radial form.  Every region
             // map is linear in xi with x - y = xi B at the node (ss_dir: the
             // singular point is the same physical point on both charts), so
             // R = xi |B|, and x = P + xi Q, y = P' + xi Q' with P, P' that
@@ -1749,7 +1748,19 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             // (Maxwell) Q S_1, Q' S_1, (Q.Q') S_2, or (Helmholtz) the shape
             // products of f = F0 + xi F1, g = G0 + xi G1; the P, P' terms
             // are added after the reduction.  The Jacobian xi^3 and 1/R give
-            // the node weight w xi^2 / |B|.
+            // the node weight w xi^2 / |B|.  Double layer: grad G = q r with
+            // q ~ 1/R^3, so the node weight is w / |B|^3 and every moment has
+            // a factor r = xi B: the lane adds the (n.r)(x.y), (n.y)(x.r),
+            // x.r, (n.y) r - (n.r) y, (n.r) x, r moments from S_1, S_2, S_3
+            // (their P, P' parts through M1 = sum h n.r after the reduction).
+            double Px[3], Py[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                Px[d] = X.o[d] + X.t1[d];
+                Py[d] = Y.o[d] + Y.t1[d];
+            }
+            const double ny0 = nrm[0] * Py[0] + nrm[1] * Py[1] + nrm[2] * Py[2];
+            const double c0 = Px[0] * Py[0] + Px[1] * Py[1] + Px[2] * Py[2];
             auto region = [&](auto cls_c, auto r_c) {
                 constexpr int C = decltype(cls_c)::value, RR = decltype(r_c)::value;
 #pragma unroll 1
@@ -1769,6 +1780,63 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                     const double rb = rsqrt64(bb);
                     const double nb = bb * rb;
                     const double u64 = nb * (64.0 * a.op.k2pi);
+                    if constexpr (OP == OPK_DL) {
+                        // h = -w (a + i kR) exp(-gamma R) / (4 pi |B|^3)
+                        const double mr = -(nd.w * f * rb * rb * rb * (1.0 / (4.0 * M_PI)));
+                        const double kb = nb * (2.0 * M_PI * a.op.k2pi);
+                        double s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0, s3r = 0.0, s3i = 0.0;
+#pragma unroll 1
+                        for (int i4 = 0; i4 < NG; ++i4) {
+                            const double xi = a.glx[0][i4];
+                            double sn, cs;
+                            sincos_64ths_tab(u64 * xi, s_sc, sn, cs);
+                            double hm = mr * a.glw[0][i4], ga = 1.0;
+                            if (lossy) {
+                                ga = fma(a.op.gamma_re, nb * xi, 1.0);
+                                hm *= exp(-a.op.gamma_re * (nb * xi));
+                            }
+                            const double kr = kb * xi;
+                            const double hr = hm * fma(ga, cs, kr * sn), hi = hm * fma(kr, cs, -(ga * sn));
+                            s1r = fma(hr, xi, s1r);
+                            s1i = fma(hi, xi, s1i);
+                            s2r = fma(hr, xi * xi, s2r);
+                            s2i = fma(hi, xi * xi, s2i);
+                            s3r = fma(hr, xi * xi * xi, s3r);
+                            s3i = fma(hi, xi * xi * xi, s3i);
+                        }
+                        double B[3], nB = 0.0, ny1 = 0.0, c1 = 0.0, c2 = 0.0, xr1 = 0.0, xr2 = 0.0;
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            B[d] = qx[d] - qy[d];
+                            nB = fma(nrm[d], B[d], nB);
+                            ny1 = fma(nrm[d], qy[d], ny1);
+                            c1 = fma(Px[d], qy[d], fma(qx[d], Py[d], c1));
+                            c2 = fma(qx[d], qy[d], c2);
+                            xr1 = fma(Px[d], B[d], xr1);
+                            xr2 = fma(qx[d], B[d], xr2);
+                        }
+                        const double k1 = fma(nB, c0, -ny0 * xr1);
+                        const double k2 = fma(nB, c1, -fma(ny0, xr2, ny1 * xr1));
+                        const double k3 = fma(nB, c2, -ny1 * xr2);
+                        acc.v[0] = fma(nB, s1r, acc.v[0]);
+                        acc.v[1] = fma(nB, s1i, acc.v[1]);
+                        acc.v[2] = fma(k1, s1r, fma(k2, s2r, fma(k3, s3r, acc.v[2])));
+                        acc.v[3] = fma(k1, s1i, fma(k2, s2i, fma(k3, s3i, acc.v[3])));
+                        acc.v[4] = fma(xr1, s1r, fma(xr2, s2r, acc.v[4]));
+                        acc.v[5] = fma(xr1, s1i, fma(xr2, s2i, acc.v[5]));
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            const double g2 = fma(ny1, B[d], -nB * qy[d]), g1 = ny0 * B[d];
+                            acc.v[6 + d] = fma(g1, s1r, fma(g2, s2r, acc.v[6 + d]));
+                            acc.v[9 + d] = fma(g1, s1i, fma(g2, s2i, acc.v[9 + d]));
+                            const double q4 = nB * qx[d];
+                            acc.v[12 + d] = fma(q4, s2r, acc.v[12 + d]);
+                            acc.v[15 + d] = fma(q4, s2i, acc.v[15 + d]);
+                            acc.v[18 + d] = fma(B[d], s1r, acc.v[18 + d]);
+                            acc.v[21 + d] = fma(B[d], s1i, acc.v[21 + d]);
+                        }
+                        continue;
+                    }
                     const double mr = nd.w * f * rb * (1.0 / (4.0 * M_PI));
                     double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0;
 #pragma unroll
@@ -1843,7 +1911,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                 region(CCE{}, I2{});
                 region(CCE{}, I3{});
                 region(CCE{}, I4{});
-            } else if (a.sym) {
+            } else if (OP != OPK_DL && a.sym) {
                 // regions 2m + 1 are the x <-> y swaps of regions 2m: added
                 // after the reduction
                 region(CCF{}, I0{});
@@ -1856,58 +1924,6 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                 region(CCF{}, I3{});
                 region(CCF{}, I4{});
                 region(CCF{}, I5{});
-            }
-        } else if constexpr (NG != 0) {
-            // compile-time order, double layer (every class the same
-            // Gauss-Legendre rule): lanes over the (i2, i3, i4) node table,
-            // the region and i1 uniform over the wave.  Each (class, region)
-            // loop is its own code, the region map folded in at compile time
-            // (no per-node dispatch).
-            constexpr int U = 1;
-            auto region = [&](auto cls_c, auto r_c, auto pair_c) {
-                constexpr int C = decltype(cls_c)::value, RR = decltype(r_c)::value;
-                constexpr bool PR = decltype(pair_c)::value;
-#pragma unroll 1
-                for (int i1 = 0; i1 < NG; ++i1) {
-                    const double e1 = s_glx[0][i1], w1 = s_glw[0][i1];
-#pragma unroll 1
-                    for (int h0 = 0; h0 < NH3; h0 += U) {
-#pragma unroll
-                        for (int h2 = 0; h2 < U; ++h2) {
-                            const double4 nd = s_node[lane + 64 * (h0 + h2)];
-                            double u1, u2, v1, v2, m;
-                            ss_map(C, RR, e1, nd.x, nd.y, nd.z, w1 * nd.w, u1, u2, v1, v2, m);
-                            spec_point<OP, false, PR>(a.op, lossy, s_sc, acc, X, Y, dc, u1, u2, v1, v2, m, nrm);
-                        }
-                    }
-                }
-            };
-            using I0 = std::integral_constant<int, 0>;
-            using I1 = std::integral_constant<int, 1>;
-            using I2 = std::integral_constant<int, 2>;
-            using I3 = std::integral_constant<int, 3>;
-            using I4 = std::integral_constant<int, 4>;
-            using I5 = std::integral_constant<int, 5>;
-            using CCV = std::integral_constant<int, CLS_CV>;
-            using CCE = std::integral_constant<int, CLS_CE>;
-            using CCF = std::integral_constant<int, CLS_CF>;
-            using NP = std::false_type;
-            if (cls == CLS_CV) {
-                region(CCV{}, I0{}, NP{});
-                region(CCV{}, I1{}, NP{});
-            } else if (cls == CLS_CE) {
-                region(CCE{}, I0{}, NP{});
-                region(CCE{}, I1{}, NP{});
-                region(CCE{}, I2{}, NP{});
-                region(CCE{}, I3{}, NP{});
-                region(CCE{}, I4{}, NP{});
-            } else {
-                region(CCF{}, I0{}, NP{});
-                region(CCF{}, I1{}, NP{});
-                region(CCF{}, I2{}, NP{});
-                region(CCF{}, I3{}, NP{});
-                region(CCF{}, I4{}, NP{});
-                region(CCF{}, I5{}, NP{});
             }
         } else {
             const int g = cls - 1;
@@ -1979,6 +1995,17 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             acc.v[1] = sc * a0i;
             acc.v[14] = sc * xyr;
             acc.v[15] = sc * xyi;
+        } else if constexpr (NG != 0 && !NEARK && OP == OPK_DL) {
+            // radial form: the P, P' parts of (n.y) r - (n.r) y and (n.r) x
+            // are -P' M1 and P M1
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double px = X.o[d] + X.t1[d], py = Y.o[d] + Y.t1[d];
+                acc.v[6 + d] = fma(-py, acc.v[0], acc.v[6 + d]);
+                acc.v[9 + d] = fma(-py, acc.v[1], acc.v[9 + d]);
+                acc.v[12 + d] = fma(px, acc.v[0], acc.v[12 + d]);
+                acc.v[15 + d] = fma(px, acc.v[1], acc.v[15 + d]);
+            }
         } else if constexpr (NG != 0 && !NEARK && OP == OPK_HH3D) {
             // radial form, paired common faces: the swaps add the transposed
             // shape products, f_i g_j + g_i f_j
