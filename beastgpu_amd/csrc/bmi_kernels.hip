@@ -1785,7 +1785,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                         const double mr = -(nd.w * f * rb * rb * rb * (1.0 / (4.0 * M_PI)));
                         const double kb = nb * (2.0 * M_PI * a.op.k2pi);
                         double s1r = 0.0, s1i = 0.0, s2r = 0.0, s2i = 0.0, s3r = 0.0, s3i = 0.0;
-#pragma unroll 1
+#pragma unroll
                         for (int i4 = 0; i4 < NG; ++i4) {
                             const double xi = a.glx[0][i4];
                             double sn, cs;
