@@ -1201,15 +1201,57 @@ hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob
 
 // ------------------------------------------------------- special pairs
 
-// Accumulator of one lane for a special pair: moment form for RWG (16 of
-// the 18 for the single layer, 24 for the double layer), 3x3 shape products
-// for P1.
-constexpr int spec_nacc(int op) { return op == OPK_DL ? 24 : 18; }
+// Accumulator of one lane for a special pair: moment form for RWG (16 for
+// the single layer, 24 for the double layer), 3x3 shape products for P1.
+constexpr int spec_nacc(int op) { return op == OPK_DL ? 24 : (op == OPK_MW ? 16 : 18); }
 template <int OP>
 struct SpecAcc {
     double v[spec_nacc(OP)];
 };
 
+
+// Sums of each of the N accumulators over the 64 lanes of the wave, returned
+// to every lane.  Groups of eight accumulators are reduced and scattered at
+// once: at the exchanges with lanes 32, 16 and 8 apart each lane keeps half
+// of its remaining values and sends the other half (4 + 2 + 1 exchanges), so
+// that lane l ends with value (l >> 3) & 7 summed over its eight lanes, then
+// three butterfly steps finish that sum, and one exchange per accumulator
+// hands it to every lane: eighteen exchanges per group instead of six per
+// accumulator.  The order of the additions is fixed, and both lanes
+// of an exchange add the same two numbers, so every lane that holds a sum
+// holds the same one.
+template <int N>
+__device__ __forceinline__ void wave_sum_all(double (&v)[N]) {
+    constexpr int G = (N + 7) / 8;
+    const int lane = threadIdx.x & 63;
+    double t[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        double w[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) w[j] = 8 * g + j < N ? v[8 * g + j] : 0.0;
+        // (three rounds written out: a loop with two counters kept the
+        // halves' indices at run time, and w in scratch memory)
+        auto halve = [&](auto o_c) {
+            constexpr int o = decltype(o_c)::value, h = o / 8;
+            const bool up = (lane & o) != 0;
+#pragma unroll
+            for (int j = 0; j < h; ++j) {
+                const double keep = up ? w[h + j] : w[j];
+                const double give = up ? w[j] : w[h + j];
+                w[j] = keep + __shfl_xor(give, o);
+            }
+        };
+        halve(std::integral_constant<int, 32>{});
+        halve(std::integral_constant<int, 16>{});
+        halve(std::integral_constant<int, 8>{});
+#pragma unroll
+        for (int o = 4; o >= 1; o >>= 1) w[0] += __shfl_xor(w[0], o);
+        t[g] = w[0];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = __shfl(t[i / 8], 8 * (i % 8));
+}
 
 // Entry (i, j) of the EFIE local matrix from the moments m[0..16) (M0, Mx,
 // My, Mxy as real/imaginary pairs in the order of the special-pair
@@ -1566,8 +1608,8 @@ __device__ __forceinline__ void near_static(const OpConst &op, SpecAcc<OP> &acc,
 // is integrated once, in the sub-batch the cell calls home.  Touching pairs:
 // regions in turn, lanes striding over the hypercube nodes, one Green
 // evaluation each.  Near pairs: closed-form singular terms, one outer point
-// per lane, then lanes striding over the product rule.  A butterfly over the
-// wave reduces the accumulators.  The two kinds are separate instantiations
+// per lane, then lanes striding over the product rule.  wave_sum_all
+// reduces the accumulators over the wave.  The two kinds are separate instantiations
 // so that the near-pair code does not add to the registers of the
 // Sauter-Schwab loop.
 // NG: Gauss-Legendre points per hypercube dimension fixed at compile time
@@ -1958,17 +2000,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                 }
             }
         }
-        // butterfly over the 64 lanes, all accumulators at each step: six
-        // dependent rounds of lane exchanges in all (one accumulator after
-        // another would chain six exchanges per accumulator, each waiting
-        // for the last); the exchanges use the LDS crossbar, not the VALU.
-        // Every lane then holds the sums; lane l < 9 forms entry l of the
-        // chart-order 3 x 3 matrix.
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-#pragma unroll
-            for (int i = 0; i < spec_nacc(OP); ++i) acc.v[i] += __shfl_xor(acc.v[i], o);
-        }
+        // the sums over the 64 lanes, to every lane (wave_sum_all); lane
+        // l < 9 then forms entry l of the chart-order 3 x 3 matrix
+        wave_sum_all(acc.v);
         if constexpr (NG != 0 && !NEARK && OP == OPK_MW) {
             // radial form: the moments of x = P + xi Q and y = P' + xi Q'
             // from the lane sums (S_0, Q S_1, Q' S_1, (Q.Q') S_2), P and P'
