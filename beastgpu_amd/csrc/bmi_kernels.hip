@@ -947,7 +947,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
     __shared__ int tt[BMI_PATCH], bt[CG][BMI_PATCH];
     __shared__ double trad[BMI_PATCH];  // test cell: largest vertex distance from its centroid
     __shared__ int nearq[BMI_BLOCK_THREADS];
-    __shared__ int wsum[BMI_BLOCK_THREADS / 64];
+    __shared__ int wsum[CG][BMI_BLOCK_THREADS / 64];
     __shared__ int s_off;
 
     const int grp = (int)blockIdx.x / split, part = (int)blockIdx.x - grp * split;
@@ -1013,13 +1013,13 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
             if (!cand) a.blk_cnt[(long long)p * a.npb + q] = 0;
         }
         const unsigned long long m = __ballot(cand);
-        if (lane == 0) wsum[wv] = __popcll(m);
+        if (lane == 0) wsum[0][wv] = __popcll(m);
         __syncthreads();
         int base = 0, ncand = 0;
 #pragma unroll
         for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
-            base += i < wv ? wsum[i] : 0;
-            ncand += wsum[i];
+            base += i < wv ? wsum[0][i] : 0;
+            ncand += wsum[0][i];
         }
         if (cand) nearq[base + __popcll(m & ((1ull << lane) - 1ull))] = q;
         __syncthreads();
@@ -1048,13 +1048,16 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
               barea[g][c] = bt[g][c] >= 0 ? a.Y.area[bt[g][c]] : 1.0;
           }
           __syncthreads();
-          for (int g = 0; g < ng; ++g) {
-            const int qq = nearq[u0 + g];
-            const long long b = (long long)p * a.npb + qq;
-            const int kl = t >> 4, ll = t & 15;
-            const int K = tt[kl], L = bt[g][ll];
+          // the group's blocks are classified first, then their entries get
+          // one range reserved with one atomic for the group (in block order)
+          const int kl = t >> 4, ll = t & 15;
+          const int K = tt[kl];
+          int gcls[CG];
+#pragma unroll
+          for (int g = 0; g < CG; ++g) {
+            const int L = bt[g][ll];
             int cls = -1;
-            if (K >= 0 && L >= 0) {
+            if (g < ng && K >= 0 && L >= 0) {
                 int hits = 0;
                 double dmin2 = 1.79769313486231570815e308;
 #pragma unroll
@@ -1081,41 +1084,58 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_classify(const ClassifyAr
                     }
                 }
             }
-            const int special = cls > 0;
-            const unsigned long long sm = __ballot(special);
-            const int before = __popcll(sm & ((1ull << lane) - 1ull));
-            __syncthreads();  // wsum reuse
-            if (lane == 0) wsum[wv] = __popcll(sm);
-            __syncthreads();
-            int sbase = 0, total = 0;
+            gcls[g] = cls;
+            const unsigned long long sm = __ballot(cls > 0);
+            if (lane == 0) wsum[g][wv] = __popcll(sm);
+          }
+          __syncthreads();
+          // block g's entries start at gstart[g] of the group's range; a
+          // wave's at sbase[g] within its block
+          int gstart[CG], sbase[CG], gtotal = 0;
+#pragma unroll
+          for (int g = 0; g < CG; ++g) {
+            int tot = 0, sb = 0;
 #pragma unroll
             for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) {
-                sbase += i < wv ? wsum[i] : 0;
-                total += wsum[i];
+                sb += i < wv ? wsum[g][i] : 0;
+                tot += wsum[g][i];
             }
-            // the block's entries go to a range reserved with one atomic: the
-            // ranges' order varies from run to run, their contents do not
-            if (t == 0) {
-                const int off = total > 0 ? atomicAdd(a.counter, total) : 0;
-                s_off = off;
-                a.blk_off[b] = off;
-                a.blk_cnt[b] = total;
-            }
-            __syncthreads();
-            if (special) {
-                const int e = s_off + sbase + before;
+            gstart[g] = gtotal;
+            sbase[g] = sb;
+            gtotal += g < ng ? tot : 0;
+          }
+          // the range's place varies from run to run, its contents do not
+          if (t == 0) s_off = gtotal > 0 ? atomicAdd(a.counter, gtotal) : 0;
+          __syncthreads();
+          const int off = s_off;
+          if (t < ng) {
+            const long long b = (long long)p * a.npb + nearq[u0 + t];
+            int tot = 0;
+#pragma unroll
+            for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) tot += wsum[t][i];
+            int st = 0;
+            for (int g = 0; g < t; ++g)
+#pragma unroll
+                for (int i = 0; i < BMI_BLOCK_THREADS / 64; ++i) st += wsum[g][i];
+            a.blk_off[b] = tot > 0 ? off + st : 0;
+            a.blk_cnt[b] = tot;
+          }
+#pragma unroll
+          for (int g = 0; g < CG; ++g) {
+            const int cls = gcls[g];
+            const unsigned long long sm = __ballot(cls > 0);
+            if (cls > 0) {
+                const int e = off + gstart[g] + sbase[g] + __popcll(sm & ((1ull << lane) - 1ull));
                 if (e < a.capacity) {
                     a.spec_K[e] = K;
-                    a.spec_L[e] = L;
+                    a.spec_L[e] = bt[g][ll];
                     // bit 7: the pair is integrated in the test cell's home group, not here
                     a.spec_cls[e] = (unsigned char)(cls | (a.X.home[K] == p ? 0 : SPEC_NOT_OWNER));
                     a.spec_loc[e] = (unsigned char)t;
                 }
             }
-            __syncthreads();  // wsum, s_off reused by the next block
           }
-          // bt, bv, bc, barea are refilled by the next group: the barrier
-          // above ends every thread's reads of this group
+          __syncthreads();  // wsum, s_off and the group's cells are reused by the next group
         }
     }
 }
@@ -2120,8 +2140,10 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
     }
 }
 
-// One thread per far pair that the mixed path keeps in FP64 (class 5), owner
-// entries only; thread u takes entry u - ebase[i] of problem i.
+// One thread per entry of a far pair that the mixed path keeps in FP64
+// (class 5), owners and the other entries of the pair alike (computing those
+// again costs less than k_link's search for their owner); thread u takes
+// entry u - ebase[i] of problem i.
 // FAST: the default 3 x 4 point rule of a lossless operator at compile time.
 template <int OP, int FAST>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *args, const int *ebase, int nprob) {
@@ -2137,7 +2159,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *
     if (u0 < ebase[nprob]) {
         const int p0 = find_problem(ebase, nprob, u0);
         const int e0 = u0 - ebase[p0];
-        mine = e0 < min(*args[p0].n_special, args[p0].capacity) && args[p0].spec_cls[e0] == CLS_FAR64;
+        mine = e0 < min(*args[p0].n_special, args[p0].capacity) &&
+               (args[p0].spec_cls[e0] & ~SPEC_NOT_OWNER) == CLS_FAR64;
     }
     const unsigned long long m = __ballot(mine);
     if (lane == 0) s_wsum[wv] = __popcll(m);
@@ -2194,7 +2217,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *
 // cell).  The source entry is found in its block by binary search: a block's
 // entries are in the order of their pair's thread kl * 16 + ll, kl the test
 // cell's place in its sub-batch and ll the trial cell's place in its patch.
-// A missing source (which the classifier rules out: both blocks classify the
+// Class-5 entries are all k_far64's.  A missing source (which the classifier rules out: both blocks classify the
 // same pair with the same arithmetic, and the vertex test is symmetric)
 // leaves NaN.
 __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *args, const int *ebase, int nprob) {
@@ -2206,6 +2229,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *a
     const int e = u - ebase[pi];
     if (e >= ns) return;
     const int raw = a.spec_cls[e], cls = raw & ~SPEC_NOT_OWNER;
+    if (cls == CLS_FAR64) return;  // k_far64 integrates every entry of the pair, owner or not
     if (cls == CLS_CF && a.cf_zero) {
 #pragma unroll
         for (int v = 0; v < 9; ++v) a.spec_val[9 * (long long)e + v] = make_double2(0.0, 0.0);
