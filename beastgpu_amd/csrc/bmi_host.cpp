@@ -303,6 +303,8 @@ struct bmi_space {
     // narrow patches)
     int prows = BMI_PANEL_ROWS, maxcols = BMI_MAX_COLS;
     int npanel = 0, nsb = 0;
+    std::vector<int> prow;  // npanel + 1: first row (function) of each panel
+    DevBuf<int> d_prow;
     std::vector<int> sb_off, sb_cells, sb_row_ptr;
     std::vector<unsigned char> sb_slot_kl;
     std::vector<double> sb_slot_cf, sb_sph, sb_amax;
@@ -514,7 +516,47 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
     const bmi_mesh *m = s->mesh;
     const int64_t nt = m->nt;
     const int R = s->prows;
-    const int np = (int)((s->nfun + R - 1) / R);
+    const int64_t nf = s->nfun;
+    // panel cuts: runs of at most R consecutive functions.  An RWG panel also
+    // ends before its cells would outnumber four sub-batches (a 64-row
+    // panel carries about 60 cells where the numbering is compact, more
+    // where it is not: the cut keeps its sub-batches full)
+    s->prow.assign(1, 0);
+    std::vector<int> pan_of((size_t)nf, 0);
+    {
+        const int cap = R == BMI_PANEL_ROWS ? 4 * BMI_PATCH : INT32_MAX;
+        std::vector<int> fptr((size_t)nf + 1, 0), fcell;
+        for (int64_t K = 0; K < nt; ++K)
+            for (int i = 0; i < 3; ++i)
+                if (gidx[3 * K + i] >= 0) fptr[(size_t)gidx[3 * K + i] + 1]++;
+        for (int64_t f = 0; f < nf; ++f) fptr[(size_t)f + 1] += fptr[(size_t)f];
+        fcell.resize((size_t)fptr[(size_t)nf]);
+        {
+            std::vector<int> pos(fptr.begin(), fptr.end() - 1);
+            for (int64_t K = 0; K < nt; ++K)
+                for (int i = 0; i < 3; ++i)
+                    if (gidx[3 * K + i] >= 0) fcell[(size_t)pos[(size_t)gidx[3 * K + i]]++] = (int)K;
+        }
+        std::vector<int> stamp((size_t)nt, -1);
+        int64_t r = 0;
+        int P = 0;
+        while (r < nf) {
+            const int64_t r_first = r;
+            int ncell = 0;
+            while (r < nf && r - r_first < R) {
+                int add = 0;
+                for (int u = fptr[(size_t)r]; u < fptr[(size_t)r + 1]; ++u) add += stamp[(size_t)fcell[(size_t)u]] != P;
+                if (r > r_first && ncell + add > cap) break;
+                for (int u = fptr[(size_t)r]; u < fptr[(size_t)r + 1]; ++u) stamp[(size_t)fcell[(size_t)u]] = P;
+                ncell += add;
+                pan_of[(size_t)r] = P;
+                ++r;
+            }
+            s->prow.push_back((int)r);
+            ++P;
+        }
+    }
+    const int np = (int)s->prow.size() - 1;
     s->npanel = np;
     std::vector<int> cnt(np + 1, 0);
     for (int64_t K = 0; K < nt; ++K) {
@@ -522,7 +564,7 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
         for (int i = 0; i < 3; ++i) {
             const int64_t f = gidx[3 * K + i];
             if (f < 0) continue;
-            const int P = (int)(f / R);
+            const int P = pan_of[(size_t)f];
             if (P != seen[0] && P != seen[1] && P != seen[2]) {
                 seen[i] = P;
                 cnt[P + 1]++;
@@ -536,7 +578,7 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
         for (int i = 0; i < 3; ++i) {
             const int64_t f = gidx[3 * K + i];
             if (f < 0) continue;
-            const int P = (int)(f / R);
+            const int P = pan_of[(size_t)f];
             if (P != seen[0] && P != seen[1] && P != seen[2]) {
                 seen[i] = P;
                 cells[pos[P]++] = (int)K;
@@ -575,8 +617,8 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
                 std::array<uint64_t, 2> rows = {0, 0};
                 for (int j = 0; j < 3; ++j) {
                     const int64_t f = gidx[3 * (int64_t)K + j];
-                    if (f >= 0 && f / R == P) {
-                        const int r = (int)(f - (int64_t)P * R);
+                    if (f >= 0 && pan_of[(size_t)f] == P) {
+                        const int r = (int)(f - s->prow[P]);
                         rows[r >> 6] |= 1ull << (r & 63);
                     }
                 }
@@ -604,8 +646,8 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
                 if (K < 0) continue;
                 for (int i = 0; i < 3; ++i) {
                     const int64_t f = gidx[3 * (int64_t)K + i];
-                    if (f < 0 || f / R != P) continue;
-                    sl.push_back({(int)(f - (int64_t)P * R), 3 * k + i, coeff[3 * (int64_t)K + i]});
+                    if (f < 0 || pan_of[(size_t)f] != P) continue;
+                    sl.push_back({(int)(f - s->prow[P]), 3 * k + i, coeff[3 * (int64_t)K + i]});
                 }
             }
             std::stable_sort(sl.begin(), sl.end(), [](const Slot &x, const Slot &y) {
@@ -673,8 +715,8 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
             if (K < 0) continue;
             for (int i = 0; i < 3; ++i) {
                 const int64_t f = gidx[3 * (int64_t)K + i];
-                if (f < 0 || f / R != P) continue;
-                const int r = (int)(f - (int64_t)P * R);
+                if (f < 0 || pan_of[(size_t)f] != P) continue;
+                const int r = (int)(f - s->prow[P]);
                 const int rank = seen[r]++;
                 if (rank > 255) too_many_cells = 1;
                 tslot[(size_t)b * BMI_MAX_ROWS + 3 * k + i] = r | ((rank & 0xff) << 8);
@@ -697,6 +739,8 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
     hipError_t e;
     if ((e = s->d_tslot.upload(tslot)) != hipSuccess || (e = s->d_tscf.upload(tscf)) != hipSuccess ||
         (e = s->d_home_sb.upload(home)) != hipSuccess)
+        return fail(BMI_EDEVICE, std::string("panel upload: ") + hipGetErrorString(e));
+    if ((e = s->d_prow.upload(s->prow)) != hipSuccess)
         return fail(BMI_EDEVICE, std::string("panel upload: ") + hipGetErrorString(e));
     if ((e = s->d_sb_off.upload(s->sb_off)) != hipSuccess || (e = s->d_sb_cells.upload(s->sb_cells)) != hipSuccess ||
         (e = s->d_sb_sph.upload(s->sb_sph)) != hipSuccess || (e = s->d_sb_amax.upload(s->sb_amax)) != hipSuccess ||
@@ -1151,6 +1195,7 @@ PanelArgs panel_args(const bmi_plan *P, void *Z, long long ld) {
     B.npanel = P->X->npanel;
     B.panel0 = 0;
     B.nrows = (int)P->X->nfun;
+    B.prow = P->X->d_prow.p;
     B.sb_off = P->X->d_sb_off.p;
     B.tgeo = P->d_tgeo.p;
     B.trs = P->X->d_trs.p;
@@ -1876,7 +1921,11 @@ bmi_status bmi_assemble_to_host(bmi_plan *P, void *dst, int32_t dtype, int64_t d
     ppb = std::min<int64_t>(ppb, std::max<int64_t>(1, (npanel + 7) / 8));
     const int nblk = (int)((npanel + ppb - 1) / ppb);
     const int nslot = nblk > 1 ? 2 : 1;
-    const size_t slot_elems = (size_t)std::min<int64_t>(rows, ppb * PR) * (size_t)cols;
+    // the tallest block's rows (panels are at most PR rows)
+    int64_t max_rows = 0;
+    for (int64_t p0 = 0; p0 < npanel; p0 += ppb)
+        max_rows = std::max<int64_t>(max_rows, P->X->prow[std::min<int64_t>(npanel, p0 + ppb)] - P->X->prow[p0]);
+    const size_t slot_elems = (size_t)max_rows * (size_t)cols;
     DevBuf<unsigned char> slot[2];
     for (int i = 0; i < nslot; ++i) {
         hipError_t e = slot[i].alloc(slot_elems * es);
@@ -1895,8 +1944,8 @@ bmi_status bmi_assemble_to_host(bmi_plan *P, void *dst, int32_t dtype, int64_t d
     int launches = 0;
     auto block_rows = [&](int b, int64_t &r0, int64_t &nrb) {
         const int64_t p0 = (int64_t)b * ppb, p1 = std::min<int64_t>(npanel, p0 + ppb);
-        r0 = p0 * PR;
-        nrb = std::min<int64_t>(rows, p1 * PR) - r0;
+        r0 = P->X->prow[p0];
+        nrb = P->X->prow[p1] - r0;
     };
     auto issue_block = [&](int b) -> bmi_status {
         int64_t r0, nrb;

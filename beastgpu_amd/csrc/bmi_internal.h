@@ -85,9 +85,10 @@ struct RuleConst {
 static_assert((BMI_GEOF_VREL + 9 * BMI_PATCH) * sizeof(float) <= BMI_GEO_CEN * sizeof(double),
               "FP32 geometry overlaps the centroids");
 
-// Far-pair assembly.  Test side: panels of BMI_PANEL_ROWS consecutive
-// functions (BMI_PANEL_ROWS_WIDE for Lagrange spaces), each with the cells that carry them (cells of several panels are
-// repeated), cut into sub-batches of BMI_PATCH cells; per sub-batch a slot
+// Far-pair assembly.  Test side: panels of at most BMI_PANEL_ROWS consecutive
+// functions (BMI_PANEL_ROWS_WIDE for Lagrange spaces), each with the cells
+// that carry them (cells of several panels are repeated), cut into
+// sub-batches of BMI_PATCH cells; per sub-batch a slot
 // table: for each of the 64 rows the slots (local cell * 3 + shape,
 // coefficient) of that row's function.  Trial side: Morton patches of
 // BMI_PATCH cells with their columns (functions) and slot tables.  All
@@ -97,6 +98,7 @@ struct PanelArgs {
     int npanel;                  // panels of this launch: panel0 .. panel0 + npanel - 1
     int panel0;
     int nrows;                   // functions of the test space
+    const int *prow;             // npanel + 1: first row of each panel (at most its height apart)
     const int *sb_off;           // npanel + 1: sub-batches of each panel
     const double *tgeo;          // nsb x BMI_GEO
     const int *trs;              // nsb x (BMI_PANEL_ROWS + 1): slot range of each row (0..48); 64-row panels only

@@ -2396,8 +2396,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
     const int pl = w / a.nbq;  // panel of the launch's range
     const int P = a.panel0 + pl;
     const int q = a.bq_list[w - pl * a.nbq];
-    const int r0 = P * BMI_PANEL_ROWS;
-    const int nr = min(BMI_PANEL_ROWS, a.nrows - r0);
+    const int r0 = a.prow[P];
+    const int nr = a.prow[P + 1] - r0;
     const int nc = a.bnc[q];
     const int myr = t & (BMI_PANEL_ROWS - 1);
     const int myc = t >> 6;
@@ -2598,8 +2598,8 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     const double2 *s_sc = sh.s_sc;
     const int t = threadIdx.x;
     constexpr int PR = panel_rows(OP);
-    const int r0 = P * PR;
-    const int nr = min(PR, a.nrows - r0);
+    const int r0 = a.prow[P];
+    const int nr = a.prow[P + 1] - r0;  // at most PR
     const int nc = a.bnc[q];
     const int kl = t >> 4, ll = t & 15;
     const int sb0 = a.sb_off[P], sb1 = a.sb_off[P + 1];
