@@ -754,8 +754,9 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
 // Rows of a patch: the functions its cells carry, ascending.  Patches that
 // share a function get different colours (greedy); the first-touch flag of a
 // (patch, function) row is set on the patch with the smallest colour.
-bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff, std::vector<int> &morton_rank) {
-    const bmi_mesh *m = s->mesh;
+// Cells in Morton order of their centroids (ties by index): order[i] is the
+// i-th cell, rank its inverse.
+void morton_order(const bmi_mesh *m, std::vector<int> &order, std::vector<int> &morton_rank) {
     const int64_t nt = m->nt;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int64_t k = 0; k < nt; ++k)
@@ -777,12 +778,17 @@ bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff,
         key[k] = {spread3(c[0]) | spread3(c[1]) << 1 | spread3(c[2]) << 2, (int)k};
     }
     std::sort(key.begin(), key.end());
-    std::vector<int> order(nt);
+    order.assign(nt, 0);
     morton_rank.assign(nt, 0);
     for (int64_t i = 0; i < nt; ++i) {
         order[i] = key[i].second;
         morton_rank[order[i]] = (int)i;
     }
+}
+
+bmi_status build_patches(bmi_space *s, const int64_t *gidx, const double *coeff, const std::vector<int> &order) {
+    const bmi_mesh *m = s->mesh;
+    const int64_t nt = m->nt;
 
     // runs of at most BMI_PATCH consecutive cells carrying at most
     // s->maxcols functions (bounds the LDS tile of the far kernel)
@@ -1419,11 +1425,34 @@ bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, c
         s->prows = BMI_PANEL_ROWS_WIDE;
         s->maxcols = BMI_MAX_COLS_WIDE;
     }
-    std::vector<int> rank;
-    bmi_status st = build_patches(s.get(), gidx, coeff, rank);
+    // the test-role panels and the trial-role patches are built side by side
+    // (they share the Morton order and write disjoint parts of the space)
+    std::vector<int> order, rank;
+    morton_order(mesh, order, rank);
+    bmi_status st_pan = BMI_OK;
+    std::string msg_pan;
+    const hipStream_t stream = mesh->ctx->stream;
+    const int dev = mesh->ctx->device;
+    std::thread panels([&] {
+        try {
+            (void)hipSetDevice(dev);
+            WorkStream w(stream);
+            st_pan = build_panels(s.get(), gidx, coeff, rank);
+        } catch (...) {
+            st_pan = fail_exception();
+        }
+        if (st_pan != BMI_OK) msg_pan = bmi_last_error();
+    });
+    bmi_status st = BMI_OK;
+    try {
+        st = build_patches(s.get(), gidx, coeff, order);
+    } catch (...) {
+        panels.join();
+        throw;
+    }
+    panels.join();
     if (st != BMI_OK) return st;
-    st = build_panels(s.get(), gidx, coeff, rank);
-    if (st != BMI_OK) return st;
+    if (st_pan != BMI_OK) return set_error(st_pan, msg_pan);
     *out = s.release();
     return BMI_OK;
 } catch (...) {
