@@ -24,7 +24,7 @@ EXPORTS = (
     "bmi_version", "bmi_last_error", "bmi_options_default",
     "bmi_context_create", "bmi_context_destroy",
     "bmi_mesh_create", "bmi_mesh_free",
-    "bmi_space_create", "bmi_space_free",
+    "bmi_space_create", "bmi_space_create_shaped", "bmi_space_free",
     "bmi_plan_create", "bmi_plan_create_rules", "bmi_plan_free",
     "bmi_matrix_create", "bmi_matrix_free", "bmi_matrix_device_ptr",
     "bmi_assemble", "bmi_plan_stats", "bmi_matrix_to_host", "bmi_assemble_to_host",
@@ -195,6 +195,7 @@ def lib():
     L.bmi_mesh_create.argtypes = [vp, vp, i64, vp, i64, pp]
     L.bmi_mesh_free.argtypes = [vp]
     L.bmi_space_create.argtypes = [vp, i32, vp, vp, i64, pp]
+    L.bmi_space_create_shaped.argtypes = [vp, i32, vp, vp, i64, i32, i32, pp]
     L.bmi_space_free.argtypes = [vp]
     L.bmi_plan_create.argtypes = [vp, ctypes.POINTER(Operator), vp, vp, ctypes.POINTER(Options), pp]
     L.bmi_plan_create_rules.argtypes = [vp, ctypes.POINTER(Operator), vp, vp, ctypes.POINTER(Options),

@@ -98,6 +98,10 @@ class GPUConfiguration:
         strategy (the reference's field of that name, 24 GiB by default).
     floatType / precision: "fp64" (plain) or "mixed" (FP32 far pairs).
     output: "c64" (ComplexF64) or "c32" (ComplexF32) matrix.
+    tile: (panel rows, patch columns) of the far-pair kernels' tile, at most
+        (64, 40) on RWG and (128, 20) on Lagrange spaces; 0 takes that
+        maximum, the fastest shape measured on every workload.  Other shapes
+        give the same matrix up to summation order.
     """
 
     writeBackStrategy: str = "gpu"
@@ -111,6 +115,7 @@ class GPUConfiguration:
     near_order: int = 4
     near_factor: float = 2.0
     rules: QuadratureRules | None = None  # replaces far_test_order, far_trial_order and ss_order
+    tile: tuple = (0, 0)
 
     def options(self) -> _bmi.Options:
         o = _bmi.default_options()
@@ -173,15 +178,16 @@ class DeviceMesh(_Handle):
 class DeviceSpace(_Handle):
     _free = "bmi_space_free"
 
-    def __init__(self, dmesh: DeviceMesh, space: Space):
+    def __init__(self, dmesh: DeviceMesh, space: Space, tile: tuple = (0, 0)):
         super().__init__()
         self.dmesh = dmesh
         self.space = space
         kind = {P1: _bmi.SPACE_LAGRANGE_C0D1, RWG: _bmi.SPACE_RWG}[space.kind]
         idx = np.ascontiguousarray(space.idx, dtype=np.int64)
         cf = np.ascontiguousarray(space.coeff, dtype=np.float64)
-        _bmi.check(_bmi.lib().bmi_space_create(dmesh.h, kind, idx.ctypes.data, cf.ctypes.data, int(space.nfun),
-                                               ctypes.byref(self.h)))
+        rows, cols = (int(v) for v in tile)
+        _bmi.check(_bmi.lib().bmi_space_create_shaped(dmesh.h, kind, idx.ctypes.data, cf.ctypes.data,
+                                                      int(space.nfun), rows, cols, ctypes.byref(self.h)))
 
 
 def _c_operator(op: Operator) -> _bmi.Operator:
@@ -267,12 +273,12 @@ class Assembler:
         self.ctx = ctx or Context(self.config.device)
         try:
             self.dmx = DeviceMesh(self.ctx, X.mesh)
-            self.dX = DeviceSpace(self.dmx, X)
+            self.dX = DeviceSpace(self.dmx, X, self.config.tile)
             if Y is X:
                 self.dmy, self.dY = self.dmx, self.dX
             else:
                 self.dmy = self.dmx if Y.mesh is X.mesh else DeviceMesh(self.ctx, Y.mesh)
-                self.dY = DeviceSpace(self.dmy, Y)
+                self.dY = DeviceSpace(self.dmy, Y, self.config.tile)
             self.plan = Plan(self.ctx, op, self.dX, self.dY, self.config)
             self.shape = (X.nfun, Y.nfun)
             self.matrix = DeviceMatrix(self.ctx, X.nfun, Y.nfun, self.config.dtype()) if resident else None

@@ -508,10 +508,11 @@ void parallel_for(int n, F f) {
     HostPool::get().run(n, 8, g);
 }
 
-// Panels: functions [64 P, 64 P + 64) and every cell carrying one of them,
-// ordered by Morton rank, cut into sub-batches of BMI_PATCH cells.  For each
-// sub-batch and each of the 64 rows: the slots (local cell * 3 + shape,
-// coefficient) of that row's function on the sub-batch's cells.
+// Panels: runs of at most prows consecutive functions (rows) and every cell
+// carrying one of them, ordered by Morton rank, cut into sub-batches of
+// BMI_PATCH cells.  For each sub-batch and each row of its panel: the slots
+// (local cell * 3 + shape, coefficient) of that row's function on the
+// sub-batch's cells.
 bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, const std::vector<int> &morton_rank) {
     const bmi_mesh *m = s->mesh;
     const int64_t nt = m->nt;
@@ -524,7 +525,7 @@ bmi_status build_panels(bmi_space *s, const int64_t *gidx, const double *coeff, 
     s->prow.assign(1, 0);
     std::vector<int> pan_of((size_t)nf, 0);
     {
-        const int cap = R == BMI_PANEL_ROWS ? 4 * BMI_PATCH : INT32_MAX;
+        const int cap = s->kind == BMI_SPACE_RWG ? 4 * BMI_PATCH : INT32_MAX;
         std::vector<int> fptr((size_t)nf + 1, 0), fcell;
         for (int64_t K = 0; K < nt; ++K)
             for (int i = 0; i < 3; ++i)
@@ -1226,9 +1227,10 @@ PanelArgs panel_args(const bmi_plan *P, void *Z, long long ld) {
     B.rounds_b = by_level ? P->Y->levels_b : P->Y->rounds_b;
     B.brounds = P->Y->d_brounds.p + (by_level ? P->Y->npatch : 0);
     B.ncm = P->Y->ncmax;
-    // Lagrange test spaces have wide panels, which only the rank-round
-    // kernels handle
-    B.rr = P->X->prows != BMI_PANEL_ROWS || P->X->rounds_t * B.rounds_b <= 6;
+    // Lagrange test spaces and panels of other heights than 64 rows: only
+    // the rank-round kernels handle them
+    B.rr = P->X->kind == BMI_SPACE_LAGRANGE_C0D1 || P->X->prows != BMI_PANEL_ROWS ||
+           P->X->rounds_t * B.rounds_b <= 6;
     B.npb = P->npb;
     B.bq_all = P->Y->d_by_colour.p;
     B.blk_off = P->d_blk_off.p;
@@ -1406,9 +1408,23 @@ bmi_status bmi_mesh_free(bmi_mesh *m) try {
 }
 
 bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, const double *coeff, int64_t nfun,
-                            bmi_space **out) try {
+                            bmi_space **out) {
+    return bmi_space_create_shaped(mesh, kind, gidx, coeff, nfun, 0, 0, out);
+}
+
+bmi_status bmi_space_create_shaped(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, const double *coeff,
+                                   int64_t nfun, int32_t panel_rows, int32_t patch_cols, bmi_space **out) try {
     if (!mesh || !gidx || !coeff || !out) return fail(BMI_EINVAL, "NULL argument");
     if (kind != BMI_SPACE_LAGRANGE_C0D1 && kind != BMI_SPACE_RWG) return fail(BMI_EUNSUPPORTED, "unknown space kind");
+    // the far kernels' tile: rows up to the kernel's panel height, columns up
+    // to its widest patch
+    const bool lagrange = kind == BMI_SPACE_LAGRANGE_C0D1;
+    const int max_rows = lagrange ? BMI_PANEL_ROWS_WIDE : BMI_PANEL_ROWS;
+    const int max_cols = lagrange ? BMI_MAX_COLS_WIDE : BMI_MAX_COLS;
+    if (panel_rows < 0 || panel_rows > max_rows)
+        return fail(BMI_EINVAL, "panel_rows out of range (1.." + std::to_string(max_rows) + ", 0 = default)");
+    if (patch_cols != 0 && (patch_cols < 3 || patch_cols > max_cols))
+        return fail(BMI_EINVAL, "patch_cols out of range (3.." + std::to_string(max_cols) + ", 0 = default)");
     if (nfun < 1 || nfun > (1LL << 31) - 1) return fail(BMI_EINVAL, "bad number of functions");
     for (int64_t i = 0; i < 3 * mesh->nt; ++i) {
         if (gidx[i] >= nfun) return fail(BMI_EINVAL, "function index out of range");
@@ -1421,10 +1437,8 @@ bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, c
     s->device = mesh->device;
     s->kind = kind;
     s->nfun = nfun;
-    if (kind == BMI_SPACE_LAGRANGE_C0D1) {
-        s->prows = BMI_PANEL_ROWS_WIDE;
-        s->maxcols = BMI_MAX_COLS_WIDE;
-    }
+    s->prows = panel_rows ? panel_rows : max_rows;
+    s->maxcols = patch_cols ? patch_cols : max_cols;
     // the test-role panels and the trial-role patches are built side by side
     // (they share the Morton order and write disjoint parts of the space)
     std::vector<int> order, rank;

@@ -13,8 +13,8 @@ against the FP64 path, the roofline of the far-pair kernel, and the CPU oracle
 timed on a bounded sample of the same workload; then the 64-problem batched
 EFIE/MFIE workload with each problem's mixed-vs-FP64 error, and the
 2,000-request serving stream.  Other
-workloads (--workload): efie-L3, efie-L3-k4, ellipsoid-60k, hh3d-cubeD,
-batched, serving.
+workloads (--workload): efie-L3, efie-L3-k4, efie-L4, ellipsoid-60k,
+hh3d-cubeD, batched, serving; --tile-sweep times the far-pair tile shapes.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -62,6 +62,10 @@ def workload(name: str):
         return dict(op=Maxwell3D.singlelayer(wavenumber=2.0), X=spaces.raviartthomas(m), kind="efie",
                     desc="EFIE (Maxwell single layer) on RWG, icosphere L5 radially perturbed 3% (8 SH, l<=6, seed 2), "
                          "k=2.0, 20,480 triangles, 30,720 RWG")
+    if name == "efie-L4":
+        m = meshes.perturb_radial(meshes.icosphere(4), seed=2, amplitude=0.03)
+        return dict(op=Maxwell3D.singlelayer(wavenumber=2.0), X=spaces.raviartthomas(m), kind="efie",
+                    desc="EFIE on RWG, icosphere L4 radially perturbed 3% (seed 2), k=2.0, 5,120 triangles, 7,680 RWG")
     if name == "efie-L3":
         m = meshes.icosphere(3)
         return dict(op=Maxwell3D.singlelayer(wavenumber=1.0), X=spaces.raviartthomas(m), kind="efie",
@@ -340,6 +344,41 @@ def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4, errors
     return out
 
 
+TILE_SHAPES = {"efie": [(0, 0), (48, 40), (32, 40), (64, 28), (32, 24)],
+               "hh3d": [(0, 0), (96, 20), (64, 20), (128, 12)]}
+
+
+def tile_sweep(steps: int, warmup: int, names=("efie-L3", "efie-L4", "efie-L5", "hh3d-cube24")) -> list:
+    """Far-pair tile shapes (panel rows, patch columns; (0, 0) = the default
+    (64, 40) on RWG, (128, 20) on Lagrange spaces) on each workload: FP64 and
+    mixed C32 time per assembly, and each shape's FP64 matrix against the
+    default shape's (relative Frobenius)."""
+    ctx = Context(0)
+    rows = []
+    for name in names:
+        w = workload(name)
+        ref = None
+        for shape in TILE_SHAPES[w["kind"]]:
+            print(f"[bench] tile sweep {name} shape {shape}", file=sys.stderr, flush=True)
+            a = Assembler(w["op"], w["X"], w["X"], GPUConfiguration(tile=shape), ctx=ctx)
+            t, far, _, _, _ = time_steps(a, steps, warmup)
+            if ref is None:
+                ref, rel = a, 0.0
+            else:
+                rel = a.matrix.compare(ref.matrix)["rel_fro"]
+            m = Assembler(w["op"], w["X"], w["X"], GPUConfiguration(tile=shape, precision="mixed", output="c32"),
+                          ctx=ctx)
+            tm, farm, _, _, _ = time_steps(m, steps, warmup)
+            m.close()
+            rows.append({"workload": name, "tile": list(shape), "fp64_ms": round(t * 1e3, 3),
+                         "fp64_far_ms": round(far, 3), "mixed_c32_ms": round(tm * 1e3, 3),
+                         "mixed_c32_far_ms": round(farm, 3), "fp64_rel_fro_vs_default": rel})
+            if a is not ref:
+                a.close()
+        ref.close()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,7 +398,14 @@ def main():
     ap.add_argument("--serving-total", type=int, default=2000, help="requests of --workload serving")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
+    ap.add_argument("--tile-sweep", action="store_true",
+                    help="only time the far-pair tile shapes on efie-L3/L4/L5 and hh3d-cube24 and exit")
     args = ap.parse_args()
+    if args.tile_sweep:
+        print(json.dumps({"tile_sweep": tile_sweep(args.steps, args.warmup),
+                          "columns": "ms per assembly (wall clock, mean over the steps); far ms from HIP events; tile (0, 0) = default"}),
+              flush=True)
+        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return

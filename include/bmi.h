@@ -164,6 +164,17 @@ bmi_status bmi_mesh_free(bmi_mesh *mesh);
  * of extract_values_and_indexes (load_parameters.jl:50-55). */
 bmi_status bmi_space_create(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, const double *coeff, int64_t nfun,
                             bmi_space **out);
+/* bmi_space_create with the far-pair tile shape given: test panels of at most
+ * panel_rows functions (1 .. 64 on RWG, 1 .. 128 on Lagrange spaces) and
+ * trial patches of at most patch_cols functions (3 .. 40 on RWG, 3 .. 20 on
+ * Lagrange spaces); 0 takes the kind's default, the largest of each, which
+ * is the fastest shape on every workload measured (bench.py --tile-sweep).
+ * Any shape assembles the same matrix up to summation order.
+ * No counterpart in the reference; its analogue is the thread and chunk
+ * counts that testing/determin_amount_of_threads.jl:44-75 and
+ * testing/Amdahl's_law_3.jl:142-154 sweep by hand. */
+bmi_status bmi_space_create_shaped(bmi_mesh *mesh, int32_t kind, const int64_t *gidx, const double *coeff,
+                                   int64_t nfun, int32_t panel_rows, int32_t patch_cols, bmi_space **out);
 bmi_status bmi_space_free(bmi_space *space);
 
 /* Assembly plan for (operator, test space, trial space, options): device
