@@ -1230,19 +1230,21 @@ struct SpecAcc {
 };
 
 
-// Sums of each of the N accumulators over the 64 lanes of the wave, returned
-// to every lane.  Groups of eight accumulators are reduced and scattered at
-// once: at the exchanges with lanes 32, 16 and 8 apart each lane keeps half
-// of its remaining values and sends the other half (4 + 2 + 1 exchanges), so
-// that lane l ends with value (l >> 3) & 7 summed over its eight lanes, then
-// three butterfly steps finish that sum, and one exchange per accumulator
-// hands it to every lane: eighteen exchanges per group instead of six per
-// accumulator.  The order of the additions is fixed, and both lanes
-// of an exchange add the same two numbers, so every lane that holds a sum
-// holds the same one.
-template <int N>
+// Sums of each of the N accumulators over each segment of SEG lanes (64: the
+// whole wave; 32: its halves), returned to every lane of the segment.
+// Groups of eight accumulators are reduced and scattered at once: at the
+// exchanges with lanes SEG/2, SEG/4 and SEG/8 apart each lane keeps half of
+// its remaining values and sends the other half (4 + 2 + 1 exchanges), so
+// that lane l ends with value (l / (SEG/8)) mod 8 summed over SEG/8 lanes;
+// butterfly steps over the lowest bits finish that sum, and one exchange per
+// accumulator hands it to every lane of the segment: 17-18 exchanges per
+// group instead of 5-6 per accumulator.  The order of the additions is
+// fixed, and both lanes of an exchange add the same two numbers, so every
+// lane that holds a sum holds the same one.
+template <int N, int SEG = 64>
 __device__ __forceinline__ void wave_sum_all(double (&v)[N]) {
-    constexpr int G = (N + 7) / 8;
+    static_assert(SEG == 64 || SEG == 32, "segments of 64 or 32 lanes");
+    constexpr int G = (N + 7) / 8, Q = SEG / 8;
     const int lane = threadIdx.x & 63;
     double t[G];
 #pragma unroll
@@ -1253,7 +1255,7 @@ __device__ __forceinline__ void wave_sum_all(double (&v)[N]) {
         // (three rounds written out: a loop with two counters kept the
         // halves' indices at run time, and w in scratch memory)
         auto halve = [&](auto o_c) {
-            constexpr int o = decltype(o_c)::value, h = o / 8;
+            constexpr int o = decltype(o_c)::value, h = o / Q;
             const bool up = (lane & o) != 0;
 #pragma unroll
             for (int j = 0; j < h; ++j) {
@@ -1262,15 +1264,16 @@ __device__ __forceinline__ void wave_sum_all(double (&v)[N]) {
                 w[j] = keep + __shfl_xor(give, o);
             }
         };
-        halve(std::integral_constant<int, 32>{});
-        halve(std::integral_constant<int, 16>{});
-        halve(std::integral_constant<int, 8>{});
+        halve(std::integral_constant<int, 4 * Q>{});
+        halve(std::integral_constant<int, 2 * Q>{});
+        halve(std::integral_constant<int, Q>{});
 #pragma unroll
-        for (int o = 4; o >= 1; o >>= 1) w[0] += __shfl_xor(w[0], o);
+        for (int o = Q / 2; o >= 1; o >>= 1) w[0] += __shfl_xor(w[0], o);
         t[g] = w[0];
     }
+    const int base = lane & ~(SEG - 1);
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = __shfl(t[i / 8], 8 * (i % 8));
+    for (int i = 0; i < N; ++i) v[i] = __shfl(t[i / 8], base + Q * (i % 8));
 }
 
 // Entry (i, j) of the EFIE local matrix from the moments m[0..16) (M0, Mx,
@@ -1623,13 +1626,15 @@ __device__ __forceinline__ void near_static(const OpConst &op, SpecAcc<OP> &acc,
     }
 }
 
-// One wave per touching pair (NEARK false) or per near pair (NEARK true),
+// One wave per touching pair (NEARK false; the Maxwell operators with the
+// compile-time order: half a wave per pair) or per near pair (NEARK true),
 // owner entries only: a pair whose test cell belongs to several sub-batches
 // is integrated once, in the sub-batch the cell calls home.  Touching pairs:
-// regions in turn, lanes striding over the hypercube nodes, one Green
-// evaluation each.  Near pairs: closed-form singular terms, one outer point
-// per lane, then lanes striding over the product rule.  wave_sum_all
-// reduces the accumulators over the wave.  The two kinds are separate instantiations
+// regions in turn, the pair's lanes striding over the hypercube nodes, one
+// Green evaluation (or, in the radial form, one 1/|B| and the xi loop) each.
+// Near pairs: closed-form singular terms, one outer point per lane, then
+// lanes striding over the product rule.  wave_sum_all reduces the
+// accumulators over the pair's lanes.  The two kinds are separate instantiations
 // so that the near-pair code does not add to the registers of the
 // Sauter-Schwab loop.
 // NG: Gauss-Legendre points per hypercube dimension fixed at compile time
@@ -1642,7 +1647,13 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
     __shared__ double s_glx[3][BMI_MAX_GL], s_glw[3][BMI_MAX_GL];
     __shared__ double s_nb1[BMI_MAX_NEAR], s_nb2[BMI_MAX_NEAR], s_nw[BMI_MAX_NEAR];
     __shared__ double2 s_sc[64];
-    __shared__ double4 s_node[NG ? 64 * ((NG * NG * NG + 63) / 64) : 1];
+    // compile-time order (touching pairs) of the Maxwell operators: two
+    // pairs per wave, 32 lanes each, so that the per-pair set-up and the
+    // reduction serve two pairs (EFIE special phase -14 %; four pairs per
+    // wave, or two in the Helmholtz kernel, gained nothing)
+    constexpr int PPW = (NG != 0 && !NEARK && OP != OPK_HH3D) ? 2 : 1, LPP = 64 / PPW;
+    constexpr int N3 = NG * NG * NG, NH3 = NG ? (N3 + LPP - 1) / LPP : 1;
+    __shared__ double4 s_node[NG ? LPP * NH3 : 1];
     __shared__ int s_items[BMI_BLOCK_THREADS];
     __shared__ int s_wsum[BMI_BLOCK_THREADS / 64];
     const int t = threadIdx.x;
@@ -1662,11 +1673,10 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
     const bool lossy = a.op.lossy != 0;
     // compile-time order: node table of three Gauss-Legendre indices, entry
     // u = (x[i], x[j], x[k], w[i] w[j] w[k]) for u = (i NG + j) NG + k < NG^3,
-    // padded to whole waves with the last node at weight zero; (i, j, k) are
-    // (e1, e2, e3) of the radial form
-    constexpr int N3 = NG * NG * NG, NH3 = NG ? (N3 + 63) / 64 : 1;
+    // padded to whole rounds of a pair's lanes with the last node at weight
+    // zero; (i, j, k) are (e1, e2, e3) of the radial form
     if constexpr (NG != 0) {
-        for (int u = t; u < 64 * NH3; u += BMI_BLOCK_THREADS) {
+        for (int u = t; u < LPP * NH3; u += BMI_BLOCK_THREADS) {
             const int nd = min(u, N3 - 1);
             const int i2 = nd / (NG * NG), i3 = (nd / NG) % NG, i4 = nd % NG;
             s_node[u] = make_double4(s_glx[0][i2], s_glx[0][i3], s_glx[0][i4],
@@ -1707,8 +1717,13 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             nitem += n;
         }
         __syncthreads();
-    for (int w = wv; w < nitem; w += BMI_BLOCK_THREADS / 64) {
-        const int e = s_items[w];
+    // pair slot ps of the wave takes item w0 + ps on its LPP lanes (lane pl
+    // of the pair); a slot past the last item repeats the wave's first and
+    // stores nothing
+    const int pl = lane % LPP, ps = lane / LPP;
+    for (int w0 = wv * PPW; w0 < nitem; w0 += (BMI_BLOCK_THREADS / 64) * PPW) {
+        const bool valid = w0 + ps < nitem;
+        const int e = s_items[valid ? w0 + ps : w0];
         const int cls = a.spec_cls[e];
         const int K = a.spec_K[e], L = a.spec_L[e];
         // vertex permutations of the two cells (identity for near pairs):
@@ -1828,7 +1843,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
 #pragma unroll 1
                 for (int h0 = 0; h0 < NH3; ++h0) {
                     // (e1, e2, e3, w1 w2 w3), weight zero past the table
-                    const double4 nd = s_node[lane + 64 * h0];
+                    const double4 nd = s_node[pl + LPP * h0];
                     double b1, b2, d1, d2, f;
                     ss_dir(C, RR, nd.x, nd.y, nd.z, b1, b2, d1, d2, f);
                     double qx[3], qy[3], bb = 0.0;
@@ -2020,9 +2035,10 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                 }
             }
         }
-        // the sums over the 64 lanes, to every lane (wave_sum_all); lane
-        // l < 9 then forms entry l of the chart-order 3 x 3 matrix
-        wave_sum_all(acc.v);
+        // the sums over the pair's lanes, to each of them (wave_sum_all);
+        // lane l < 9 of the pair then forms entry l of the chart-order 3 x 3
+        // matrix
+        wave_sum_all<spec_nacc(OP), LPP>(acc.v);
         if constexpr (NG != 0 && !NEARK && OP == OPK_MW) {
             // radial form: the moments of x = P + xi Q and y = P' + xi Q'
             // from the lane sums (S_0, Q S_1, Q' S_1, (Q.Q') S_2), P and P'
@@ -2080,11 +2096,11 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             for (int i = 0; i < 6; ++i) acc.v[8 + i] = acc.v[2 + i];  // paired regions: moments of y = those of x
         }
         const double *red = acc.v;
-        if (lane < 9) {
+        if (valid && pl < 9) {
             // chart vertices c (test) and c2 (trial) relative to the
             // centroids (o + t1, o + t2, o); entry (c, c2) belongs to the
             // local shapes (PT[c], PS[c2])
-            const int c = lane / 3, c2 = lane - 3 * (lane / 3);
+            const int c = pl / 3, c2 = pl - 3 * (pl / 3);
             double Xc[3], Yc[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -2122,7 +2138,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
                     pi = red[1];
 #pragma unroll
                     for (int k = 1; k < 9; ++k)
-                        if (lane == k) {
+                        if (pl == k) {
                             pr = red[2 * k];
                             pi = red[2 * k + 1];
                         }
