@@ -1657,33 +1657,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
     __shared__ int s_items[BMI_BLOCK_THREADS];
     __shared__ int s_wsum[BMI_BLOCK_THREADS / 64];
     const int t = threadIdx.x;
-    fill_sincos_table(s_sc);
-    if (t < 3 * BMI_MAX_GL) {
-        s_glx[t / BMI_MAX_GL][t % BMI_MAX_GL] = a.glx[t / BMI_MAX_GL][t % BMI_MAX_GL];
-        s_glw[t / BMI_MAX_GL][t % BMI_MAX_GL] = a.glw[t / BMI_MAX_GL][t % BMI_MAX_GL];
-    }
-    if (t < BMI_MAX_NEAR) {
-        s_nb1[t] = a.nb1[t];
-        s_nb2[t] = a.nb2[t];
-        s_nw[t] = a.nw[t];
-    }
-    __syncthreads();
     const int lane = t & 63, wv = t >> 6;
     const int ns = min(*a.n_special, a.capacity);
     const bool lossy = a.op.lossy != 0;
-    // compile-time order: node table of three Gauss-Legendre indices, entry
-    // u = (x[i], x[j], x[k], w[i] w[j] w[k]) for u = (i NG + j) NG + k < NG^3,
-    // padded to whole rounds of a pair's lanes with the last node at weight
-    // zero; (i, j, k) are (e1, e2, e3) of the radial form
-    if constexpr (NG != 0) {
-        for (int u = t; u < LPP * NH3; u += BMI_BLOCK_THREADS) {
-            const int nd = min(u, N3 - 1);
-            const int i2 = nd / (NG * NG), i3 = (nd / NG) % NG, i4 = nd % NG;
-            s_node[u] = make_double4(s_glx[0][i2], s_glx[0][i3], s_glx[0][i4],
-                                     (u < N3 ? 1.0 : 0.0) * s_glw[0][i2] * s_glw[0][i3] * s_glw[0][i4]);
-        }
-        __syncthreads();
-    }
     // one chunk of cs (<= 256) entries: the threads pick out the touching (or
     // near) entries the problem owns, listed by decreasing cost (common face:
     // 6 regions, common edge: 5, common vertex: 2), then each wave integrates
@@ -1716,7 +1692,34 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
             if (mine) s_items[nitem + base + __popcll(m & ((1ull << lane) - 1ull))] = ec;
             nitem += n;
         }
+        // a chunk without such entries (the mixed path's chunks of class-5
+        // entries, the near kernel's chunks of touching ones) ends here,
+        // before the tables
+        if (nitem == 0) return;
+        fill_sincos_table(s_sc);
+        if (t < 3 * BMI_MAX_GL) {
+            s_glx[t / BMI_MAX_GL][t % BMI_MAX_GL] = a.glx[t / BMI_MAX_GL][t % BMI_MAX_GL];
+            s_glw[t / BMI_MAX_GL][t % BMI_MAX_GL] = a.glw[t / BMI_MAX_GL][t % BMI_MAX_GL];
+        }
+        if (t < BMI_MAX_NEAR) {
+            s_nb1[t] = a.nb1[t];
+            s_nb2[t] = a.nb2[t];
+            s_nw[t] = a.nw[t];
+        }
         __syncthreads();
+        // compile-time order: node table of three Gauss-Legendre indices, entry
+        // u = (x[i], x[j], x[k], w[i] w[j] w[k]) for u = (i NG + j) NG + k < NG^3,
+        // padded to whole rounds of a pair's lanes with the last node at weight
+        // zero; (i, j, k) are (e1, e2, e3) of the radial form
+        if constexpr (NG != 0) {
+            for (int u = t; u < LPP * NH3; u += BMI_BLOCK_THREADS) {
+                const int nd = min(u, N3 - 1);
+                const int i2 = nd / (NG * NG), i3 = (nd / NG) % NG, i4 = nd % NG;
+                s_node[u] = make_double4(s_glx[0][i2], s_glx[0][i3], s_glx[0][i4],
+                                         (u < N3 ? 1.0 : 0.0) * s_glw[0][i2] * s_glw[0][i3] * s_glw[0][i4]);
+            }
+            __syncthreads();
+        }
     // pair slot ps of the wave takes item w0 + ps on its LPP lanes (lane pl
     // of the pair); a slot past the last item repeats the wave's first and
     // stores nothing
@@ -2167,9 +2170,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *
     __shared__ int s_items[BMI_BLOCK_THREADS];
     __shared__ int s_wsum[BMI_BLOCK_THREADS / 64];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    fill_sincos_table(s_sc);
-    // the work-group's entries of class 5 (owners) packed to its first
-    // threads, so that the waves past them exit at once
+    // the work-group's entries of class 5 packed to its first threads, so
+    // that the waves past them exit at once (all of them, before the table,
+    // where there are none)
     const int u0 = blockIdx.x * BMI_BLOCK_THREADS + t;
     bool mine = false;
     if (u0 < ebase[nprob]) {
@@ -2187,7 +2190,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *
         base += i < wv ? s_wsum[i] : 0;
         n += s_wsum[i];
     }
+    if (n == 0) return;
     if (mine) s_items[base + __popcll(m & ((1ull << lane) - 1ull))] = u0;
+    fill_sincos_table(s_sc);
     __syncthreads();
     if (t >= n) return;
     const int u = s_items[t];
