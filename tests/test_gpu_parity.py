@@ -129,14 +129,14 @@ def test_repeat_assembly_bitwise_deterministic(gpu_ctx):
     assert np.array_equal(Z1, Z2)
 
 
-def test_config2_sampled_rows_and_mixed(gpu_ctx):
-    """Full-size config 2 (20,480 triangles, 30,720 RWG): 48 rows against the
-    oracle at 1e-12, the mixed path against FP64 on the whole matrix, and the
-    class counts against the Euler closed forms."""
+def test_icosphere_l5_sampled_rows_and_mixed(gpu_ctx):
+    """The full-size perturbed level-5 icosphere (20,480 triangles, 30,720
+    RWG): 48 rows against the oracle at 1e-12, the mixed path against FP64 on
+    the whole matrix, and the class counts against the Euler closed forms."""
     m = meshes.perturb_radial(meshes.icosphere(5), seed=2, amplitude=0.03)
     X = spaces.raviartthomas(m)
     op = Maxwell3D.singlelayer(wavenumber=2.0)
-    print("case config2", flush=True)
+    print("case icosphere L5 EFIE", flush=True)
     a = Assembler(op, X, X, GPUConfiguration(), ctx=gpu_ctx)
     M = a.assemble()
     st = a.stats()
@@ -149,12 +149,12 @@ def test_config2_sampled_rows_and_mixed(gpu_ctx):
     # rows of the device matrix through apply with unit vectors would be slow; copy the matrix
     Z = M.to_host()
     err = relfro(Z[rows], Zr)
-    print("config2 sampled-row err", err, flush=True)
+    print("icosphere L5 sampled-row err", err, flush=True)
     assert err <= FP64_TOL
     del Z
     b = Assembler(op, X, X, GPUConfiguration(precision="mixed"), ctx=gpu_ctx)
     cmp = b.assemble().compare(M)
-    print("config2 mixed", cmp, b.stats(), flush=True)
+    print("icosphere L5 mixed", cmp, b.stats(), flush=True)
     assert cmp["rel_fro"] <= MIXED_TOL
     x = np.sqrt(rng.uniform(size=X.nfun)) * np.exp(2j * np.pi * rng.uniform(size=X.nfun))
     y64, ym = M.apply(x), b.matrix.apply(x)
@@ -200,7 +200,7 @@ def test_lossy_operator_matches_oracle(gpu_ctx, kind, precision, tol):
     assert err <= tol
 
 
-def test_config3_ellipsoid_full_size(gpu_ctx):
+def test_ellipsoid_60k_full_size(gpu_ctx):
     """The 60,000-triangle ellipsoid (90,000 RWG, k = 4): FP64 matrix (129.6 GB)
     and mixed ComplexF32 matrix (64.8 GB) resident together.  Checks: class
     counts against the Euler closed forms; (Z x) on 24 sampled rows against
@@ -216,7 +216,7 @@ def test_config3_ellipsoid_full_size(gpu_ctx):
     m = meshes.ellipsoid()
     X = spaces.raviartthomas(m)
     op = Maxwell3D.singlelayer(wavenumber=4.0)
-    print("case config3 ellipsoid-60k", flush=True)
+    print("case ellipsoid-60k", flush=True)
     a = Assembler(op, X, X, GPUConfiguration(), ctx=gpu_ctx)
     M = a.assemble()
     st = a.stats()
@@ -230,13 +230,13 @@ def test_config3_ellipsoid_full_size(gpu_ctx):
     Zr, _ = O.assemble(op, X, X, rows=rows)
     yr = Zr @ x
     err_rows = np.linalg.norm(y64[rows] - yr) / np.linalg.norm(yr)
-    print("config3 sampled-row Zx err", err_rows, st, flush=True)
+    print("ellipsoid sampled-row Zx err", err_rows, st, flush=True)
     assert err_rows <= FP64_TOL
     b = Assembler(op, X, X, GPUConfiguration(precision="mixed", output="c32"), ctx=gpu_ctx)
     cmp = b.assemble().compare(M)
     ym = b.matrix.apply(x)
     zx = np.linalg.norm(ym - y64) / np.linalg.norm(y64)
-    print("config3 mixed c32", cmp, "zx", zx, b.stats(), flush=True)
+    print("ellipsoid mixed c32", cmp, "zx", zx, b.stats(), flush=True)
     b.close()
     a.close()
     assert cmp["rel_fro"] <= MIXED_TOL

@@ -21,13 +21,13 @@ def _resident(op, X, Y, cfg, ctx):
     return Z
 
 
-def test_config2_under_a_4gb_budget_equals_resident(gpu_ctx):
+def test_icosphere_l5_under_a_4gb_budget_equals_resident(gpu_ctx):
     """The 30,720-RWG EFIE matrix (15.1 GB) through 4 GB of staging (8 blocks
     of 60 panels, two on the device at a time): bitwise the resident result."""
     m = meshes.perturb_radial(meshes.icosphere(5), seed=2, amplitude=0.03)
     X = spaces.raviartthomas(m)
     op = Maxwell3D.singlelayer(wavenumber=2.0)
-    print("case streamed config2 4 GB", flush=True)
+    print("case streamed icosphere L5, 4 GB", flush=True)
     a = Assembler(op, X, X, GPUConfiguration(), ctx=gpu_ctx)
     Zr = a.assemble().to_host()
     a.close()
