@@ -356,6 +356,7 @@ struct bmi_plan {
     int cs = 256, nchunks = 0;
     int64_t pairs[6] = {0, 0, 0, 0, 0, 0};
     bool pairs_ready = false;
+    DevBuf<double2> d_sc;  // the 64-entry sine/cosine table of the FP64 Green evaluations
     bool rule_default = true;  // the far rule is the default 3 x 4 point rule
     mutable bool filled = false;  // the special-pair entries have been written
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -1238,6 +1239,7 @@ PanelArgs panel_args(const bmi_plan *P, void *Z, long long ld) {
     B.spec_loc = P->d_spec_loc.p;
     B.spec_val = P->d_spec_val.p;
     B.capacity = P->capacity;
+    B.sc = P->d_sc.p;
     B.Z = Z;
     B.ld = ld;
     B.op = P->opc;
@@ -1683,8 +1685,9 @@ static bmi_status plan_create(bmi_context *ctx, const bmi_operator *op, const bm
     if (P->nblocks >= (1LL << 31) - 1) return fail(BMI_EUNSUPPORTED, "too many patch blocks");
     const int nbk = (int)P->nblocks;
     if ((e = P->d_blk_cnt.alloc(nbk)) != hipSuccess || (e = P->d_blk_off.alloc(nbk)) != hipSuccess ||
-        (e = P->d_total.alloc(1)) != hipSuccess)
+        (e = P->d_total.alloc(1)) != hipSuccess || (e = P->d_sc.alloc(64)) != hipSuccess)
         return fail(BMI_ENOMEM, std::string("plan buffers: ") + hipGetErrorString(e));
+    BMI_HIP(launch_sincos_table(P->d_sc.p, s));
 
     ClassifyArgs &C = P->cargs;
     C.X = dev_view_test(test, P->d_xpts.p);
@@ -1742,6 +1745,7 @@ static bmi_status plan_create(bmi_context *ctx, const bmi_operator *op, const bm
     S.blk_cnt = P->d_blk_cnt.p;
     S.npb = P->npb;
     S.capacity = total;
+    S.sc = P->d_sc.p;
     S.spec_val = P->d_spec_val.p;
     S.op = P->opc;
     S.rule = P->rule;
@@ -1865,7 +1869,7 @@ bmi_status bmi_plan_stats(const bmi_plan *Pc, bmi_stats *st) try {
                                buf_bytes(P->d_total) + buf_bytes(P->d_spec_K) + buf_bytes(P->d_spec_L) +
                                buf_bytes(P->d_spec_cls) + buf_bytes(P->d_spec_loc) + buf_bytes(P->d_spec_val) +
                                buf_bytes(P->d_cargs) + buf_bytes(P->d_sargs) + buf_bytes(P->d_gbase) +
-                               buf_bytes(P->d_cbase) + buf_bytes(P->d_ebase));
+                               buf_bytes(P->d_cbase) + buf_bytes(P->d_ebase) + buf_bytes(P->d_sc));
     return BMI_OK;
 } catch (...) {
     return fail_exception();

@@ -134,6 +134,7 @@ struct PanelArgs {
     const unsigned char *spec_loc;
     const double2 *spec_val;
     int capacity;                // special-pair entries allocated
+    const double2 *sc;           // 64 x (sin, cos) of 2 pi j / 64, the FP64 Green evaluations' table
     void *Z;
     long long ld;
     OpConst op;
@@ -163,6 +164,7 @@ struct SpecialArgs {
     const int *blk_off, *blk_cnt;
     int npb;
     int capacity;        // special-pair entries allocated
+    const double2 *sc;   // 64 x (sin, cos) of 2 pi j / 64 (as PanelArgs::sc)
     double2 *spec_val;
     OpConst op;
     RuleConst rule;
@@ -187,6 +189,8 @@ hipError_t launch_rule_points(const double *vtx, const double *cen, long long nt
                               double *pts, hipStream_t s);
 hipError_t launch_geo_records(const double *pts, const double *vtx, const double *cen, const double *area,
                               const int *cells, int ngroups, double *g, hipStream_t s);
+// the 64-entry sine/cosine table of the FP64 Green evaluations, written to tab
+hipError_t launch_sincos_table(double2 *tab, hipStream_t s);
 hipError_t launch_classify(const ClassifyArgs *args, const int *gbase, int nprob, int total_groups, hipStream_t s);
 hipError_t launch_zero_each(int *const *ptrs, int n, hipStream_t s);
 // special pairs, one problem or a batch of one operator: cbase / ebase are

@@ -102,6 +102,13 @@ __device__ __forceinline__ void fill_sincos_table(double2 *tab) {
     }
 }
 
+// The same table copied into LDS from the plan's copy in global memory (the
+// kernels that start many work-groups: a load instead of a sine and cosine
+// per work-group).
+__device__ __forceinline__ void load_sincos_table(double2 *tab, const double2 *src) {
+    if (threadIdx.x < 64) tab[threadIdx.x] = src[threadIdx.x];
+}
+
 // G * w for one point pair in FP64: h = w4pi * exp(-gamma R) / R, returned as
 // (re, im).  TAB: sine and cosine from the 64-entry table in LDS, else from the
 // quarter-turn polynomials; lossy = false drops the attenuation factor.
@@ -1696,7 +1703,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_MW && !NEARK) ? 3 : 2
         // entries, the near kernel's chunks of touching ones) ends here,
         // before the tables
         if (nitem == 0) return;
-        fill_sincos_table(s_sc);
+        load_sincos_table(s_sc, a.sc);
         if (t < 3 * BMI_MAX_GL) {
             s_glx[t / BMI_MAX_GL][t % BMI_MAX_GL] = a.glx[t / BMI_MAX_GL][t % BMI_MAX_GL];
             s_glw[t / BMI_MAX_GL][t % BMI_MAX_GL] = a.glw[t / BMI_MAX_GL][t % BMI_MAX_GL];
@@ -2192,7 +2199,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_far64(const SpecialArgs *
     }
     if (n == 0) return;
     if (mine) s_items[base + __popcll(m & ((1ull << lane) - 1ull))] = u0;
-    fill_sincos_table(s_sc);
+    load_sincos_table(s_sc, args[0].sc);
     __syncthreads();
     if (t >= n) return;
     const int u = s_items[t];
@@ -2286,6 +2293,13 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_link(const SpecialArgs *a
         for (int j = 0; j < 3; ++j)
             a.spec_val[9 * (long long)e + 3 * i + j] =
                 o >= 0 ? a.spec_val[9 * (long long)o + (mirror ? 3 * j + i : 3 * i + j)] : nan2;
+}
+
+__global__ void k_sincos_table(double2 *tab) { fill_sincos_table(tab); }
+
+hipError_t launch_sincos_table(double2 *tab, hipStream_t s) {
+    hipLaunchKernelGGL(k_sincos_table, dim3(1), dim3(64), 0, s, tab);
+    return hipGetLastError();
 }
 
 hipError_t launch_special(const SpecialArgs *args, const int *cbase, const int *ebase, int nprob, int total_chunks,
@@ -2412,7 +2426,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS) void k_panels(PanelArgs a) {
     __shared__ double2 s_sc[64];
 
     const int t = threadIdx.x;
-    if (!MIXED) fill_sincos_table(s_sc);
+    if (!MIXED) load_sincos_table(s_sc, a.sc);
     const int w = blockIdx.x;
     const int pl = w / a.nbq;  // panel of the launch's range
     const int P = a.panel0 + pl;
@@ -2876,7 +2890,7 @@ template <int OP, int MIXED, int FAST, typename OutT>
 __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_rr(PanelArgs a) {
     extern __shared__ double2 tile[];             // panel_rows(OP) x a.ncm
     __shared__ PanelShared sh;
-    if (!MIXED) fill_sincos_table(sh.s_sc);
+    if (!MIXED) load_sincos_table(sh.s_sc, a.sc);
     const int w = blockIdx.x;
     const int pl = w / a.nbq;  // panel of the launch's range
     const int P = a.panel0 + pl;
@@ -2891,8 +2905,8 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
     const PanelArgs *args, const int4 *items) {
     extern __shared__ double2 tile[];  // panel_rows(OP) x (the batch's widest patch)
     __shared__ PanelShared sh;
-    if (!MIXED) fill_sincos_table(sh.s_sc);
     const int4 it = items[blockIdx.x];
+    if (!MIXED) load_sincos_table(sh.s_sc, args[it.x].sc);
     if constexpr (MIXED) {
         // a copy: the kernel's stores to the matrix cannot alias it, so its
         // fields are loaded once instead of after each store (the FP64
@@ -2920,7 +2934,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
     __shared__ PanelShared sh;
     __shared__ int s_item;
     const int t = threadIdx.x;
-    if (!MIXED) fill_sincos_table(sh.s_sc);
+    if (!MIXED) load_sincos_table(sh.s_sc, args[0].sc);
     for (;;) {
         __syncthreads();  // the previous block is done with the LDS and s_item; its stores are visible here
         if (t == 0) s_item = atomicAdd(head, 1);
