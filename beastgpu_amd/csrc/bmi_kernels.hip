@@ -2848,41 +2848,54 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     // write-back, one column per wave at a time, lane l taking position l of
     // the column's segment (row l ^ (c mod 16)).  The stored values of all of
     // the wave's read-modify-write columns are loaded first, so that the
-    // loads are in flight together, then added and stored.
+    // loads are in flight together, then added and stored.  A column is the
+    // same for the whole wave: its function, first-touch flag and address
+    // are read into scalar registers, so that each lane adds only its row.
     // (wide panels: each column in PR / 64 segments of 64 rows)
     OutT *Z = (OutT *)a.Z;
     constexpr int NH = PR / 64;
     constexpr int NCW = (panel_cols(OP) + BMI_BLOCK_THREADS / 64 - 1) / (BMI_BLOCK_THREADS / 64);
+    const int wvu = __builtin_amdgcn_readfirstlane(wv);
+    auto column = [&](int c) -> OutT * {
+        return Z + (long long)__builtin_amdgcn_readfirstlane(s_cfn[c]) * a.ld + r0;
+    };
     OutT old[NCW * NH];
 #pragma unroll
-    for (int k = 0; k < NCW; ++k)
+    for (int k = 0; k < NCW; ++k) {
+        const int c = wvu + k * (BMI_BLOCK_THREADS / 64);
 #pragma unroll
-        for (int h = 0; h < NH; ++h) {
-            const int c = wv + k * (BMI_BLOCK_THREADS / 64);
-            const int r = 64 * h + (lane ^ (c & 15));
-            old[NH * k + h] = OutT{};
-            if (c < nc && r < nr && !s_cfirst[c])
-                old[NH * k + h] = load_stream(&Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld]);
-        }
+        for (int h = 0; h < NH; ++h) old[NH * k + h] = OutT{};
+        if (c < nc && !__builtin_amdgcn_readfirstlane(s_cfirst[c])) {
+            const OutT *zc = column(c);
 #pragma unroll
-    for (int k = 0; k < NCW; ++k)
-#pragma unroll
-        for (int h = 0; h < NH; ++h) {
-            const int c = wv + k * (BMI_BLOCK_THREADS / 64);
-            const int r = 64 * h + (lane ^ (c & 15));
-            if (c < nc && r < nr) {
-                double2 v = tile[c * PR + 64 * h + lane];
-                if constexpr (FAST) {  // the tile holds the entries divided by alpha * fast_scale
-                    constexpr double ks = fast_scale<OP, MIXED, FAST>();
-                    const double ar = ks * a.op.alpha_re, ai = ks * a.op.alpha_im;
-                    v = make_double2(fma(ar, v.x, -ai * v.y), fma(ar, v.y, ai * v.x));
-                }
-                OutT o;
-                o.x = (decltype(o.x))((double)old[NH * k + h].x + v.x);
-                o.y = (decltype(o.y))((double)old[NH * k + h].y + v.y);
-                store_stream(&Z[(long long)(r0 + r) + (long long)s_cfn[c] * a.ld], o);
+            for (int h = 0; h < NH; ++h) {
+                const int r = 64 * h + (lane ^ (c & 15));
+                if (r < nr) old[NH * k + h] = load_stream(zc + r);
             }
         }
+    }
+    // the tile holds the entries divided by alpha * fast_scale (FAST)
+    constexpr double ks = fast_scale<OP, MIXED, FAST>();
+    const double ar = ks * a.op.alpha_re, ai = ks * a.op.alpha_im;
+#pragma unroll
+    for (int k = 0; k < NCW; ++k) {
+        const int c = wvu + k * (BMI_BLOCK_THREADS / 64);
+        if (c < nc) {
+            OutT *zc = column(c);
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                const int r = 64 * h + (lane ^ (c & 15));
+                if (r < nr) {
+                    double2 v = tile[c * PR + 64 * h + lane];
+                    if constexpr (FAST) v = make_double2(fma(ar, v.x, -ai * v.y), fma(ar, v.y, ai * v.x));
+                    OutT o;
+                    o.x = (decltype(o.x))((double)old[NH * k + h].x + v.x);
+                    o.y = (decltype(o.y))((double)old[NH * k + h].y + v.y);
+                    store_stream(zc + r, o);
+                }
+            }
+        }
+    }
 }
 
 // One work-group per (test panel P of the launch's range, trial patch q of its colour).
