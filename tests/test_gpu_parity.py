@@ -78,6 +78,40 @@ def test_mixed_icosphere_l3_k4(gpu_ctx, output):
     assert err <= MIXED_TOL and zx <= MIXED_TOL
 
 
+@pytest.mark.parametrize("kind", ["efie", "mfie", "hh3d"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_complexf32_output_matches_oracle(gpu_ctx, kind, precision):
+    """ComplexF32 matrices (the FP64 path rounded once at the write-back, and
+    the mixed path) of every operator on a perturbed level-3 mesh: relative
+    Frobenius and Z x errors against the FP64 oracle within 1e-6."""
+    m = meshes.perturb_radial(meshes.icosphere(3), 11, 0.04)
+    if kind == "hh3d":
+        op, X = Helmholtz3D.singlelayer(wavenumber=3.0), spaces.lagrangec0d1(m)
+    elif kind == "mfie":
+        op, X = Maxwell3D.doublelayer(wavenumber=3.0), spaces.raviartthomas(m)
+    else:
+        op, X = Maxwell3D.singlelayer(wavenumber=3.0), spaces.raviartthomas(m)
+    print(f"case c32 {kind} {precision}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(precision=precision, output="c32"), ctx=gpu_ctx)
+    M = a.assemble()
+    Z = M.to_host()
+    assert Z.dtype == np.complex64
+    rng = np.random.default_rng(33)
+    x = np.sqrt(rng.uniform(size=X.nfun)) * np.exp(2j * np.pi * rng.uniform(size=X.nfun))
+    y = M.apply(x)
+    a.close()
+    Zref, _ = O.assemble(op, X, X)
+    yref = Zref @ x
+    err = relfro(Z.astype(np.complex128), Zref)
+    zx = np.linalg.norm(y - yref) / np.linalg.norm(yref)
+    print("c32 rel err", err, "zx", zx, flush=True)
+    assert err <= MIXED_TOL and zx <= MIXED_TOL
+    if precision == "fp64":
+        # one or two roundings to ComplexF32 per entry (read-modify-written
+        # columns are rounded again after their second part is added)
+        assert err <= 2e-7
+
+
 @pytest.mark.parametrize("d,precision", [(4, "fp64"), (6, "fp64"), (6, "mixed")])
 def test_meshcuboid_hh3d_multi_panel(gpu_ctx, d, precision):
     """The reference's own workload shape, meshcuboid(1,1,1,0.5/d) with the
