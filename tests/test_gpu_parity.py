@@ -79,6 +79,38 @@ def test_mixed_icosphere_l3_k4(gpu_ctx, output):
 
 
 @pytest.mark.parametrize("kind", ["efie", "mfie", "hh3d"])
+def test_level4_full_matrix_matches_oracle(gpu_ctx, kind):
+    """A perturbed level-4 icosphere (5,120 triangles; 7,680 RWG or 2,562 P1
+    functions, several panels and trial colours per launch): the whole FP64
+    matrix against the oracle at 1e-12 and the pair counts, then the mixed
+    path within 1e-6."""
+    m = meshes.perturb_radial(meshes.icosphere(4), 13, 0.04)
+    if kind == "hh3d":
+        op, X = Helmholtz3D.singlelayer(wavenumber=2.5), spaces.lagrangec0d1(m)
+    elif kind == "mfie":
+        op, X = Maxwell3D.doublelayer(wavenumber=2.5), spaces.raviartthomas(m)
+    else:
+        op, X = Maxwell3D.singlelayer(wavenumber=2.5), spaces.raviartthomas(m)
+    print(f"case L4 full {kind} nfun={X.nfun}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(), ctx=gpu_ctx)
+    Z = a.assemble().to_host()
+    st = a.stats()
+    a.close()
+    Zref, counts = O.assemble(op, X, X)
+    err = relfro(Z, Zref)
+    print("L4 fp64 rel err", err, flush=True)
+    assert [st["pairs"][n] for n in ("far", "common_vertex", "common_edge", "common_face", "near")] == list(counts)
+    assert err <= FP64_TOL
+    del Z
+    b = Assembler(op, X, X, GPUConfiguration(precision="mixed"), ctx=gpu_ctx)
+    Zm = b.assemble().to_host()
+    b.close()
+    errm = relfro(Zm, Zref)
+    print("L4 mixed rel err", errm, flush=True)
+    assert errm <= MIXED_TOL
+
+
+@pytest.mark.parametrize("kind", ["efie", "mfie", "hh3d"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_complexf32_output_matches_oracle(gpu_ctx, kind, precision):
     """ComplexF32 matrices (the FP64 path rounded once at the write-back, and
