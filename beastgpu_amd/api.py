@@ -100,8 +100,10 @@ class GPUConfiguration:
     output: "c64" (ComplexF64) or "c32" (ComplexF32) matrix.
     tile: (panel rows, patch columns) of the far-pair kernels' tile, at most
         (64, 40) on RWG and (128, 20) on Lagrange spaces; 0 takes that
-        maximum, the fastest shape measured on every workload.  Other shapes
-        give the same matrix up to summation order.
+        maximum, the fastest shape measured on every workload.  "auto": the
+        shape the tile autotuner (``tune_tile``) times fastest for this
+        problem's signature.  Other shapes give the same matrix up to
+        summation order.
     """
 
     writeBackStrategy: str = "gpu"
@@ -115,7 +117,7 @@ class GPUConfiguration:
     near_order: int = 4
     near_factor: float = 2.0
     rules: QuadratureRules | None = None  # replaces far_test_order, far_trial_order and ss_order
-    tile: tuple = (0, 0)
+    tile: tuple | str = (0, 0)
 
     def options(self) -> _bmi.Options:
         o = _bmi.default_options()
@@ -261,27 +263,91 @@ class DeviceMatrix(_Handle):
         return out
 
 
+# Far-pair tile shapes the autotuner times, per space kind ((0, 0): the
+# largest tile, the default)
+TILE_CANDIDATES = {RWG: ((0, 0), (48, 40), (64, 28), (32, 40)), P1: ((0, 0), (96, 20), (64, 20), (128, 12))}
+_TILE_CACHE: dict = {}
+
+
+def tile_signature(ctx: Context, op: Operator, X: Space, Y: Space, config: GPUConfiguration) -> tuple:
+    """What the fastest tile shape depends on: device, operator kind and
+    lossiness, precision and output type, quadrature (orders or caller
+    tables), space kind, the test and trial meshes' size classes (powers of
+    two of their cell counts) and whether test and trial space are one."""
+    rule = ("tables", id(config.rules)) if config.rules is not None else (config.far_test_order, config.far_trial_order)
+    return (ctx.device, op.kind, complex(op.gamma).real != 0.0, config.precision, config.output, rule, X.kind,
+            int(X.mesh.nt).bit_length(), int(Y.mesh.nt).bit_length(), X is Y)
+
+
+def tune_tile(ctx: Context, op: Operator, X: Space, Y: Space, config: GPUConfiguration,
+              matrix: "DeviceMatrix", dmx: DeviceMesh, dmy: DeviceMesh) -> tuple:
+    """The far-pair tile shape with the shortest assembly among
+    TILE_CANDIDATES[X.kind], and every candidate's time (ms): each candidate
+    builds its spaces and plan on the problem itself and assembles into
+    ``matrix`` once to warm up, then twice timed by the library's events
+    (the shorter kept).  Cached per tile_signature for the process, so
+    problems of one signature are timed once.  No counterpart in the
+    reference, whose analogous knobs are swept by hand
+    (testing/determin_amount_of_threads.jl:44-75)."""
+    key = tile_signature(ctx, op, X, Y, config)
+    hit = _TILE_CACHE.get(key)
+    if hit is not None:
+        return hit
+    times = {}
+    for tile in TILE_CANDIDATES[X.kind]:
+        made = []
+        try:
+            dX = DeviceSpace(dmx, X, tile)
+            made.append(dX)
+            dY = dX
+            if Y is not X:
+                dY = DeviceSpace(dmy, Y, tile)
+                made.append(dY)
+            plan = Plan(ctx, op, dX, dY, config)
+            made.append(plan)
+            best = float("inf")
+            for rep in range(3):
+                _bmi.check(_bmi.lib().bmi_assemble(plan.h, matrix.h))
+                if rep:
+                    best = min(best, plan.stats()["total_ms"])
+            times[tile] = best
+        finally:
+            for o in reversed(made):
+                o.close()
+    choice = min(times, key=times.get)
+    _TILE_CACHE[key] = (choice, times)
+    return choice, times
+
+
 class Assembler:
     """Resident assembly of one (operator, test space, trial space).  With
     ``resident=False`` no device matrix is allocated: only assemble_to_host
-    is available (matrices larger than device memory)."""
+    is available (matrices larger than device memory), and ``tile="auto"``
+    takes the default shape (the tuner needs a resident matrix)."""
 
     def __init__(self, op: Operator, X: Space, Y: Space, config: GPUConfiguration | None = None,
                  ctx: Context | None = None, resident: bool = True):
         self.config = config or GPUConfiguration()
         self.matrix = self.plan = self.dY = self.dX = self.dmy = self.dmx = None
+        tile = self.config.tile
+        if isinstance(tile, str) and tile != "auto":
+            raise ValueError('tile must be (panel rows, patch columns) or "auto"')
         self.ctx = ctx or Context(self.config.device)
+        self.tile_times = None
         try:
             self.dmx = DeviceMesh(self.ctx, X.mesh)
-            self.dX = DeviceSpace(self.dmx, X, self.config.tile)
-            if Y is X:
-                self.dmy, self.dY = self.dmx, self.dX
-            else:
-                self.dmy = self.dmx if Y.mesh is X.mesh else DeviceMesh(self.ctx, Y.mesh)
-                self.dY = DeviceSpace(self.dmy, Y, self.config.tile)
-            self.plan = Plan(self.ctx, op, self.dX, self.dY, self.config)
+            self.dmy = self.dmx if Y.mesh is X.mesh else DeviceMesh(self.ctx, Y.mesh)
             self.shape = (X.nfun, Y.nfun)
             self.matrix = DeviceMatrix(self.ctx, X.nfun, Y.nfun, self.config.dtype()) if resident else None
+            if isinstance(tile, str):
+                tile = (0, 0)
+                if self.matrix is not None:
+                    tile, self.tile_times = tune_tile(self.ctx, op, X, Y, self.config, self.matrix, self.dmx,
+                                                      self.dmy)
+            self.tile = tuple(tile)
+            self.dX = DeviceSpace(self.dmx, X, self.tile)
+            self.dY = self.dX if Y is X else DeviceSpace(self.dmy, Y, self.tile)
+            self.plan = Plan(self.ctx, op, self.dX, self.dY, self.config)
         except BaseException:
             # release what was built, children first (the exception's
             # traceback would otherwise keep it alive)

@@ -12,9 +12,9 @@ ComplexF32 and ComplexF64 outputs) with its relative Frobenius and Z.x errors
 against the FP64 path, the roofline of the far-pair kernel, and the CPU oracle
 timed on a bounded sample of the same workload; then the 64-problem batched
 EFIE/MFIE workload with each problem's mixed-vs-FP64 error, and the
-2,000-request serving stream.  Other
-workloads (--workload): efie-L3, efie-L3-k4, efie-L4, ellipsoid-60k,
-hh3d-cubeD, batched, serving; --tile-sweep times the far-pair tile shapes.
+2,000-request serving stream.  Other workloads (--workload): efie-L3,
+efie-L3-k4, efie-L4, ellipsoid-60k, hh3d-cubeD, batched, serving;
+--tile-sweep times the far-pair tile shapes and the autotuner's choice.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -375,6 +375,16 @@ def tile_sweep(steps: int, warmup: int, names=("efie-L3", "efie-L4", "efie-L5", 
                          "mixed_c32_far_ms": round(farm, 3), "fp64_rel_fro_vs_default": rel})
             if a is not ref:
                 a.close()
+        print(f"[bench] tile sweep {name} auto", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        a = Assembler(w["op"], w["X"], w["X"], GPUConfiguration(tile="auto"), ctx=ctx)
+        tune_s = time.perf_counter() - t0
+        t, far, _, _, _ = time_steps(a, steps, warmup)
+        rows.append({"workload": name, "tile": "auto", "chosen": list(a.tile), "tune_s": round(tune_s, 3),
+                     "candidate_ms": {f"{r}x{c}": round(v, 3) for (r, c), v in a.tile_times.items()},
+                     "fp64_ms": round(t * 1e3, 3), "fp64_far_ms": round(far, 3),
+                     "fp64_rel_fro_vs_default": a.matrix.compare(ref.matrix)["rel_fro"]})
+        a.close()
         ref.close()
     return rows
 
@@ -399,7 +409,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--tile-sweep", action="store_true",
-                    help="only time the far-pair tile shapes on efie-L3/L4/L5 and hh3d-cube24 and exit")
+                    help="only time the far-pair tile shapes and tile=\"auto\" on efie-L3/L4/L5 and hh3d-cube24, then exit")
     args = ap.parse_args()
     if args.tile_sweep:
         print(json.dumps({"tile_sweep": tile_sweep(args.steps, args.warmup),

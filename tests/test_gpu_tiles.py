@@ -1,11 +1,12 @@
 """Far-pair tile shapes other than the default (bmi_space_create_shaped):
 panels of fewer rows and patches of fewer columns assemble the oracle's
 matrix, alone and in a batch, and shapes outside the kernels' tile are
-refused."""
+refused; the tile autotuner (tile="auto") picks the fastest of its
+candidate shapes."""
 import numpy as np
 import pytest
 
-from beastgpu_amd import Helmholtz3D, Maxwell3D, _bmi, meshes, spaces
+from beastgpu_amd import Helmholtz3D, Maxwell3D, _bmi, api, meshes, spaces
 from beastgpu_amd.api import Assembler, Batch, DeviceMesh, DeviceSpace, GPUConfiguration
 from oracle import oracle as O
 
@@ -101,3 +102,39 @@ def test_batch_of_shaped_spaces_bitwise(gpu_ctx):
     b.close()
     for a in asms + others:
         a.close()
+
+
+@pytest.mark.parametrize("kind", ["efie", "hh3d"])
+def test_auto_tile_chooses_fastest_candidate_and_matches_oracle(gpu_ctx, kind):
+    """tile="auto" times every candidate shape on the problem, keeps the
+    fastest, assembles the oracle's matrix with it, and a second problem of
+    the same signature reuses the choice without timing again."""
+    api._TILE_CACHE.clear()
+    op, X = _case(kind, seed=7)
+    print(f"case auto tile {kind} nfun={X.nfun}", flush=True)
+    a = Assembler(op, X, X, GPUConfiguration(tile="auto"), ctx=gpu_ctx)
+    print("tile", a.tile, "times", a.tile_times, flush=True)
+    cands = api.TILE_CANDIDATES[X.kind]
+    assert a.tile in cands and set(a.tile_times) == set(cands)
+    assert a.tile_times[a.tile] == min(a.tile_times.values())
+    assert all(0 < t < 1e4 for t in a.tile_times.values())
+    Z = a.assemble().to_host()
+    st = a.stats()
+    a.close()
+    Zref, counts = O.assemble(op, X, X)
+    assert [st["pairs"][n] for n in CLASSES] == list(counts)
+    assert relfro(Z, Zref) <= FP64_TOL
+    op2, X2 = _case(kind, seed=8)
+    b = Assembler(op2, X2, X2, GPUConfiguration(tile="auto"), ctx=gpu_ctx)
+    assert b.tile == a.tile and b.tile_times is a.tile_times and len(api._TILE_CACHE) == 1
+    Z2 = b.assemble().to_host()
+    b.close()
+    Zref2, _ = O.assemble(op2, X2, X2)
+    assert relfro(Z2, Zref2) <= FP64_TOL
+
+
+def test_auto_tile_without_resident_matrix_takes_default(gpu_ctx):
+    op, X = _case("efie", seed=9)
+    a = Assembler(op, X, X, GPUConfiguration(tile="auto"), ctx=gpu_ctx, resident=False)
+    assert a.tile == (0, 0) and a.tile_times is None and a.matrix is None
+    a.close()

@@ -231,3 +231,35 @@ def test_request_bytes_bound_covers_special_pairs(mixed):
     bad = np.full(3 * m.nt, m.nv + 5, dtype=np.int32)
     r.tri = bad.ctypes.data
     assert _bmi.lib().bmi_request_bytes(ctypes.byref(r), ctypes.byref(mb), ctypes.byref(pb)) == _bmi.EINVAL
+
+
+def test_tile_autotuner_signature_and_refusal():
+    """The tile autotuner's cache key separates what changes the fastest
+    shape (operator, precision, space kind, size class) and no more; a tile
+    that is neither a shape nor "auto" is refused before any device call."""
+    from types import SimpleNamespace
+
+    from beastgpu_amd import Helmholtz3D, Maxwell3D
+    from beastgpu_amd.api import TILE_CANDIDATES, Assembler, GPUConfiguration, tile_signature
+
+    ctx = SimpleNamespace(device=0)
+    m2a = meshes.perturb_radial(meshes.icosphere(2), 1, 0.03)
+    m2b = meshes.perturb_radial(meshes.icosphere(2), 2, 0.03)
+    m3 = meshes.icosphere(3)
+    Xa, Xb, X3 = spaces.raviartthomas(m2a), spaces.raviartthomas(m2b), spaces.raviartthomas(m3)
+    efie, mfie = Maxwell3D.singlelayer(wavenumber=2.0), Maxwell3D.doublelayer(wavenumber=2.0)
+    cfg = GPUConfiguration(tile="auto")
+    sig = tile_signature(ctx, efie, Xa, Xa, cfg)
+    assert sig == tile_signature(ctx, Maxwell3D.singlelayer(wavenumber=3.5), Xb, Xb, cfg)
+    assert sig != tile_signature(ctx, mfie, Xa, Xa, cfg)
+    assert sig != tile_signature(ctx, efie, X3, X3, cfg)
+    assert sig != tile_signature(ctx, efie, Xa, Xa, GPUConfiguration(tile="auto", precision="mixed"))
+    assert sig != tile_signature(ctx, efie, Xa, Xb, cfg)
+    P = spaces.lagrangec0d1(m2a)
+    assert sig != tile_signature(ctx, Helmholtz3D.singlelayer(wavenumber=2.0), P, P, cfg)
+    for kind, shapes in TILE_CANDIDATES.items():
+        assert (0, 0) in shapes and len(set(shapes)) == len(shapes)
+        lim = (64, 40) if kind == spaces.RWG else (128, 20)
+        assert all(0 <= r <= lim[0] and (c == 0 or 3 <= c <= lim[1]) for r, c in shapes)
+    with pytest.raises(ValueError):
+        Assembler(efie, Xa, Xa, GPUConfiguration(tile="fastest"))
