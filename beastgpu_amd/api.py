@@ -280,13 +280,17 @@ def tile_signature(ctx: Context, op: Operator, X: Space, Y: Space, config: GPUCo
 
 
 def tune_tile(ctx: Context, op: Operator, X: Space, Y: Space, config: GPUConfiguration,
-              matrix: "DeviceMatrix", dmx: DeviceMesh, dmy: DeviceMesh) -> tuple:
+              matrix: "DeviceMatrix", dmx: DeviceMesh, dmy: DeviceMesh, margin: float = 0.03,
+              budget_ms: float = 4.0) -> tuple:
     """The far-pair tile shape with the shortest assembly among
     TILE_CANDIDATES[X.kind], and every candidate's time (ms): each candidate
     builds its spaces and plan on the problem itself and assembles into
-    ``matrix`` once to warm up, then twice timed by the library's events
-    (the shorter kept).  Cached per tile_signature for the process, so
-    problems of one signature are timed once.  No counterpart in the
+    ``matrix`` once to warm up, then is timed by the library's events over
+    enough assemblies to fill ``budget_ms`` (2 to 20; the shortest kept).
+    Another shape replaces the default (0, 0) only when it is faster by
+    more than ``margin`` (a fraction), so timing noise on small problems
+    does not flip the choice.  Cached per tile_signature for the process,
+    so problems of one signature are timed once.  No counterpart in the
     reference, whose analogous knobs are swept by hand
     (testing/determin_amount_of_threads.jl:44-75)."""
     key = tile_signature(ctx, op, X, Y, config)
@@ -305,16 +309,19 @@ def tune_tile(ctx: Context, op: Operator, X: Space, Y: Space, config: GPUConfigu
                 made.append(dY)
             plan = Plan(ctx, op, dX, dY, config)
             made.append(plan)
+            _bmi.check(_bmi.lib().bmi_assemble(plan.h, matrix.h))
+            first = plan.stats()["total_ms"]
             best = float("inf")
-            for rep in range(3):
+            for _ in range(min(20, max(2, int(budget_ms / max(first, 1e-3))))):
                 _bmi.check(_bmi.lib().bmi_assemble(plan.h, matrix.h))
-                if rep:
-                    best = min(best, plan.stats()["total_ms"])
+                best = min(best, plan.stats()["total_ms"])
             times[tile] = best
         finally:
             for o in reversed(made):
                 o.close()
     choice = min(times, key=times.get)
+    if times[choice] >= times[(0, 0)] * (1.0 - margin):
+        choice = (0, 0)
     _TILE_CACHE[key] = (choice, times)
     return choice, times
 

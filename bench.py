@@ -260,7 +260,8 @@ def serving_run(n_req: int, rate: float = 20.0, seed: int = 5, slots: int = 4, p
     }
 
 
-def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4, errors: bool = True) -> dict:
+def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4, errors: bool = True,
+                tile=(0, 0)) -> dict:
     """64 independent EFIE and MFIE (n x K) assemblies, mixed precision,
     ComplexF64 output: perturbed icospheres of level 2..4 (amplitude
     U[0, 5%]), k ~ U[0.5, 8] capped so that the longest edge is at most a
@@ -268,14 +269,22 @@ def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4, errors
     plans assembled one after another; checks every batched matrix bitwise
     against its own single assembly and reports each problem's error against
     an FP64 assembly (relative Frobenius and relative Z x, x uniform in the
-    complex unit disk, seed 33)."""
+    complex unit disk, seed 33).  ``tile``: the far-pair tile shape of every
+    problem, "auto" = the tile autotuner per problem signature (its timing
+    is part of assemblers_setup_s, its choices are reported per level and
+    operator in schedule.tiles)."""
     from beastgpu_amd.api import Batch, DeviceMatrix
 
     from beastgpu_amd import _bmi as _b
 
     probs = workloads.batched_efie_mfie(n_prob, seed)
     ctx = Context(0)
-    asms = [Assembler(p.op, p.X, p.X, GPUConfiguration(precision="mixed"), ctx=ctx) for p in probs]
+    t0 = time.perf_counter()
+    asms = [Assembler(p.op, p.X, p.X, GPUConfiguration(precision="mixed", tile=tile), ctx=ctx) for p in probs]
+    t_setup = time.perf_counter() - t0
+    tiles = {}
+    for p, a in zip(probs, asms):
+        tiles.setdefault(f"L{p.level} {p.kind}", set()).add(a.tile)
     entries = sum(a.matrix.rows * a.matrix.cols for a in asms)
     for _ in range(warmup):
         for a in asms:
@@ -329,9 +338,10 @@ def batched_run(steps: int, warmup: int, n_prob: int = 64, seed: int = 4, errors
         "speedup_vs_sequential": t_seq / t_b,
         "entries_per_s": entries / t_b,
         "create_and_tune_s": t_create,
+        "assemblers_setup_s": t_setup,
         "bitwise_equal_to_single_assemblies": bool(same),
         "checked_problems": n_prob,
-        "schedule": st,
+        "schedule": dict(st, tile=str(tile), tiles={k: sorted(map(list, v)) for k, v in sorted(tiles.items())}),
         "levels": {f"L{L}": sum(1 for p in probs if p.level == L) for L in (2, 3, 4)},
     }
     if per:

@@ -107,8 +107,9 @@ def test_batch_of_shaped_spaces_bitwise(gpu_ctx):
 @pytest.mark.parametrize("kind", ["efie", "hh3d"])
 def test_auto_tile_chooses_fastest_candidate_and_matches_oracle(gpu_ctx, kind):
     """tile="auto" times every candidate shape on the problem, keeps the
-    fastest, assembles the oracle's matrix with it, and a second problem of
-    the same signature reuses the choice without timing again."""
+    fastest (the default unless beaten by more than 3 %), assembles the
+    oracle's matrix with it, and a second problem of the same signature
+    reuses the choice without timing again."""
     api._TILE_CACHE.clear()
     op, X = _case(kind, seed=7)
     print(f"case auto tile {kind} nfun={X.nfun}", flush=True)
@@ -116,7 +117,9 @@ def test_auto_tile_chooses_fastest_candidate_and_matches_oracle(gpu_ctx, kind):
     print("tile", a.tile, "times", a.tile_times, flush=True)
     cands = api.TILE_CANDIDATES[X.kind]
     assert a.tile in cands and set(a.tile_times) == set(cands)
-    assert a.tile_times[a.tile] == min(a.tile_times.values())
+    # the fastest candidate, or the default when none beats it by 3 %
+    best = min(a.tile_times, key=a.tile_times.get)
+    assert a.tile == (best if a.tile_times[best] < 0.97 * a.tile_times[(0, 0)] else (0, 0))
     assert all(0 < t < 1e4 for t in a.tile_times.values())
     Z = a.assemble().to_host()
     st = a.stats()
