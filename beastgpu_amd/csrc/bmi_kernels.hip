@@ -2594,9 +2594,17 @@ struct PanelShared {
 // Tile entry (row r, column c) of a panel block: column-major, 64 rows per
 // column, the row index XOR-ed with c mod 16, so that the 16 lanes of a rank
 // round (one row, up to 16 columns) fall into different LDS banks while each
-// column stays one contiguous 1 KB segment (one wave's direct-to-LDS load).
+// column stays one contiguous segment of 64 entries.
 template <int PR>
 __device__ __forceinline__ int tile_ix(int r, int c) { return c * PR + (r ^ (c & 15)); }
+// Tile entry type: FP64 for the FP64 path.  The mixed path keeps its tile in
+// FP32: a tile entry sums the cell pairs of one (test, trial) function pair
+// within one patch (at most 4 on RWG, the two vertex degrees' product on
+// P1), each added by an FP64 multiply-add rounded once to FP32, which adds
+// an error of the order of the FP32 evaluation's own; the 20 KB tile (not
+// 40 KB) lets four work-groups share a CU.  Sums across patches are in the
+// output type.
+template <int MIXED> using tile_t = typename std::conditional<MIXED != 0, float2, double2>::type;
 
 // Rows of a test panel and the widest trial patch of the operator's spaces:
 // the Helmholtz operator works on Lagrange spaces (wide panels, narrow
@@ -2623,7 +2631,7 @@ __host__ __device__ constexpr int panel_cols(int op) { return op == OPK_HH3D ? B
 // (the single-problem kernels; measured faster there, slower in the batched
 // ones).
 template <int OP, int MIXED, int FAST, typename OutT, bool REFETCH = false>
-__device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, double2 *tile, PanelShared &sh) {
+__device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, tile_t<MIXED> *tile, PanelShared &sh) {
     auto &sg = sh.sg;
     auto &bg = sh.bg;
     auto &s_tslot = sh.s_tslot;
@@ -2647,7 +2655,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
     const int lane = t & 63, wv = t >> 6;
     for (int c = wv; c < nc; c += BMI_BLOCK_THREADS / 64)
 #pragma unroll
-        for (int h = 0; h < PR / 64; ++h) tile[c * PR + 64 * h + lane] = make_double2(0.0, 0.0);
+        for (int h = 0; h < PR / 64; ++h) tile[c * PR + 64 * h + lane] = tile_t<MIXED>{};
     short *s_ent = sh.s_ent;
     s_ent[t] = -1;
     // MIXED: the rule points and vertices of a geometry record are kept in
@@ -2805,7 +2813,7 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                         const bool bj = (bsl[j] & 0xff) != 0xff && ((bsl[j] >> 16) & 0xff) == rb;
                         int ix[3];
                         bool act[3];
-                        double2 z[3];
+                        tile_t<MIXED> z[3];
 #pragma unroll
                         for (int i = 0; i < 3; ++i) {
                             act[i] = tact[i] && bj;
@@ -2816,8 +2824,8 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                         for (int i = 0; i < 3; ++i) {
                             if (!act[i]) continue;
                             const double cc = tcf[i] * bcf[j];
-                            z[i].x = fma(cc, loc[3 * i + j].x, z[i].x);
-                            z[i].y = fma(cc, loc[3 * i + j].y, z[i].y);
+                            z[i].x = fma(cc, loc[3 * i + j].x, (double)z[i].x);
+                            z[i].y = fma(cc, loc[3 * i + j].y, (double)z[i].y);
                             tile[ix[i]] = z[i];
                         }
                     }
@@ -2832,9 +2840,9 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
                         for (int i = 0; i < 3; ++i) {
                             if (!tact[i]) continue;
                             const double cc = tcf[i] * bcf[j];
-                            double2 &z = tile[tile_ix<PR>(tsl[i] & 0xff, col)];
-                            z.x = fma(cc, loc[3 * i + j].x, z.x);
-                            z.y = fma(cc, loc[3 * i + j].y, z.y);
+                            auto &z = tile[tile_ix<PR>(tsl[i] & 0xff, col)];
+                            z.x = fma(cc, loc[3 * i + j].x, (double)z.x);
+                            z.y = fma(cc, loc[3 * i + j].y, (double)z.y);
                         }
                     }
                 }
@@ -2886,7 +2894,8 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
             for (int h = 0; h < NH; ++h) {
                 const int r = 64 * h + (lane ^ (c & 15));
                 if (r < nr) {
-                    double2 v = tile[c * PR + 64 * h + lane];
+                    const auto tv = tile[c * PR + 64 * h + lane];
+                    double2 v = make_double2(tv.x, tv.y);
                     if constexpr (FAST) v = make_double2(fma(ar, v.x, -ai * v.y), fma(ar, v.y, ai * v.x));
                     OutT o;
                     o.x = (decltype(o.x))((double)old[NH * k + h].x + v.x);
@@ -2899,24 +2908,28 @@ __device__ __forceinline__ void panel_block(const PanelArgs &a, int P, int q, do
 }
 
 // One work-group per (test panel P of the launch's range, trial patch q of its colour).
+// Work-groups per CU: four on the mixed path (128 VGPRs, 20 KB FP32 tile;
+// measured faster than three despite a few spilled registers), three on
+// the FP64 path (the 40 KB tile and 161 VGPRs), two for the FP64 double layer.
 template <int OP, int MIXED, int FAST, typename OutT>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_rr(PanelArgs a) {
-    extern __shared__ double2 tile[];             // panel_rows(OP) x a.ncm
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, MIXED ? 4 : OP == OPK_DL ? 2 : 3) void k_panels_rr(PanelArgs a) {
+    extern __shared__ double2 tile[];             // panel_rows(OP) x a.ncm entries of tile_t<MIXED>
     __shared__ PanelShared sh;
     if (!MIXED) load_sincos_table(sh.s_sc, a.sc);
     const int w = blockIdx.x;
     const int pl = w / a.nbq;  // panel of the launch's range
     const int P = a.panel0 + pl;
     const int q = a.bq_list[w - pl * a.nbq];
-    panel_block<OP, MIXED, FAST, OutT, true>(a, P, q, tile, sh);
+    panel_block<OP, MIXED, FAST, OutT, true>(a, P, q, (tile_t<MIXED> *)tile, sh);
 }
 
 // Batched, one colour at a time: work-group i takes item i = (problem, panel,
 // patch, -) of the launch's colour, across the problems of a batch.
+// Work-groups per CU as in k_panels_rr.
 template <int OP, int MIXED, int FAST, typename OutT>
-__global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3) void k_panels_items(
+__global__ __launch_bounds__(BMI_BLOCK_THREADS, MIXED ? 4 : OP == OPK_DL ? 2 : 3) void k_panels_items(
     const PanelArgs *args, const int4 *items) {
-    extern __shared__ double2 tile[];  // panel_rows(OP) x (the batch's widest patch)
+    extern __shared__ double2 tile[];  // panel_rows(OP) x (the batch's widest patch) entries of tile_t<MIXED>
     __shared__ PanelShared sh;
     const int4 it = items[blockIdx.x];
     if (!MIXED) load_sincos_table(sh.s_sc, args[it.x].sc);
@@ -2925,9 +2938,9 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
         // fields are loaded once instead of after each store (the FP64
         // kernels lack the registers for it)
         const PanelArgs a = args[it.x];
-        panel_block<OP, MIXED, FAST, OutT>(a, it.y, it.z, tile, sh);
+        panel_block<OP, MIXED, FAST, OutT>(a, it.y, it.z, (tile_t<MIXED> *)tile, sh);
     } else {
-        panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, tile, sh);
+        panel_block<OP, MIXED, FAST, OutT>(args[it.x], it.y, it.z, (tile_t<MIXED> *)tile, sh);
     }
 }
 
@@ -2958,7 +2971,7 @@ __global__ __launch_bounds__(BMI_BLOCK_THREADS, (OP == OPK_DL && !MIXED) ? 2 : 3
         auto run = [&](const PanelArgs &a) {
             for (int u = 0; u < a.npb; ++u) {
                 if (u > 0) __syncthreads();
-                panel_block<OP, MIXED, FAST, OutT>(a, it.y, a.bq_all[u], tile, sh);
+                panel_block<OP, MIXED, FAST, OutT>(a, it.y, a.bq_all[u], (tile_t<MIXED> *)tile, sh);
             }
         };
         if constexpr (MIXED) {
@@ -2977,7 +2990,7 @@ static void launch_panels_t(const PanelArgs &a, int c32, hipStream_t s) {
     if (a.rr || panel_rows(OP) != BMI_PANEL_ROWS) {
         // few rounds (RWG: at most 2; Lagrange by level: 2-4): rounds into an
         // LDS tile
-        const size_t lds = sizeof(double2) * panel_rows(OP) * a.ncm;
+        const size_t lds = sizeof(tile_t<MIXED>) * panel_rows(OP) * a.ncm;
         if (c32)
             hipLaunchKernelGGL((k_panels_rr<OP, MIXED, FAST, float2>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, a);
         else
@@ -3016,14 +3029,14 @@ hipError_t launch_panels(const PanelArgs &a, int op_kind, int mixed, int c32, hi
 namespace {
 template <int OP, int MIXED, int FAST, typename OutT>
 void items_t(const PanelArgs *args, const int4 *items, int nitems, int ncm, hipStream_t s) {
-    const size_t lds = sizeof(double2) * panel_rows(OP) * ncm;
+    const size_t lds = sizeof(tile_t<MIXED>) * panel_rows(OP) * ncm;
     hipLaunchKernelGGL((k_panels_items<OP, MIXED, FAST, OutT>), dim3(nitems), dim3(BMI_BLOCK_THREADS), lds, s, args,
                        items);
 }
 
 template <int OP, int MIXED, int FAST, typename OutT>
 void persist_t(const PanelArgs *args, const int2 *items, int nitems, int ncm, int *head, int grid, hipStream_t s) {
-    const size_t lds = sizeof(double2) * panel_rows(OP) * ncm;
+    const size_t lds = sizeof(tile_t<MIXED>) * panel_rows(OP) * ncm;
     hipLaunchKernelGGL((k_panels_persist<OP, MIXED, FAST, OutT>), dim3(grid), dim3(BMI_BLOCK_THREADS), lds, s, args,
                        items, nitems, head);
 }
