@@ -271,10 +271,13 @@ _TILE_CACHE: dict = {}
 
 def tile_signature(ctx: Context, op: Operator, X: Space, Y: Space, config: GPUConfiguration) -> tuple:
     """What the fastest tile shape depends on: device, operator kind and
-    lossiness, precision and output type, quadrature (orders or caller
-    tables), space kind, the test and trial meshes' size classes (powers of
-    two of their cell counts) and whether test and trial space are one."""
-    rule = ("tables", id(config.rules)) if config.rules is not None else (config.far_test_order, config.far_trial_order)
+    lossiness, precision and output type, the far rules (orders, or the
+    point counts of caller tables), space kind, the test and trial meshes'
+    size classes (powers of two of their cell counts) and whether test and
+    trial space are one."""
+    r = config.rules
+    rule = (("tables", r.test_weights.size, r.trial_weights.size) if r is not None
+            else (config.far_test_order, config.far_trial_order))
     return (ctx.device, op.kind, complex(op.gamma).real != 0.0, config.precision, config.output, rule, X.kind,
             int(X.mesh.nt).bit_length(), int(Y.mesh.nt).bit_length(), X is Y)
 

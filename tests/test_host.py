@@ -255,6 +255,15 @@ def test_tile_autotuner_signature_and_refusal():
     assert sig != tile_signature(ctx, efie, X3, X3, cfg)
     assert sig != tile_signature(ctx, efie, Xa, Xa, GPUConfiguration(tile="auto", precision="mixed"))
     assert sig != tile_signature(ctx, efie, Xa, Xb, cfg)
+    # caller tables: keyed by their point counts, not by object identity
+    from beastgpu_amd.api import QuadratureRules
+    gl = tuple((np.array([0.5]), np.array([1.0])) for _ in range(3))
+    def rules(nt, nb):
+        return QuadratureRules(np.full((nt, 2), 1 / 3), np.full(nt, 0.5 / nt), np.full((nb, 2), 1 / 3),
+                               np.full(nb, 0.5 / nb), gl)
+    r34 = tile_signature(ctx, efie, Xa, Xa, GPUConfiguration(tile="auto", rules=rules(3, 4)))
+    assert r34 == tile_signature(ctx, efie, Xa, Xa, GPUConfiguration(tile="auto", rules=rules(3, 4)))
+    assert r34 != tile_signature(ctx, efie, Xa, Xa, GPUConfiguration(tile="auto", rules=rules(4, 4)))
     P = spaces.lagrangec0d1(m2a)
     assert sig != tile_signature(ctx, Helmholtz3D.singlelayer(wavenumber=2.0), P, P, cfg)
     for kind, shapes in TILE_CANDIDATES.items():
